@@ -1,0 +1,187 @@
+/*
+ * ples_mg.h -- C ABI of the MI355X pressure-Poisson multigrid for PLES.
+ *
+ * Drop-in replacement for the pressure solve of the PLES projection step:
+ *     call multi(eps, ficycle, nlevel, jxc, jyc, jzc, ...)
+ * (fsrc/stratiParticle.f03:2881-2887, signature fsrc/multigrid.f03:42-48),
+ * together with the metric construction it depends on (metrica,
+ * fsrc/metrica.f03:2-501; mul_met, fsrc/multigrid.f03:2293-3052) and the
+ * projection-step kernels around it (diver, fsrc/diver.f03:2-46; gradie,
+ * fsrc/gradie.f03:2-120).  All entry points are plain C: pointers, sizes and
+ * status codes; no exceptions cross the boundary.
+ *
+ * Array conventions (host side, as the Fortran caller holds them, first
+ * index fastest, single MPI rank = whole domain):
+ *   fi   (0:jx+1, 0:jy+1, 1-deepl:jz+deepr)   pressure incl. ghosts (in/out)
+ *   rhs  (1:jx, 1:jy, 1:jz)
+ *   cs1,cs2 (jy,jz); cs3,cs4 (jx,jz); cs5,cs6 (jx,jy)   boundary fluxes
+ *   g11,g12,g13 (0:jx, 1:jy, 1:jz)   xi-face metrics   (myarrays_metri3)
+ *   g21,g22,g23 (1:jx, 0:jy, 1:jz)   eta-face metrics
+ *   g31,g32,g33 (1:jx, 1:jy, 0:jz)   zeta-face metrics
+ *   x,y,z (0:jx, 0:jy, 0:jz)          grid nodes (the interior part of the
+ *                                      reference's (-8:n1+8,...) arrays)
+ * Periodicity flags follow Aboundary.in: ip/jp/kp = 1 walls, 0 periodic.
+ * Walls in eta (jp = 1) are required, as in the reference
+ * (fsrc/multigrid.f03:1744-1746).
+ */
+#ifndef PLES_MG_H
+#define PLES_MG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference has none: fsrc/multigrid.f03:174) ---- */
+#define PLES_MG_OK 0
+#define PLES_MG_NOT_CONVERGED 1 /* iteration cap reached before the tolerance */
+#define PLES_MG_ERR_NONFINITE 2 /* NaN/Inf in the input or the iteration */
+#define PLES_MG_ERR_OOM 3       /* device allocation failed / over budget */
+#define PLES_MG_ERR_BAD_DIMS 4  /* dims not divisible by 2^(nlevel-1), too few cells */
+#define PLES_MG_ERR_HIP 5       /* HIP runtime error */
+#define PLES_MG_ERR_ARG 6       /* NULL or out-of-range argument */
+#define PLES_MG_ERR_UNSUPPORTED 7
+#define PLES_MG_ERR_BUSY 8      /* serving: request not admitted */
+
+/* ---- options ---- */
+#define PLES_MG_FP64 0  /* FP64 V-cycles */
+#define PLES_MG_MIXED 1 /* FP32 V-cycles inside FP64 refinement */
+
+#define PLES_MG_STOP_REL_L2 0  /* ||b - A p||_2 / ||b||_2 < eps */
+#define PLES_MG_STOP_ABS_MAX 1 /* max|b - A p| < eps  (the reference's test,
+                                  fsrc/multigrid.f03:598-631) */
+
+typedef struct ples_mg_grid {
+    int jx, jy, jz;        /* fine-level cells (reference n1, n2, n3) */
+    int ip, jp, kp;        /* 1 = walls, 0 = periodic */
+    int nlevel;            /* multigrid levels, >= 1 (the reference caps it at 4) */
+    const double* x;       /* nodes, required */
+    const double* y;
+    const double* z;
+    const double* g[9];    /* optional level-1 metrics g11..g33 in the reference
+                              shapes; all NULL -> built from the nodes on device */
+} ples_mg_grid;
+
+typedef struct ples_mg_opts {
+    int precision;   /* PLES_MG_FP64 or PLES_MG_MIXED */
+    int nu_pre;      /* smoothing sweeps per level on the way down (default 2) */
+    int nu_post;     /* ... on the way up (default 2) */
+    int nu_coarse;   /* sweeps on the coarsest level (default 24) */
+    double omega;    /* damping of the eta-line Jacobi smoother (default 0.8) */
+    int krylov;      /* 1 = GCR acceleration of the refinement loop (default), 0 = plain */
+    int stop_mode;   /* PLES_MG_STOP_REL_L2 (default) or PLES_MG_STOP_ABS_MAX */
+    int project_rhs; /* 1 = FP64 compatibility projection of the rhs (default) */
+    int halo_passes; /* red/black passes of the exact wall-ghost solve (default 6) */
+    int device;      /* HIP device ordinal (default 0) */
+} ples_mg_opts;
+
+typedef struct ples_mg_result {
+    int status;             /* PLES_MG_* */
+    int cycles;             /* V-cycles applied */
+    double rel_res;         /* ||b - A p||_2 / ||b||_2, FP64 */
+    double abs_res;         /* max |b - A p| */
+    double rhs_mean_removed;/* mean taken off the rhs by the projection */
+    double seconds;         /* device time of the solve */
+} ples_mg_result;
+
+typedef struct ples_mg_handle ples_mg_handle;
+
+void ples_mg_default_opts(ples_mg_opts* opts);
+const char* ples_mg_status_string(int status);
+
+/* Build the level hierarchy on the device: level-1 metrics (metrica) and
+ * coarse metrics (mul_met) from the nodes, smoother factors, work vectors.
+ * Replaces the init-time metrica -> mul_met -> wall sequence
+ * (fsrc/stratiParticle.f03:767-769). */
+int ples_mg_create(const ples_mg_grid* grid, const ples_mg_opts* opts, ples_mg_handle** out);
+int ples_mg_destroy(ples_mg_handle* h);
+
+/* Drop-in solve (replaces `call multi(...)`, fsrc/multigrid.f03:42-653).
+ * fi: in = warm start, out = solution incl. ghost layers.  fi has
+ * (jx+2)*(jy+2)*(jz+deepl+deepr) entries; only k = 0..jz+1 are touched.
+ * rhs as `rhs(1:jx,1:jy,1:jz)`.  cs1..cs6 may be NULL (zero fluxes).
+ * eps / ficycle as in the reference; the stop test is opts.stop_mode. */
+int ples_mg_solve(ples_mg_handle* h, double* fi, int deepl, int deepr, const double* rhs,
+                  const double* cs1, const double* cs2, const double* cs3, const double* cs4,
+                  const double* cs5, const double* cs6, double dt, double eps, int ficycle,
+                  ples_mg_result* res);
+
+/* Residual r = s^2 * sum(face fluxes) - rhs of the fine level
+ * (fsrc/resid_sndrcv.f03:80-151) after the exact ghost fill of fi with the
+ * boundary fluxes.  r as `r(1:jx,1:jy,1:jz)`.  fi is not modified. */
+int ples_mg_residual(ples_mg_handle* h, const double* fi, int deepl, int deepr,
+                     const double* rhs, const double* cs1, const double* cs2,
+                     const double* cs3, const double* cs4, const double* cs5,
+                     const double* cs6, double dt, double* r);
+
+/* Level-1 metrics as metrica builds them (fsrc/metrica.f03:85-483), in the
+ * reference shapes: g[9] (see above), face vectors csv[9] = csx,csy,csz
+ * (0:jx,1:jy,1:jz), etx,ety,etz (1:jx,0:jy,1:jz), ztx,zty,ztz
+ * (1:jx,1:jy,0:jz), giac (1:jx,1:jy,1:jz).  Any output may be NULL. */
+int ples_mg_metrica(ples_mg_handle* h, double* const g[9], double* const csv[9], double* giac);
+
+/* Metrics of multigrid level n (1-based, n >= 1) in the ghosted device layout
+ * copied to the host: each g[m] receives (jx/2^(n-1)+2)(jy/..+2)(jz/..+2)
+ * values, face index f stored at cell index f.  For parity tests. */
+int ples_mg_level_metrics(ples_mg_handle* h, int level, double* const g[9]);
+
+/* rhs = (div of contravariant fluxes)/dt (fsrc/diver.f03:20-28) with
+ * uc (0:jx,1:jy,1:jz), vc (1:jx,0:jy,1:jz), wc (1:jx,1:jy,0:jz). */
+int ples_mg_diver(ples_mg_handle* h, const double* uc, const double* vc, const double* wc,
+                  double dt, double* rhs);
+
+/* Pressure gradients of gradie (fsrc/gradie.f03:28-117) from fi (ghosted as
+ * in ples_mg_solve): gra1..3 (1:jx,1:jy,1:jz), cgra1 (0:jx,1:jy,1:jz),
+ * cgra2 (1:jx,0:jy,1:jz), cgra3 (1:jx,1:jy,0:jz).  Outputs may be NULL. */
+int ples_mg_gradie(ples_mg_handle* h, const double* fi, int deepl, int deepr, double dt,
+                   double* gra1, double* gra2, double* gra3, double* cgra1, double* cgra2,
+                   double* cgra3);
+
+/* ---- device-resident use (inputs stay in HBM between calls) ---- */
+
+/* Synthetic divergence rhs on the device: N(0,1) cell velocity from a
+ * counter-based generator (seed), contra-style face fluxes with zero wall
+ * fluxes, diver, zero-mean projection.  Also resets nothing else. */
+int ples_mg_case_rhs(ples_mg_handle* h, uint64_t seed, double dt);
+/* Resident iterate: set to zero / perturb by rel*std(p)*N(0,1) (seed). */
+int ples_mg_zero_fi(ples_mg_handle* h);
+int ples_mg_perturb_fi(ples_mg_handle* h, uint64_t seed, double rel);
+/* Solve with the resident rhs and resident iterate as warm start. */
+int ples_mg_solve_resident(ples_mg_handle* h, double eps, int maxcycles, ples_mg_result* res);
+/* Copy the resident iterate / rhs to or from host arrays (layouts as in
+ * ples_mg_solve, deepl = deepr = 1 for fi). */
+int ples_mg_get_fi(ples_mg_handle* h, double* fi);
+int ples_mg_set_fi(ples_mg_handle* h, const double* fi);
+int ples_mg_get_rhs(ples_mg_handle* h, double* rhs);
+int ples_mg_set_rhs(ples_mg_handle* h, const double* rhs);
+
+/* HIP stream the handle launches on (hipStream_t as void*). */
+void* ples_mg_stream(ples_mg_handle* h);
+/* Device bytes held by the handle. */
+size_t ples_mg_device_bytes(const ples_mg_handle* h);
+/* Device bytes ples_mg_create would allocate for this grid and options. */
+size_t ples_mg_estimate_bytes(const ples_mg_grid* grid, const ples_mg_opts* opts);
+
+/* ---- per-kernel timing (HIP events on the handle's stream) ---- */
+#define PLES_MG_K_SMOOTH 0   /* eta-line smoother, all levels */
+#define PLES_MG_K_SMOOTH_FINE 1 /* eta-line smoother, finest level only */
+#define PLES_MG_K_RESID 2    /* V-cycle residual, all levels */
+#define PLES_MG_K_RESID_FINE 3
+#define PLES_MG_K_OUTER_APPLY 4 /* FP64 operator + dots of the refinement loop */
+#define PLES_MG_K_OUTER_UPDATE 5
+#define PLES_MG_K_TRANSFER 6 /* restriction + prolongation */
+#define PLES_MG_K_HALO 7
+#define PLES_MG_K_COUNT 8
+/* Enable (1) / disable (0) event timing of every launch; resets the stats. */
+int ples_mg_set_timing(ples_mg_handle* h, int on);
+/* Launch count, summed device milliseconds and algorithmic bytes per kernel
+ * family since timing was enabled. */
+int ples_mg_kernel_stats(ples_mg_handle* h, int family, long* launches, double* ms,
+                         double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

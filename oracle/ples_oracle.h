@@ -1,0 +1,156 @@
+/*
+ * ples_oracle.h -- CPU FP64 restatement of the PLES pressure-Poisson path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP library is compared
+ * against; it is never linked into, loaded by, or called from the product
+ * library (ples_amd/).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.
+ *
+ * Every function cites the reference routine it restates (paths relative to
+ * the PLES source tree).  The reference runs on MPI k-slabs; this
+ * restatement holds the whole domain in one array, so every MPI halo,
+ * periodic and edge-strip exchange of the reference becomes a plain copy
+ * between planes of that array ("global" semantics: what the reference
+ * computes with >= 2 ranks, where both periodic partners exist).
+ *
+ * Layout conventions (shared with the HIP library):
+ *   - cell arrays are "ghosted": index (i,j,k) with i in 0..nx+1,
+ *     j in 0..ny+1, k in 0..nz+1, i fastest:
+ *         GIDX(i,j,k) = i + (nx+2)*(j + (ny+2)*k)
+ *     rhs / residual arrays use the same layout (only the interior is read).
+ *   - face metrics use the same ghosted layout: the x-face i (0..nx) of cell
+ *     row (j,k) is stored at GIDX(i,j,k); y-face j at GIDX(i,j,k);
+ *     z-face k at GIDX(i,j,k).  Order of the 9 metric pointers:
+ *         g11 g12 g13 | g21 g22 g23 | g31 g32 g33
+ *   - node coordinates are (nx+1)(ny+1)(nz+1), i fastest:
+ *         NIDX(i,j,k) = i + (nx+1)*(j + (ny+1)*k)
+ *   - boundary flux arrays follow the reference shapes
+ *     (fsrc/iniz.f03:63-68): cs1,cs2 (ny,nz); cs3,cs4 (nx,nz); cs5,cs6 (nx,ny),
+ *     first index fastest.
+ *   - periodic flags follow the reference (Aboundary.in): ip/jp/kp = 1 means
+ *     walls (non-periodic), 0 means periodic.
+ */
+#ifndef PLES_ORACLE_H
+#define PLES_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_dims {
+    int nx, ny, nz; /* cells on this level */
+    int ip, jp, kp; /* 1 = wall, 0 = periodic */
+} orc_dims;
+
+/* ---- case generation (test inputs) ---------------------------------- */
+
+/* Node coordinates of the channel / terrain-warped grids:
+ *   x_i = i*Lx/nx, z_k = k*Lz/nz, s(eta) = 0.5*(1 + tanh(gamma*(2 eta-1))/tanh gamma)
+ *   y = zb(x,z) + (Ly - zb(x,z)) * s(eta_j),
+ *   zb = amp * sin(2 pi x/(Lx/4)) * cos(2 pi z/(Lz/4))   (amp = 0 -> channel)
+ * gamma <= 0 gives a uniform y spacing. */
+void orc_make_nodes(int nx, int ny, int nz, double Lx, double Ly, double Lz,
+                    double gamma, double amp, double* x, double* y, double* z);
+
+/* Counter-based N(0,1) sample (SplitMix64 -> Box-Muller, cosine branch). */
+double orc_gauss(uint64_t seed, uint32_t field, int64_t ctr);
+
+/* Cell-centred velocity u,v,w (ghosted layout) ~ N(0,1) per cell, periodic
+ * ghost layers copied from the opposite interior layer. */
+void orc_make_velocity(const orc_dims* d, uint64_t seed, double* u, double* v, double* w);
+
+/* ---- geometry: metrica / mul_met / gprima family -------------------- */
+
+/* Node coordinate with the periodic extension of cellep
+ * (fsrc/cellep.f03:29-98): out-of-range indices in periodic directions
+ * wrap and shift the coordinate along that direction by the period. */
+void orc_node(const orc_dims* fine, const double* x, const double* y, const double* z,
+              int i, int j, int k, double out[3]);
+
+/* Face metrics J^-1 g^{ij} of the level with node stride st (1 = finest),
+ * divided by pot = st, as metrica (fsrc/metrica.f03:85-419, st = 1) and
+ * mul_met (fsrc/multigrid.f03:2520-2988, st = 2^(n-1)).  `fine` holds the
+ * finest-level dims; the level dims are fine/st.  g[9] are outputs in the
+ * ghosted layout of the level.  If csv != NULL it receives the 9 face area
+ * vectors csx,csy,csz | etx,ety,etz | ztx,zty,ztz (st must be 1); if
+ * giac != NULL it receives the centroid Jacobian (fsrc/metrica.f03:432-483). */
+void orc_metrics(const orc_dims* fine, const double* x, const double* y, const double* z,
+                 int st, double* const g[9], double* const csv[9], double* giac);
+
+/* ---- multigrid operators ------------------------------------------- */
+
+/* Ghost fill of mul_boun_sndrcv (fsrc/multigrid.f03:1109-1933): npass
+ * passes (the reference uses 5) of face ghosts (walls: face flux = an*cs/dt,
+ * periodic: copies) followed by the edge rules.  cs may be NULL (all zero)
+ * and is only read when an != 0. */
+void orc_ghost_fill(const orc_dims* d, double* p, const double* const g[9],
+                    const double* const cs[6], double an, double dt, int npass);
+
+/* One lexicographic in-place SOR sweep (fsrc/multigrid.f03:784-919) incl.
+ * the per-row periodic ghost refresh; pot = 2^(n-1). */
+void orc_sor_sweep(const orc_dims* d, double* p, const double* rh,
+                   const double* const g[9], double pot, double omega);
+
+/* Residual r = s^2 * sum(face flux differences) - rh, s = 1/pot
+ * (fsrc/resid_sndrcv.f03:80-151).  r in ghosted layout, interior written. */
+void orc_resid(const orc_dims* d, const double* p, const double* rh,
+               const double* const g[9], double pot, double* r);
+
+/* Restriction rc = -(1/8) * sum of the 8 children (fsrc/restrict_sndrcv.f03:32-53). */
+void orc_restrict(const orc_dims* dc, const double* rf, double* rc);
+
+/* Trilinear cell-centred prolongation-and-add over the ghost-inclusive
+ * range (fsrc/prolong.f03:64-261). */
+void orc_prolong(const orc_dims* dc, const double* ec, double* pf);
+
+/* The reference V-cycle driver multi (fsrc/multigrid.f03:163-637) with L
+ * levels (the reference caps L at 4).  fi (ghosted, in/out) is the warm
+ * start and result; rhs ghosted; cs may be NULL.  g1 may be NULL, then the
+ * level-1 metrics are computed from the nodes.  kss1/kssc are the sweeps
+ * per visit (reference: 5 / 10), nbc the ghost passes per sweep (5).
+ * Stops when max|r| < eps or after ficycle cycles.  res_hist (may be NULL)
+ * receives max|r| after each cycle (up to ficycle entries).
+ * Returns the number of cycles. */
+int orc_multi(const orc_dims* d, int nlevel, const double* x, const double* y,
+              const double* z, const double* const g1[9], double* fi,
+              const double* rhs, const double* const cs[6], double dt, double omega,
+              double eps, int ficycle, int kss1, int kssc, int nbc,
+              double* resmax_out, double* res_hist);
+
+/* ---- projection step ------------------------------------------------ */
+
+/* Contravariant face fluxes of contra (fsrc/contra.f03:73-87,132-146,204-220)
+ * from cell velocities, with wall-face fluxes set to zero.  uc/vc/wc are in
+ * the ghosted face layout (x-face i at GIDX(i,j,k), ...). */
+void orc_contra(const orc_dims* d, const double* const csv[9], const double* u,
+                const double* v, const double* w, double* uc, double* vc, double* wc);
+
+/* rhs = (div of face fluxes)/dt (fsrc/diver.f03:20-28). */
+void orc_diver(const orc_dims* d, const double* uc, const double* vc, const double* wc,
+               double dt, double* rhs);
+
+/* Compatibility projection: rhs -= (sum(rhs) - boundary flux sum/dt)/N.
+ * Returns the removed mean.  (The reference only prints the integral,
+ * fsrc/stratiParticle.f03:2847-2866.) */
+double orc_project(const orc_dims* d, double* rhs, const double* const cs[6], double dt);
+
+/* Centroid and contravariant pressure gradients of gradie
+ * (fsrc/gradie.f03:28-117).  gra1..3 ghosted cell layout (interior);
+ * cgra1..3 ghosted face layout (x-faces i=0..nx, y-faces j=0..ny,
+ * z-faces k=0..nz). */
+void orc_gradie(const orc_dims* d, const double* fi, const double* const csv[9],
+                const double* giac, const double* const g[9], double dt, double* gra1,
+                double* gra2, double* gra3, double* cgra1, double* cgra2, double* cgra3);
+
+/* ---- norms ------------------------------------------------------------ */
+double orc_absmax(const orc_dims* d, const double* r);
+double orc_l2(const orc_dims* d, const double* r);
+double orc_mean(const orc_dims* d, const double* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1134 @@
+// mg_kernels.hpp -- gfx950 kernels of the PLES pressure multigrid.
+//
+// Layout: every level stores cell arrays "ghosted", (nx+2)(ny+2)(nz+2),
+// i fastest; face metrics use the same layout (face f of a cell row stored
+// at cell index f).  One thread owns one eta-column (i,k) and streams up the
+// column in j: the 19-point stencil then needs only nine column loads per
+// step (rolling windows in registers), all loads of a wavefront are
+// contiguous rows in i, and the eta-line solve of the smoother runs in the
+// same thread without any cross-thread traffic.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace plesmg {
+
+struct Geo {
+    int nx, ny, nz;
+    int ip, jp, kp;  // 1 = walls, 0 = periodic
+    long sy, sz;     // strides of j and k
+    long n;          // total ghosted cells
+    __host__ __device__ long idx(int i, int j, int k) const {
+        return (long)i + sy * (long)j + sz * (long)k;
+    }
+};
+
+template <class TM>
+struct Met {
+    const TM* g[9];  // g11 g12 g13 | g21 g22 g23 | g31 g32 g33
+};
+
+// ------------------------------------------------------------------------
+// block reductions (wavefront 64, deterministic tree order)
+// ------------------------------------------------------------------------
+template <int NV>
+__device__ inline void block_reduce_store(double (&v)[NV], double* partials, int nblocks_stride,
+                                          const bool (&is_max)[NV]) {
+    __shared__ double red[NV][16];
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    const int lane = tid & 63, wid = tid >> 6;
+    const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double a = v[q];
+        for (int off = 32; off > 0; off >>= 1) {
+            double b = __shfl_down(a, off, 64);
+            a = is_max[q] ? fmax(a, b) : a + b;
+        }
+        if (lane == 0) red[q][wid] = a;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const long bid = blockIdx.x + (long)gridDim.x * blockIdx.y;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            double a = red[q][0];
+            for (int w = 1; w < nw; ++w) a = is_max[q] ? fmax(a, red[q][w]) : a + red[q][w];
+            partials[(long)q * nblocks_stride + bid] = a;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// column-streaming 19-point stencil
+// ------------------------------------------------------------------------
+// Rolling state of the nine columns around (i,k) at rows j-1, j, j+1.
+template <class T>
+struct Win {
+    T pm, p0, pp;        // (i  ,k  )
+    T xmm, xm0, xmp;     // (i-1,k  )
+    T xpm, xp0, xpp;     // (i+1,k  )
+    T zmm, zm0, zmp;     // (i  ,k-1)
+    T zpm, zp0, zpp;     // (i  ,k+1)
+    T dmm, dpm, dmp, dpp;// (i-1,k-1) (i+1,k-1) (i-1,k+1) (i+1,k+1) at row j
+};
+
+template <class T, class P>
+__device__ inline void win_init(Win<T>& w, const P* __restrict__ p, long c0, long sy, long sz) {
+    // c0 = idx(i,0,k): rows 0 (j-1 of j=1) and 1 (j)
+    w.pm = (T)p[c0];
+    w.p0 = (T)p[c0 + sy];
+    w.xmm = (T)p[c0 - 1];
+    w.xm0 = (T)p[c0 - 1 + sy];
+    w.xpm = (T)p[c0 + 1];
+    w.xp0 = (T)p[c0 + 1 + sy];
+    w.zmm = (T)p[c0 - sz];
+    w.zm0 = (T)p[c0 - sz + sy];
+    w.zpm = (T)p[c0 + sz];
+    w.zp0 = (T)p[c0 + sz + sy];
+}
+
+// load row j+1 of the five face columns and row j of the four diagonals;
+// c = idx(i,j,k)
+template <class T, class P>
+__device__ inline void win_load(Win<T>& w, const P* __restrict__ p, long c, long sy, long sz) {
+    w.pp = (T)p[c + sy];
+    w.xmp = (T)p[c - 1 + sy];
+    w.xpp = (T)p[c + 1 + sy];
+    w.zmp = (T)p[c - sz + sy];
+    w.zpp = (T)p[c + sz + sy];
+    w.dmm = (T)p[c - 1 - sz];
+    w.dpm = (T)p[c + 1 - sz];
+    w.dmp = (T)p[c - 1 + sz];
+    w.dpp = (T)p[c + 1 + sz];
+}
+
+template <class T>
+__device__ inline void win_shift(Win<T>& w) {
+    w.pm = w.p0; w.p0 = w.pp;
+    w.xmm = w.xm0; w.xm0 = w.xmp;
+    w.xpm = w.xp0; w.xp0 = w.xpp;
+    w.zmm = w.zm0; w.zm0 = w.zmp;
+    w.zpm = w.zp0; w.zp0 = w.zpp;
+}
+
+// eta-face metrics of row j-1 carried between steps
+template <class T>
+struct G2 {
+    T g21, g22, g23;
+};
+
+// s * sum of face-flux differences at (i,j,k), the reference expression
+// of fsrc/resid_sndrcv.f03:85-147 (without the "- rh").
+template <class T, class TM>
+__device__ inline T stencil(const Win<T>& w, const Met<TM>& M, long c, long sz, const G2<T>& gm,
+                            const G2<T>& g0, T inv_pot) {
+    const T q = (T)0.25;
+    const T g11 = (T)M.g[0][c], g12 = (T)M.g[1][c], g13 = (T)M.g[2][c];
+    const T h11 = (T)M.g[0][c - 1], h12 = (T)M.g[1][c - 1], h13 = (T)M.g[2][c - 1];
+    const T g31 = (T)M.g[6][c], g32 = (T)M.g[7][c], g33 = (T)M.g[8][c];
+    const T h31 = (T)M.g[6][c - sz], h32 = (T)M.g[7][c - sz], h33 = (T)M.g[8][c - sz];
+    T res_dx = (g11 * (w.xp0 - w.p0) +
+                q * (g12 * (w.xpp + w.pp - w.xpm - w.pm) + g13 * (w.dpp + w.zp0 - w.dpm - w.zm0))) *
+               inv_pot;
+    T res_sn = (h11 * (w.p0 - w.xm0) +
+                q * (h12 * (w.pp + w.xmp - w.pm - w.xmm) + h13 * (w.zp0 + w.dmp - w.zm0 - w.dmm))) *
+               inv_pot;
+    T res_sop = (g0.g22 * (w.pp - w.p0) +
+                 q * (g0.g21 * (w.xpp + w.xp0 - w.xmp - w.xm0) +
+                      g0.g23 * (w.zpp + w.zp0 - w.zmp - w.zm0))) *
+                inv_pot;
+    T res_sot = (gm.g22 * (w.p0 - w.pm) +
+                 q * (gm.g21 * (w.xp0 + w.xpm - w.xm0 - w.xmm) +
+                      gm.g23 * (w.zp0 + w.zpm - w.zm0 - w.zmm))) *
+                inv_pot;
+    T res_av = (g33 * (w.zp0 - w.p0) +
+                q * (g31 * (w.dpp + w.xp0 - w.dmp - w.xm0) + g32 * (w.zpp + w.pp - w.zpm - w.pm))) *
+               inv_pot;
+    T res_ind = (h33 * (w.p0 - w.zm0) +
+                 q * (h31 * (w.xp0 + w.dpm - w.xm0 - w.dmm) + h32 * (w.pp + w.zmp - w.pm - w.zmm))) *
+                inv_pot;
+    return (res_dx - res_sn + res_sop - res_sot + res_av - res_ind) * inv_pot;
+}
+
+template <class T, class TM>
+__device__ inline G2<T> load_g2(const Met<TM>& M, long c) {
+    return G2<T>{(T)M.g[3][c], (T)M.g[4][c], (T)M.g[5][c]};
+}
+
+__device__ inline bool column_of(const Geo& G, int& i, int& k) {
+    i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    return i <= G.nx && k <= G.nz;
+}
+
+// write value v of row j of column (i,k) also to its periodic images
+template <class T>
+__device__ inline void write_images(const Geo& G, T* __restrict__ p, int i, int j, int k, T v) {
+    int ia[3], ka[3];
+    int ni = 0, nk = 0;
+    ia[ni++] = i;
+    ka[nk++] = k;
+    if (G.ip == 0) {
+        if (i == G.nx) ia[ni++] = 0;
+        if (i == 1) ia[ni++] = G.nx + 1;
+    }
+    if (G.kp == 0) {
+        if (k == G.nz) ka[nk++] = 0;
+        if (k == 1) ka[nk++] = G.nz + 1;
+    }
+    if (ni == 1 && nk == 1) return;
+    for (int a = 0; a < ni; ++a)
+        for (int b = 0; b < nk; ++b)
+            if (a != 0 || b != 0) p[G.idx(ia[a], j, ka[b])] = v;
+}
+
+// ------------------------------------------------------------------------
+// eta-line Jacobi smoother:  p_new = p + omega * M_line^{-1} (A p - b)
+// M_line = tridiag(-s^2 g22(j-1), den_j, -s^2 g22(j)) with den the sum of
+// the s^2-scaled normal metrics of the non-wall faces.  Factors invm = 1/m_j
+// and cpr = c'_j of its Thomas factorisation are precomputed per level.
+// Forward pass stores e_j = p_j + omega d'_j + c'_j p_{j+1}; backward pass
+// p_new_j = e_j - c'_j p_new_{j+1}.  Afterwards the column writes its eta-wall
+// ghosts (homogeneous flux condition, neighbours lagged) and its periodic
+// images.  zero_in: the input iterate is identically zero.
+// ------------------------------------------------------------------------
+template <class T, class TM, class TR>
+__global__ void __launch_bounds__(256) k_yline(Geo G, Met<TM> M, const float* __restrict__ invm,
+                                               const float* __restrict__ cpr,
+                                               const T* __restrict__ pin, T* __restrict__ pout,
+                                               const TR* __restrict__ b, T omega, T inv_pot,
+                                               int zero_in) {
+    int i, k;
+    if (!column_of(G, i, k)) return;
+    const long sy = G.sy, sz = G.sz;
+    const T s2 = inv_pot * inv_pot;
+    const long c0 = G.idx(i, 0, k);
+    T dprev = 0;
+    if (zero_in) {
+        T g22m = (T)M.g[4][c0];
+        for (int j = 1; j <= G.ny; ++j) {
+            const long c = c0 + sy * j;
+            const T r = -(T)b[c];
+            const T a = (j > 1) ? -s2 * g22m : (T)0;
+            const T d = (r - a * dprev) * (T)invm[c];
+            pout[c] = omega * d;
+            dprev = d;
+            g22m = (T)M.g[4][c];
+        }
+    } else {
+        Win<T> w;
+        win_init(w, pin, c0, sy, sz);
+        G2<T> gm = load_g2<T>(M, c0);
+        for (int j = 1; j <= G.ny; ++j) {
+            const long c = c0 + sy * j;
+            win_load(w, pin, c, sy, sz);
+            const G2<T> g0 = load_g2<T>(M, c);
+            const T r = stencil(w, M, c, sz, gm, g0, inv_pot) - (T)b[c];
+            const T a = (j > 1) ? -s2 * gm.g22 : (T)0;
+            const T d = (r - a * dprev) * (T)invm[c];
+            pout[c] = w.p0 + omega * d + (T)cpr[c] * w.pp;
+            dprev = d;
+            gm = g0;
+            win_shift(w);
+        }
+    }
+    // backward substitution
+    T pn = 0, pn1 = 0, pnn = 0;
+    for (int j = G.ny; j >= 1; --j) {
+        const long c = c0 + sy * j;
+        pn = pout[c] - (T)cpr[c] * pn;
+        pout[c] = pn;
+        write_images(G, pout, i, j, k, pn);
+        if (j == G.ny) pnn = pn;
+    }
+    pn1 = pn;
+    // eta-wall ghosts, fsrc/multigrid.f03:1166-1186 with an = 0; the
+    // neighbouring columns enter with their input (lagged) values.
+    {
+        const T q = (T)0.25;
+        T lo, hi;
+        if (zero_in) {
+            lo = pn1;
+            hi = pnn;
+        } else {
+            const long cb = c0;  // (i,0,k)
+            const T inv = (T)1 / (T)M.g[4][cb];
+            lo = pn1 + q * inv *
+                           ((T)M.g[3][cb] * ((T)pin[cb + 1 + sy] + (T)pin[cb + 1] - (T)pin[cb - 1 + sy] -
+                                             (T)pin[cb - 1]) +
+                            (T)M.g[5][cb] * ((T)pin[cb + sy + sz] + (T)pin[cb + sz] - (T)pin[cb + sy - sz] -
+                                             (T)pin[cb - sz]));
+            const long ct = c0 + sy * G.ny;  // (i,ny,k)
+            const T invt = (T)1 / (T)M.g[4][ct];
+            hi = pnn - q * invt *
+                           ((T)M.g[3][ct] * ((T)pin[ct + 1 + sy] + (T)pin[ct + 1] - (T)pin[ct - 1 + sy] -
+                                             (T)pin[ct - 1]) +
+                            (T)M.g[5][ct] * ((T)pin[ct + sy + sz] + (T)pin[ct + sz] - (T)pin[ct + sy - sz] -
+                                             (T)pin[ct - sz]));
+        }
+        pout[c0] = lo;
+        write_images(G, pout, i, 0, k, lo);
+        pout[c0 + sy * (G.ny + 1)] = hi;
+        write_images(G, pout, i, G.ny + 1, k, hi);
+    }
+}
+
+// ------------------------------------------------------------------------
+// Residual family.  MODE 0: r = A p - b           (V-cycle residual)
+//                   MODE 1: r = b - A p, partial sums of r.r, max|r|, b.b
+//                   MODE 2: w = A p, partial sums of w.w, w.wp, r.w, r.wp
+// ------------------------------------------------------------------------
+template <int MODE, class T, class TM, class TR, class TO>
+__global__ void __launch_bounds__(256) k_resid(Geo G, Met<TM> M, const T* __restrict__ p,
+                                               const TR* __restrict__ b, TO* __restrict__ out,
+                                               T inv_pot, const double* __restrict__ rvec,
+                                               const double* __restrict__ wp, double* partials,
+                                               int nblocks) {
+    int i, k;
+    const bool act = column_of(G, i, k);
+    double acc[4] = {0, 0, 0, 0};
+    if (act) {
+        const long sy = G.sy, sz = G.sz;
+        const long c0 = G.idx(i, 0, k);
+        Win<T> w;
+        win_init(w, p, c0, sy, sz);
+        G2<T> gm = load_g2<T>(M, c0);
+        for (int j = 1; j <= G.ny; ++j) {
+            const long c = c0 + sy * j;
+            win_load(w, p, c, sy, sz);
+            const G2<T> g0 = load_g2<T>(M, c);
+            const T ap = stencil(w, M, c, sz, gm, g0, inv_pot);
+            if (MODE == 0) {
+                out[c] = (TO)(ap - (T)b[c]);
+            } else if (MODE == 1) {
+                const double bb = (double)b[c];
+                const double r = bb - (double)ap;
+                out[c] = (TO)r;
+                acc[0] += r * r;
+                acc[1] = fmax(acc[1], fabs(r));
+                acc[2] += bb * bb;
+            } else {
+                const double wv = (double)ap;
+                out[c] = (TO)wv;
+                const double rv = rvec[c];
+                acc[0] += wv * wv;
+                if (wp) {
+                    const double wpv = wp[c];
+                    acc[1] += wv * wpv;
+                    acc[3] += rv * wpv;
+                }
+                acc[2] += rv * wv;
+            }
+            gm = g0;
+            win_shift(w);
+        }
+    }
+    if (MODE == 1) {
+        double v[3] = {acc[0], acc[1], acc[2]};
+        const bool mx[3] = {false, true, false};
+        block_reduce_store<3>(v, partials, nblocks, mx);
+    } else if (MODE == 2) {
+        double v[4] = {acc[0], acc[1], acc[2], acc[3]};
+        const bool mx[4] = {false, false, false, false};
+        block_reduce_store<4>(v, partials, nblocks, mx);
+    }
+}
+
+// ------------------------------------------------------------------------
+// Outer-loop scalar logic on the device (no host round trip per iteration).
+// S layout (doubles):
+//  0 rr   1 rmax  2 bb   3 ww   4 wwp  5 rw   6 rwp  7 wpwp  8 alpha
+//  9 beta 10 have_prev 11 rr0
+// ------------------------------------------------------------------------
+enum { S_RR = 0, S_RMAX, S_BB, S_WW, S_WWP, S_RW, S_RWP, S_WPWP, S_ALPHA, S_BETA, S_HAVEP, S_RR0,
+       S_SUM, S_COUNT = 16 };
+
+// finalize the partial sums written by k_resid MODE 1 into S[0..2]
+__global__ void k_fin_resid(const double* __restrict__ part, int nb, double* S) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double rr = 0, rm = 0, bb = 0;
+    for (int q = 0; q < nb; ++q) {
+        rr += part[q];
+        rm = fmax(rm, part[nb + q]);
+        bb += part[2 * nb + q];
+    }
+    S[S_RR] = rr;
+    S[S_RMAX] = rm;
+    S[S_BB] = bb;
+}
+
+// finalize k_resid MODE 2 sums and compute the GCR step (beta, alpha)
+__global__ void k_fin_apply(const double* __restrict__ part, int nb, double* S, int krylov) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double ww = 0, wwp = 0, rw = 0, rwp = 0;
+    for (int q = 0; q < nb; ++q) {
+        ww += part[q];
+        wwp += part[nb + q];
+        rw += part[2 * nb + q];
+        rwp += part[3 * nb + q];
+    }
+    double beta = 0.0;
+    if (krylov && S[S_HAVEP] != 0.0 && S[S_WPWP] > 0.0) beta = wwp / S[S_WPWP];
+    const double wwn = ww - 2.0 * beta * wwp + beta * beta * S[S_WPWP];
+    const double rwn = rw - beta * rwp;
+    double alpha = krylov ? ((wwn > 0.0) ? rwn / wwn : 0.0) : 1.0;
+    S[S_WW] = ww;
+    S[S_WWP] = wwp;
+    S[S_RW] = rw;
+    S[S_RWP] = rwp;
+    S[S_BETA] = beta;
+    S[S_ALPHA] = alpha;
+    S[S_WPWP] = wwn;
+    S[S_HAVEP] = 1.0;
+}
+
+// w <- w - beta wp ; z <- z - beta zp ; x += alpha z ; r -= alpha w ;
+// partial sums of r.r and max|r| (interior cells, one thread per column)
+__global__ void __launch_bounds__(256) k_gcr_update(Geo G, double* __restrict__ x, double* __restrict__ r,
+                                                    double* __restrict__ z, double* __restrict__ w,
+                                                    const double* __restrict__ zp,
+                                                    const double* __restrict__ wp, const double* S,
+                                                    double* partials, int nblocks) {
+    int i, k;
+    const bool act = column_of(G, i, k);
+    double acc[2] = {0, 0};
+    if (act) {
+        const double alpha = S[S_ALPHA], beta = S[S_BETA];
+        const long c0 = G.idx(i, 0, k);
+        for (int j = 1; j <= G.ny; ++j) {
+            const long c = c0 + G.sy * j;
+            double zv = z[c], wv = w[c];
+            if (beta != 0.0) {
+                zv -= beta * zp[c];
+                wv -= beta * wp[c];
+                z[c] = zv;
+                w[c] = wv;
+            }
+            x[c] += alpha * zv;
+            const double rv = r[c] - alpha * wv;
+            r[c] = rv;
+            acc[0] += rv * rv;
+            acc[1] = fmax(acc[1], fabs(rv));
+        }
+    }
+    double v[2] = {acc[0], acc[1]};
+    const bool mx[2] = {false, true};
+    block_reduce_store<2>(v, partials, nblocks, mx);
+}
+
+__global__ void k_fin_update(const double* __restrict__ part, int nb, double* S) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double rr = 0, rm = 0;
+    for (int q = 0; q < nb; ++q) {
+        rr += part[q];
+        rm = fmax(rm, part[nb + q]);
+    }
+    S[S_RR] = rr;
+    S[S_RMAX] = rm;
+}
+
+// ------------------------------------------------------------------------
+// Transfers
+// ------------------------------------------------------------------------
+// rc = -(1/8) * sum of the eight children (fsrc/restrict_sndrcv.f03:32-53)
+template <class TF, class TC>
+__global__ void __launch_bounds__(256) k_restrict(Geo Gc, Geo Gf, const TF* __restrict__ rf,
+                                                  TC* __restrict__ rc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > Gc.nx || j > Gc.ny) return;
+    const long f = Gf.idx(2 * i, 2 * j, 2 * k);
+    const long sy = Gf.sy, sz = Gf.sz;
+    TF s = rf[f] + rf[f - 1] + rf[f - sy] + rf[f - 1 - sy] + rf[f - sz] + rf[f - 1 - sz] +
+           rf[f - sy - sz] + rf[f - 1 - sy - sz];
+    rc[Gc.idx(i, j, k)] = (TC)(-s * (TF)0.125);
+}
+
+// pf += trilinear cell-centred interpolation of ec over the ghost-inclusive
+// fine range (fsrc/prolong.f03:64-261): weight 3 per axis where the coarse
+// offset matches the fine child parity, 1 otherwise, over 64.
+template <class TC, class TF>
+__global__ void __launch_bounds__(256) k_prolong(Geo Gc, Geo Gf, const TC* __restrict__ ec,
+                                                 TF* __restrict__ pf) {
+    const int fi = blockIdx.x * blockDim.x + threadIdx.x;
+    const int fj = blockIdx.y * blockDim.y + threadIdx.y;
+    const int fk = blockIdx.z;
+    if (fi > Gf.nx + 1 || fj > Gf.ny + 1) return;
+    const int i = fi >> 1, j = fj >> 1, k = fk >> 1;
+    const int di = fi & 1, dj = fj & 1, dk = fk & 1;
+    const long c = Gc.idx(i, j, k);
+    const long sy = Gc.sy, sz = Gc.sz;
+    const TF wi0 = di ? (TF)1 : (TF)3, wi1 = di ? (TF)3 : (TF)1;
+    const TF wj0 = dj ? (TF)1 : (TF)3, wj1 = dj ? (TF)3 : (TF)1;
+    const TF wk0 = dk ? (TF)1 : (TF)3, wk1 = dk ? (TF)3 : (TF)1;
+    const TF s = (wi0 * wj0 * wk0) * (TF)ec[c] + (wi1 * wj0 * wk0) * (TF)ec[c + 1] +
+                 (wi0 * wj1 * wk0) * (TF)ec[c + sy] + (wi0 * wj0 * wk1) * (TF)ec[c + sz] +
+                 (wi1 * wj1 * wk0) * (TF)ec[c + 1 + sy] + (wi1 * wj0 * wk1) * (TF)ec[c + 1 + sz] +
+                 (wi0 * wj1 * wk1) * (TF)ec[c + sy + sz] + (wi1 * wj1 * wk1) * (TF)ec[c + 1 + sy + sz];
+    const long f = Gf.idx(fi, fj, fk);
+    pf[f] = pf[f] + s * (TF)(1.0 / 64.0);
+}
+
+// ------------------------------------------------------------------------
+// Exact ghost fill (outer FP64 operator, walls in xi/zeta): face relations
+// of fsrc/multigrid.f03:1113-1317 solved by red/black passes over each wall
+// face, neighbours along periodic directions read through the wrap, then
+// every non-face ghost (periodic images, edges, corners) filled from its
+// source by the edge rules of fsrc/multigrid.f03:1358-1742.
+// ------------------------------------------------------------------------
+struct BCs {
+    const double* cs[6];  // cs1..cs6 (may be null)
+    double an_over_dt;    // an / dt
+};
+
+__device__ inline int wrapi(const Geo& G, int i) {
+    if (G.ip == 0) {
+        if (i == 0) return G.nx;
+        if (i == G.nx + 1) return 1;
+    }
+    return i;
+}
+__device__ inline int wrapk(const Geo& G, int k) {
+    if (G.kp == 0) {
+        if (k == 0) return G.nz;
+        if (k == G.nz + 1) return 1;
+    }
+    return k;
+}
+
+template <class T, class TM>
+__global__ void k_halo_face(Geo G, Met<TM> M, T* __restrict__ p, int color, BCs bc) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const double an = bc.an_over_dt;
+    const T q = (T)0.25;
+#define PV(a, b, c) ((T)p[G.idx(wrapi(G, a), b, wrapk(G, c))])
+    // eta faces (jp = 1 required)
+    {
+        const long nf = 2L * G.nx * G.nz;
+        if (t < nf) {
+            const int side = (int)(t / ((long)G.nx * G.nz));
+            const long rem = t % ((long)G.nx * G.nz);
+            const int i = (int)(rem % G.nx) + 1, k = (int)(rem / G.nx) + 1;
+            if (((i + k) & 1) != color) return;
+            if (side == 0) {
+                const long cb = G.idx(i, 0, k);
+                const T inv = (T)1 / (T)M.g[4][cb];
+                const T csv = bc.cs[2] ? (T)bc.cs[2][(i - 1) + (long)G.nx * (k - 1)] : (T)0;
+                p[cb] = PV(i, 1, k) - (T)an * csv * inv +
+                        q * inv *
+                            ((T)M.g[3][cb] * (PV(i + 1, 1, k) + PV(i + 1, 0, k) - PV(i - 1, 1, k) - PV(i - 1, 0, k)) +
+                             (T)M.g[5][cb] * (PV(i, 1, k + 1) + PV(i, 0, k + 1) - PV(i, 1, k - 1) - PV(i, 0, k - 1)));
+            } else {
+                const int ny = G.ny;
+                const long ct = G.idx(i, ny, k);
+                const T inv = (T)1 / (T)M.g[4][ct];
+                const T csv = bc.cs[3] ? (T)bc.cs[3][(i - 1) + (long)G.nx * (k - 1)] : (T)0;
+                p[ct + G.sy] = PV(i, ny, k) + (T)an * csv * inv -
+                               q * inv *
+                                   ((T)M.g[3][ct] * (PV(i + 1, ny + 1, k) + PV(i + 1, ny, k) - PV(i - 1, ny + 1, k) -
+                                                     PV(i - 1, ny, k)) +
+                                    (T)M.g[5][ct] * (PV(i, ny + 1, k + 1) + PV(i, ny, k + 1) - PV(i, ny + 1, k - 1) -
+                                                     PV(i, ny, k - 1)));
+            }
+            return;
+        }
+        t -= nf;
+    }
+    if (G.ip == 1) {
+        const long nf = 2L * G.ny * G.nz;
+        if (t < nf) {
+            const int side = (int)(t / ((long)G.ny * G.nz));
+            const long rem = t % ((long)G.ny * G.nz);
+            const int j = (int)(rem % G.ny) + 1, k = (int)(rem / G.ny) + 1;
+            if (((j + k) & 1) != color) return;
+            if (side == 0) {
+                const long cb = G.idx(0, j, k);
+                const T inv = (T)1 / (T)M.g[0][cb];
+                const T csv = bc.cs[0] ? (T)bc.cs[0][(j - 1) + (long)G.ny * (k - 1)] : (T)0;
+                p[cb] = PV(1, j, k) - (T)an * csv * inv +
+                        q * inv *
+                            ((T)M.g[1][cb] * (PV(1, j + 1, k) + PV(0, j + 1, k) - PV(1, j - 1, k) - PV(0, j - 1, k)) +
+                             (T)M.g[2][cb] * (PV(1, j, k + 1) + PV(0, j, k + 1) - PV(1, j, k - 1) - PV(0, j, k - 1)));
+            } else {
+                const int nx = G.nx;
+                const long ct = G.idx(nx, j, k);
+                const T inv = (T)1 / (T)M.g[0][ct];
+                const T csv = bc.cs[1] ? (T)bc.cs[1][(j - 1) + (long)G.ny * (k - 1)] : (T)0;
+                p[ct + 1] = PV(nx, j, k) + (T)an * csv * inv -
+                            q * inv *
+                                ((T)M.g[1][ct] * (PV(nx + 1, j + 1, k) + PV(nx, j + 1, k) - PV(nx + 1, j - 1, k) -
+                                                  PV(nx, j - 1, k)) +
+                                 (T)M.g[2][ct] * (PV(nx + 1, j, k + 1) + PV(nx, j, k + 1) - PV(nx + 1, j, k - 1) -
+                                                  PV(nx, j, k - 1)));
+            }
+            return;
+        }
+        t -= nf;
+    }
+    if (G.kp == 1) {
+        const long nf = 2L * G.nx * G.ny;
+        if (t < nf) {
+            const int side = (int)(t / ((long)G.nx * G.ny));
+            const long rem = t % ((long)G.nx * G.ny);
+            const int i = (int)(rem % G.nx) + 1, j = (int)(rem / G.nx) + 1;
+            if (((i + j) & 1) != color) return;
+            if (side == 0) {
+                const long cb = G.idx(i, j, 0);
+                const T inv = (T)1 / (T)M.g[8][cb];
+                const T csv = bc.cs[4] ? (T)bc.cs[4][(i - 1) + (long)G.nx * (j - 1)] : (T)0;
+                p[cb] = PV(i, j, 1) - (T)an * csv * inv +
+                        q * inv *
+                            ((T)M.g[6][cb] * (PV(i + 1, j, 1) + PV(i + 1, j, 0) - PV(i - 1, j, 1) - PV(i - 1, j, 0)) +
+                             (T)M.g[7][cb] * (PV(i, j + 1, 1) + PV(i, j + 1, 0) - PV(i, j - 1, 1) - PV(i, j - 1, 0)));
+            } else {
+                const int nz = G.nz;
+                const long ct = G.idx(i, j, nz);
+                const T inv = (T)1 / (T)M.g[8][ct];
+                const T csv = bc.cs[5] ? (T)bc.cs[5][(i - 1) + (long)G.nx * (j - 1)] : (T)0;
+                p[ct + G.sz] = PV(i, j, nz) + (T)an * csv * inv -
+                               q * inv *
+                                   ((T)M.g[6][ct] * (PV(i + 1, j, nz + 1) + PV(i + 1, j, nz) - PV(i - 1, j, nz + 1) -
+                                                     PV(i - 1, j, nz)) +
+                                    (T)M.g[7][ct] * (PV(i, j + 1, nz + 1) + PV(i, j + 1, nz) - PV(i, j - 1, nz + 1) -
+                                                     PV(i, j - 1, nz)));
+            }
+        }
+    }
+#undef PV
+}
+
+// Source of a non-face ghost position: periodic wrap first, then the
+// wall-wall edge rules (fsrc/multigrid.f03:1640-1738 for kp = 1,
+// 1358-1635 for kp = 0).  Returns false for interior / face positions.
+__device__ inline bool ghost_source(const Geo& G, int i, int j, int k, int& si, int& sj, int& sk) {
+    const bool gi = (i == 0 || i == G.nx + 1), gj = (j == 0 || j == G.ny + 1),
+               gk = (k == 0 || k == G.nz + 1);
+    const int ng = (int)gi + (int)gj + (int)gk;
+    if (ng == 0) return false;
+    // a single ghost index in a wall direction: face ghost, not a copy
+    if (ng == 1 && ((gi && G.ip == 1) || (gj && G.jp == 1) || (gk && G.kp == 1))) return false;
+    si = i; sj = j; sk = k;
+    if (gi && G.ip == 0) si = (i == 0) ? G.nx : 1;
+    if (gk && G.kp == 0) sk = (k == 0) ? G.nz : 1;
+    if (gj && G.jp == 0) sj = (j == 0) ? G.ny : 1;
+    for (int guard = 0; guard < 3; ++guard) {
+        const bool hi = (si == 0 || si == G.nx + 1), hj = (sj == 0 || sj == G.ny + 1),
+                   hk = (sk == 0 || sk == G.nz + 1);
+        const int n2 = (int)hi + (int)hj + (int)hk;
+        if (n2 <= 1) break;
+        if (hi && hj && !hk) {
+            // xi-eta edge: kp = 0 copies along eta from the xi-ghost, kp = 1 along xi
+            if (G.kp == 0) sj = (sj == 0) ? 1 : G.ny;
+            else si = (si == 0) ? 1 : G.nx;
+        } else if (hj && hk && !hi) {
+            sj = (sj == 0) ? 1 : G.ny;  // eta-zeta edge: copy along eta from the zeta-ghost
+        } else if (hi && hk && !hj) {
+            si = (si == 0) ? 1 : G.nx;  // xi-zeta edge: copy along xi from the zeta-ghost
+        } else {
+            si = (si == 0) ? 1 : G.nx;  // corner: along xi first
+        }
+    }
+    return true;
+}
+
+template <class T>
+__global__ void k_halo_edges(Geo G, T* __restrict__ p) {
+    // every ghost position: enumerate the six ghost planes of the box, each
+    // position counted once by giving planes i-ghost > j-ghost > k-ghost
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long NX = G.nx + 2, NY = G.ny + 2, NZ = G.nz + 2;
+    int i, j, k;
+    const long ni = 2 * NY * NZ;  // i in {0, nx+1}
+    const long nj = 2 * NX * NZ;  // j in {0, ny+1}, i interior
+    const long nk = 2 * NX * NY;  // k in {0, nz+1}, i,j interior
+    if (t < ni) {
+        i = (t / (NY * NZ)) ? G.nx + 1 : 0;
+        const long r = t % (NY * NZ);
+        j = (int)(r % NY);
+        k = (int)(r / NY);
+    } else if ((t -= ni) < nj) {
+        j = (t / (NX * NZ)) ? G.ny + 1 : 0;
+        const long r = t % (NX * NZ);
+        i = (int)(r % NX);
+        k = (int)(r / NX);
+        if (i == 0 || i == G.nx + 1) return;
+    } else if ((t -= nj) < nk) {
+        k = (t / (NX * NY)) ? G.nz + 1 : 0;
+        const long r = t % (NX * NY);
+        i = (int)(r % NX);
+        j = (int)(r / NX);
+        if (i == 0 || i == G.nx + 1 || j == 0 || j == G.ny + 1) return;
+    } else {
+        return;
+    }
+    int si, sj, sk;
+    if (!ghost_source(G, i, j, k, si, sj, sk)) return;
+    p[G.idx(i, j, k)] = p[G.idx(si, sj, sk)];
+}
+
+// ------------------------------------------------------------------------
+// Smoother factors: Thomas factorisation of M_line per column.
+// ------------------------------------------------------------------------
+template <class TM>
+__global__ void k_line_factors(Geo G, Met<TM> M, double inv_pot, float* __restrict__ invm,
+                               float* __restrict__ cpr) {
+    int i, k;
+    if (!column_of(G, i, k)) return;
+    const double s2 = inv_pot * inv_pot;
+    double cprev = 0.0;
+    for (int j = 1; j <= G.ny; ++j) {
+        const long c = G.idx(i, j, k);
+        const double g11p = (G.ip == 1 && i == G.nx) ? 0.0 : (double)M.g[0][c];
+        const double g11m = (G.ip == 1 && i == 1) ? 0.0 : (double)M.g[0][c - 1];
+        const double g22p = (j == G.ny) ? 0.0 : (double)M.g[4][c];
+        const double g22m = (j == 1) ? 0.0 : (double)M.g[4][c - G.sy];
+        const double g33p = (G.kp == 1 && k == G.nz) ? 0.0 : (double)M.g[8][c];
+        const double g33m = (G.kp == 1 && k == 1) ? 0.0 : (double)M.g[8][c - G.sz];
+        const double den = s2 * (g11p + g11m + g22p + g22m + g33p + g33m);
+        const double a = -s2 * g22m, cc = -s2 * g22p;
+        const double m = den - a * cprev;
+        const double cp = cc / m;
+        invm[c] = (float)(1.0 / m);
+        cpr[c] = (float)cp;
+        cprev = cp;
+    }
+}
+
+// ------------------------------------------------------------------------
+// Geometry: face metrics J^-1 g^{ij} / pot of the level with node stride st
+// (metrica, fsrc/metrica.f03:85-419, st = 1; mul_met,
+// fsrc/multigrid.f03:2520-2988).  Node coordinates come from the finest
+// node arrays with the periodic extension of cellep (fsrc/cellep.f03:29-98)
+// evaluated on the fly.  At coarse wall faces where the reference's
+// second-order one-sided derivative inverts the face Jacobian (strongly
+// stretched walls with few coarse cells), the first-order one-sided
+// derivative is used instead.
+// ------------------------------------------------------------------------
+struct Nodes {
+    const double *x, *y, *z;
+    int nx, ny, nz, ip, jp, kp;  // finest level
+};
+
+__device__ inline int fdiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
+
+__device__ inline void node_at(const Nodes& N, int i, int j, int k, double o[3]) {
+    // periodic-style extension in every direction (as cellep fills the ghost
+    // node layers), shift by the node span per period
+    const int mk = (k < 0 || k > N.nz) ? fdiv(k, N.nz) : 0;
+    const int mj = (j < 0 || j > N.ny) ? fdiv(j, N.ny) : 0;
+    const int mi = (i < 0 || i > N.nx) ? fdiv(i, N.nx) : 0;
+    const int ii = i - mi * N.nx, jj = j - mj * N.ny, kk = k - mk * N.nz;
+    auto at = [&](int a, int b, int c, int comp) -> double {
+        const long n = (long)a + (long)(N.nx + 1) * ((long)b + (long)(N.ny + 1) * c);
+        return comp == 0 ? N.x[n] : (comp == 1 ? N.y[n] : N.z[n]);
+    };
+    o[0] = at(ii, jj, kk, 0);
+    o[1] = at(ii, jj, kk, 1);
+    o[2] = at(ii, jj, kk, 2);
+    if (mi) o[0] += mi * (at(N.nx, jj, kk, 0) - at(0, jj, kk, 0));
+    if (mj) o[1] += mj * (at(ii, N.ny, kk, 1) - at(ii, 0, kk, 1));
+    if (mk) o[2] += mk * (at(ii, jj, N.nz, 2) - at(ii, jj, 0, 2));
+}
+
+struct Frame {
+    double cs[3], et[3], zt[3], giac;
+};
+
+__device__ inline Frame make_frame(const double c[3], const double e[3], const double t[3]) {
+    Frame F;
+    const double xcsi = c[0], ycsi = c[1], zcsi = c[2];
+    const double xeta = e[0], yeta = e[1], zeta = e[2];
+    const double xzet = t[0], yzet = t[1], zzet = t[2];
+    F.giac = xcsi * (yeta * zzet - yzet * zeta) - xeta * (ycsi * zzet - yzet * zcsi) +
+             xzet * (ycsi * zeta - yeta * zcsi);
+    F.cs[0] = yeta * zzet - yzet * zeta;
+    F.cs[1] = xzet * zeta - xeta * zzet;
+    F.cs[2] = xeta * yzet - xzet * yeta;
+    F.et[0] = yzet * zcsi - ycsi * zzet;
+    F.et[1] = xcsi * zzet - xzet * zcsi;
+    F.et[2] = xzet * ycsi - xcsi * yzet;
+    F.zt[0] = ycsi * zeta - yeta * zcsi;
+    F.zt[1] = xeta * zcsi - xcsi * zeta;
+    F.zt[2] = xcsi * yeta - xeta * ycsi;
+    return F;
+}
+
+__device__ inline double dot3(const double* a, const double* b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+__device__ inline double sq3(const double* a) { return a[0] * a[0] + a[1] * a[1] + a[2] * a[2]; }
+
+// four-node average in the given order
+__device__ inline void avg4(const Nodes& N, const int q[4][3], double o[3]) {
+    double a[3], b[3], c[3], d[3];
+    node_at(N, q[0][0], q[0][1], q[0][2], a);
+    node_at(N, q[1][0], q[1][1], q[1][2], b);
+    node_at(N, q[2][0], q[2][1], q[2][2], c);
+    node_at(N, q[3][0], q[3][1], q[3][2], d);
+    for (int m = 0; m < 3; ++m) o[m] = .25 * (a[m] + b[m] + c[m] + d[m]);
+}
+
+__device__ inline void hdiff(const Nodes& N, int a0, int a1, int a2, int b0, int b1, int b2, int c0,
+                             int c1, int c2, int d0, int d1, int d2, double o[3]) {
+    double A[3], B[3], C[3], D[3];
+    node_at(N, a0, a1, a2, A);
+    node_at(N, b0, b1, b2, B);
+    node_at(N, c0, c1, c2, C);
+    node_at(N, d0, d1, d2, D);
+    for (int m = 0; m < 3; ++m) o[m] = .5 * (A[m] + B[m]) - .5 * (C[m] + D[m]);
+}
+
+// one-sided derivative at a wall face: second order as the reference, with
+// the first-order fallback when the second-order one points the wrong way
+__device__ inline void wall_deriv(const double x0[3], const double x1[3], const double x2[3],
+                                  double sgn, int st, double o[3]) {
+    double s2[3], s1[3];
+    for (int m = 0; m < 3; ++m) {
+        s2[m] = sgn * .5 * (-3. * x0[m] + 4. * x1[m] - x2[m]);
+        s1[m] = sgn * (x1[m] - x0[m]);
+    }
+    if (st > 1 && dot3(s2, s1) <= 0.25 * sq3(s1)) {
+        for (int m = 0; m < 3; ++m) o[m] = s1[m];
+    } else {
+        for (int m = 0; m < 3; ++m) o[m] = s2[m];
+    }
+}
+
+// dir 0: xi faces (i = 0..nx), 1: eta faces (j = 0..ny), 2: zeta faces
+template <class TO>
+__global__ void k_metrics(Nodes N, Geo G, int st, int dir, TO* g0, TO* g1, TO* g2, double* v0,
+                          double* v1, double* v2) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y * blockDim.y + threadIdx.y;
+    const int cidx = blockIdx.z;
+    int i, j, k;
+    if (dir == 0) {
+        i = a; j = b + 1; k = cidx + 1;
+        if (i > G.nx || j > G.ny) return;
+    } else if (dir == 1) {
+        i = a + 1; j = b; k = cidx + 1;
+        if (i > G.nx || j > G.ny) return;
+    } else {
+        i = a + 1; j = b + 1; k = cidx;
+        if (i > G.nx || j > G.ny) return;
+    }
+    const double pot = (double)st;
+    double c[3], e[3], t[3];
+    if (dir == 0) {
+        const int j2 = j * st, j1 = (j - 1) * st, k2 = k * st, k1 = (k - 1) * st, ic = i * st;
+        auto face = [&](int aa, double o[3]) {
+            const int q[4][3] = {{aa, j2, k2}, {aa, j2, k1}, {aa, j1, k2}, {aa, j1, k1}};
+            avg4(N, q, o);
+        };
+        if (i == 0 && G.ip == 1) {
+            double x0[3], x1[3], x2[3];
+            face(0, x0); face(st, x1); face(2 * st, x2);
+            wall_deriv(x0, x1, x2, 1.0, st, c);
+        } else if (i == G.nx && G.ip == 1) {
+            double x0[3], x1[3], x2[3];
+            face(N.nx, x0); face((i - 1) * st, x1); face((i - 2) * st, x2);
+            wall_deriv(x0, x1, x2, -1.0, st, c);
+        } else {
+            double xd[3], xs[3];
+            face((i + 1) * st, xd); face((i - 1) * st, xs);
+            for (int m = 0; m < 3; ++m) c[m] = .5 * (xd[m] - xs[m]);
+        }
+        hdiff(N, ic, j2, k2, ic, j2, k1, ic, j1, k2, ic, j1, k1, e);
+        hdiff(N, ic, j2, k2, ic, j1, k2, ic, j2, k1, ic, j1, k1, t);
+        const Frame F = make_frame(c, e, t);
+        const long id = G.idx(i, j, k);
+        g0[id] = (TO)((sq3(F.cs) / F.giac) / pot);
+        g1[id] = (TO)((dot3(F.cs, F.et) / F.giac) / pot);
+        g2[id] = (TO)((dot3(F.cs, F.zt) / F.giac) / pot);
+        if (v0) { v0[id] = F.cs[0]; v1[id] = F.cs[1]; v2[id] = F.cs[2]; }
+    } else if (dir == 1) {
+        const int i2 = i * st, i1 = (i - 1) * st, k2 = k * st, k1 = (k - 1) * st, jc = j * st;
+        auto face = [&](int bb, double o[3]) {
+            const int q[4][3] = {{i2, bb, k2}, {i2, bb, k1}, {i1, bb, k1}, {i1, bb, k2}};
+            avg4(N, q, o);
+        };
+        if (j == 0 && G.jp == 1) {
+            double x0[3], x1[3], x2[3];
+            face(0, x0); face(st, x1); face(2 * st, x2);
+            wall_deriv(x0, x1, x2, 1.0, st, e);
+        } else if (j == G.ny && G.jp == 1) {
+            double x0[3], x1[3], x2[3];
+            face(N.ny, x0); face((j - 1) * st, x1); face((j - 2) * st, x2);
+            wall_deriv(x0, x1, x2, -1.0, st, e);
+        } else {
+            double xp[3], xm[3];
+            face((j + 1) * st, xp); face((j - 1) * st, xm);
+            for (int m = 0; m < 3; ++m) e[m] = .5 * (xp[m] - xm[m]);
+        }
+        hdiff(N, i2, jc, k2, i1, jc, k2, i2, jc, k1, i1, jc, k1, t);
+        hdiff(N, i2, jc, k2, i2, jc, k1, i1, jc, k2, i1, jc, k1, c);
+        const Frame F = make_frame(c, e, t);
+        const long id = G.idx(i, j, k);
+        g0[id] = (TO)((dot3(F.et, F.cs) / F.giac) / pot);
+        g1[id] = (TO)((sq3(F.et) / F.giac) / pot);
+        g2[id] = (TO)((dot3(F.et, F.zt) / F.giac) / pot);
+        if (v0) { v0[id] = F.et[0]; v1[id] = F.et[1]; v2[id] = F.et[2]; }
+    } else {
+        const int i2 = i * st, i1 = (i - 1) * st, j2 = j * st, j1 = (j - 1) * st, kc = k * st;
+        auto face = [&](int cc, double o[3]) {
+            const int q[4][3] = {{i2, j2, cc}, {i2, j1, cc}, {i1, j1, cc}, {i1, j2, cc}};
+            avg4(N, q, o);
+        };
+        if (k == 0 && G.kp == 1) {
+            double x0[3], x1[3], x2[3];
+            face(0, x0); face(st, x1); face(2 * st, x2);
+            wall_deriv(x0, x1, x2, 1.0, st, t);
+        } else if (k == G.nz && G.kp == 1) {
+            double x0[3], x1[3], x2[3];
+            face(N.nz, x0); face((k - 1) * st, x1); face((k - 2) * st, x2);
+            wall_deriv(x0, x1, x2, -1.0, st, t);
+        } else {
+            double xa[3], xb[3];
+            face((k + 1) * st, xa); face((k - 1) * st, xb);
+            for (int m = 0; m < 3; ++m) t[m] = .5 * (xa[m] - xb[m]);
+        }
+        hdiff(N, i2, j2, kc, i2, j1, kc, i1, j2, kc, i1, j1, kc, c);
+        hdiff(N, i2, j2, kc, i1, j2, kc, i2, j1, kc, i1, j1, kc, e);
+        const Frame F = make_frame(c, e, t);
+        const long id = G.idx(i, j, k);
+        g0[id] = (TO)((dot3(F.zt, F.cs) / F.giac) / pot);
+        g1[id] = (TO)((dot3(F.zt, F.et) / F.giac) / pot);
+        g2[id] = (TO)((sq3(F.zt) / F.giac) / pot);
+        if (v0) { v0[id] = F.zt[0]; v1[id] = F.zt[1]; v2[id] = F.zt[2]; }
+    }
+}
+
+// centroid Jacobian giac (fsrc/metrica.f03:432-483), finest level
+__global__ void k_giac(Nodes N, Geo G, double* giac) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > G.nx || j > G.ny) return;
+    double P[2][2][2][3];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c) node_at(N, i - 1 + a, j - 1 + b, k - 1 + c, P[a][b][c]);
+    double c[3], e[3], t[3];
+    for (int q = 0; q < 3; ++q) {
+        c[q] = .25 * (P[1][1][1][q] + P[1][1][0][q] + P[1][0][0][q] + P[1][0][1][q]) -
+               .25 * (P[0][1][1][q] + P[0][1][0][q] + P[0][0][0][q] + P[0][0][1][q]);
+        e[q] = .25 * (P[1][1][1][q] + P[1][1][0][q] + P[0][1][0][q] + P[0][1][1][q]) -
+               .25 * (P[1][0][1][q] + P[1][0][0][q] + P[0][0][0][q] + P[0][0][1][q]);
+        t[q] = .25 * (P[1][1][1][q] + P[1][0][1][q] + P[0][0][1][q] + P[0][1][1][q]) -
+               .25 * (P[1][1][0][q] + P[1][0][0][q] + P[0][0][0][q] + P[0][1][0][q]);
+    }
+    giac[G.idx(i, j, k)] = make_frame(c, e, t).giac;
+}
+
+// ------------------------------------------------------------------------
+// Synthetic case: counter-based N(0,1) cell velocity, contra fluxes, diver
+// ------------------------------------------------------------------------
+__device__ inline uint64_t splitmix64(uint64_t v) {
+    v += 0x9E3779B97F4A7C15ull;
+    v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
+    v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
+    return v ^ (v >> 31);
+}
+
+__device__ inline double gauss(uint64_t seed, uint32_t field, int64_t ctr) {
+    const uint64_t key = splitmix64(seed ^ ((uint64_t)field << 56));
+    const uint64_t h1 = splitmix64(key ^ (2ull * (uint64_t)ctr));
+    const uint64_t h2 = splitmix64(key ^ (2ull * (uint64_t)ctr + 1ull));
+    const double u1 = ((double)(h1 >> 11) + 1.0) * 0x1.0p-53;
+    const double u2 = (double)(h2 >> 11) * 0x1.0p-53;
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925 * u2);
+}
+
+// velocity component c of cell (i,j,k) incl. periodic ghost layers (ghost
+// cells in wall directions are never read: wall fluxes are zero)
+__device__ inline double vel(const Geo& G, uint64_t seed, int c, int i, int j, int k) {
+    if (G.ip == 0) { if (i == 0) i = G.nx; else if (i == G.nx + 1) i = 1; }
+    if (G.jp == 0) { if (j == 0) j = G.ny; else if (j == G.ny + 1) j = 1; }
+    if (G.kp == 0) { if (k == 0) k = G.nz; else if (k == G.nz + 1) k = 1; }
+    return gauss(seed, (uint32_t)c, (int64_t)G.idx(i, j, k));
+}
+
+// contravariant flux of face f (dir) from the seeded velocity
+// (fsrc/contra.f03:73-87,132-146,204-220), zero on wall faces
+__device__ inline double face_flux(const Geo& G, const double* const* csv, uint64_t seed, int dir,
+                                   int i, int j, int k) {
+    int i2 = i, j2 = j, k2 = k;
+    if (dir == 0) {
+        if (G.ip == 1 && (i == 0 || i == G.nx)) return 0.0;
+        i2 = i + 1;
+    } else if (dir == 1) {
+        if (G.jp == 1 && (j == 0 || j == G.ny)) return 0.0;
+        j2 = j + 1;
+    } else {
+        if (G.kp == 1 && (k == 0 || k == G.nz)) return 0.0;
+        k2 = k + 1;
+    }
+    const double ui = .5 * (vel(G, seed, 0, i, j, k) + vel(G, seed, 0, i2, j2, k2));
+    const double vi = .5 * (vel(G, seed, 1, i, j, k) + vel(G, seed, 1, i2, j2, k2));
+    const double wi = .5 * (vel(G, seed, 2, i, j, k) + vel(G, seed, 2, i2, j2, k2));
+    const long id = G.idx(i, j, k);
+    return ui * csv[3 * dir][id] + vi * csv[3 * dir + 1][id] + wi * csv[3 * dir + 2][id];
+}
+
+struct CsvP {
+    const double* v[9];
+};
+
+// rhs = div(flux)/dt (fsrc/diver.f03:20-28) of the seeded case; partial sums
+__global__ void __launch_bounds__(256) k_case_rhs(Geo G, CsvP V, uint64_t seed, double dt,
+                                                  double* __restrict__ rhs, double* partials,
+                                                  int nblocks) {
+    int i, k;
+    const bool act = column_of(G, i, k);
+    double acc[1] = {0};
+    if (act) {
+        for (int j = 1; j <= G.ny; ++j) {
+            const double d = face_flux(G, V.v, seed, 0, i, j, k) - face_flux(G, V.v, seed, 0, i - 1, j, k) +
+                             face_flux(G, V.v, seed, 1, i, j, k) - face_flux(G, V.v, seed, 1, i, j - 1, k) +
+                             face_flux(G, V.v, seed, 2, i, j, k) - face_flux(G, V.v, seed, 2, i, j, k - 1);
+            const double r = d / dt;
+            rhs[G.idx(i, j, k)] = r;
+            acc[0] += r;
+        }
+    }
+    const bool mx[1] = {false};
+    block_reduce_store<1>(acc, partials, nblocks, mx);
+}
+
+// diver from caller fluxes in the ghosted face layout
+__global__ void __launch_bounds__(256) k_diver(Geo G, const double* __restrict__ uc,
+                                               const double* __restrict__ vc,
+                                               const double* __restrict__ wc, double dt,
+                                               double* __restrict__ rhs) {
+    int i, k;
+    if (!column_of(G, i, k)) return;
+    for (int j = 1; j <= G.ny; ++j) {
+        const long c = G.idx(i, j, k);
+        rhs[c] = (uc[c] - uc[c - 1] + vc[c] - vc[c - G.sy] + wc[c] - wc[c - G.sz]) / dt;
+    }
+}
+
+// column sums of the interior (for the compatibility projection)
+__global__ void __launch_bounds__(256) k_sum(Geo G, const double* __restrict__ a, double* partials,
+                                             int nblocks) {
+    int i, k;
+    const bool act = column_of(G, i, k);
+    double acc[2] = {0, 0};
+    if (act)
+        for (int j = 1; j <= G.ny; ++j) {
+            const double v = a[G.idx(i, j, k)];
+            acc[0] += v;
+            acc[1] += v * v;
+        }
+    const bool mx[2] = {false, false};
+    block_reduce_store<2>(acc, partials, nblocks, mx);
+}
+
+// sums of the partials of k_sum / k_case_rhs into S[S_SUM], S[S_SUM+1]
+__global__ void k_fin_sum(const double* __restrict__ part, int nb, int nv, double* S) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int v = 0; v < nv; ++v) {
+        double s = 0;
+        for (int q = 0; q < nb; ++q) s += part[(long)v * nb + q];
+        S[S_SUM + v] = s;
+    }
+}
+
+// a[interior] -= (S[S_SUM] - bflux) / ncell
+__global__ void __launch_bounds__(256) k_sub_mean(Geo G, double* __restrict__ a, const double* S,
+                                                  double bflux, double ncell) {
+    int i, k;
+    if (!column_of(G, i, k)) return;
+    const double m = (S[S_SUM] - bflux) / ncell;
+    for (int j = 1; j <= G.ny; ++j) a[G.idx(i, j, k)] -= m;
+}
+
+// interior perturbation p += rel * std * N(0,1)
+__global__ void __launch_bounds__(256) k_perturb(Geo G, double* __restrict__ a, const double* S,
+                                                 uint64_t seed, double rel, double ncell) {
+    int i, k;
+    if (!column_of(G, i, k)) return;
+    const double mean = S[S_SUM] / ncell;
+    const double var = fmax(S[S_SUM + 1] / ncell - mean * mean, 0.0);
+    const double sd = sqrt(var);
+    for (int j = 1; j <= G.ny; ++j) {
+        const long c = G.idx(i, j, k);
+        a[c] += rel * sd * gauss(seed, 7u, (int64_t)c);
+    }
+}
+
+// elementwise helpers over the full ghosted array
+template <class TI, class TO>
+__global__ void k_convert(long n, const TI* __restrict__ a, TO* __restrict__ b) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) b[t] = (TO)a[t];
+}
+
+// gradie (fsrc/gradie.f03:28-117)
+__global__ void __launch_bounds__(256) k_gradie(Geo G, const double* __restrict__ fi, CsvP V,
+                                                const double* __restrict__ giac, Met<double> M,
+                                                double dt, double* gra1, double* gra2, double* gra3,
+                                                double* cg1, double* cg2, double* cg3) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > G.nx || j > G.ny || k > G.nz) return;
+    const long c = G.idx(i, j, k);
+    const long sy = G.sy, sz = G.sz;
+    const bool in = i >= 1 && j >= 1 && k >= 1;
+    if (in) {
+        const double coef = (-0.5 * dt) / giac[c];
+        double* out[3] = {gra1, gra2, gra3};
+        for (int q = 0; q < 3; ++q) {
+            const double* cx = V.v[q];
+            const double* ex = V.v[3 + q];
+            const double* zx = V.v[6 + q];
+            const double s = cx[c] * (fi[c] + fi[c + 1]) - cx[c - 1] * (fi[c] + fi[c - 1]) +
+                             ex[c] * (fi[c] + fi[c + sy]) - ex[c - sy] * (fi[c] + fi[c - sy]) +
+                             zx[c] * (fi[c] + fi[c + sz]) - zx[c - sz] * (fi[c] + fi[c - sz]);
+            out[q][c] = coef * s;
+        }
+    }
+    if (j >= 1 && k >= 1)
+        cg1[c] = M.g[0][c] * (fi[c + 1] - fi[c]) +
+                 .25 * M.g[1][c] * (fi[c + 1 + sy] + fi[c + sy] - fi[c + 1 - sy] - fi[c - sy]) +
+                 .25 * M.g[2][c] * (fi[c + 1 + sz] + fi[c + sz] - fi[c + 1 - sz] - fi[c - sz]);
+    if (i >= 1 && k >= 1)
+        cg2[c] = M.g[4][c] * (fi[c + sy] - fi[c]) +
+                 .25 * M.g[3][c] * (fi[c + 1 + sy] + fi[c + 1] - fi[c - 1 + sy] - fi[c - 1]) +
+                 .25 * M.g[5][c] * (fi[c + sy + sz] + fi[c + sz] - fi[c + sy - sz] - fi[c - sz]);
+    if (i >= 1 && j >= 1)
+        cg3[c] = M.g[8][c] * (fi[c + sz] - fi[c]) +
+                 .25 * M.g[6][c] * (fi[c + 1 + sz] + fi[c + 1] - fi[c - 1 + sz] - fi[c - 1]) +
+                 .25 * M.g[7][c] * (fi[c + sy + sz] + fi[c + sy] - fi[c - sy + sz] - fi[c - sy]);
+}
+
+// reference-shaped array <-> ghosted array: the box [i0..i1]x[j0..j1]x[k0..k1]
+// of the ghosted layout stored compactly (first index fastest).
+template <class T>
+__global__ void k_pack(Geo G, int i0, int i1, int j0, int j1, int k0, const T* __restrict__ compact,
+                       T* __restrict__ ghosted) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y * blockDim.y + threadIdx.y;
+    const int c = blockIdx.z;
+    const int ni = i1 - i0 + 1, nj = j1 - j0 + 1;
+    if (a >= ni || b >= nj) return;
+    ghosted[G.idx(i0 + a, j0 + b, k0 + c)] = compact[(long)a + (long)ni * ((long)b + (long)nj * c)];
+}
+
+template <class T>
+__global__ void k_unpack(Geo G, int i0, int i1, int j0, int j1, int k0, const T* __restrict__ ghosted,
+                         T* __restrict__ compact) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y * blockDim.y + threadIdx.y;
+    const int c = blockIdx.z;
+    const int ni = i1 - i0 + 1, nj = j1 - j0 + 1;
+    if (a >= ni || b >= nj) return;
+    compact[(long)a + (long)ni * ((long)b + (long)nj * c)] = ghosted[G.idx(i0 + a, j0 + b, k0 + c)];
+}
+
+}  // namespace plesmg

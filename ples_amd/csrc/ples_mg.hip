@@ -1,0 +1,1099 @@
+// ples_mg.hip -- host side of the MI355X pressure multigrid: level
+// hierarchy, V-cycle, FP64 refinement loop (GCR-accelerated), C ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/ples_mg.h"
+#include "mg_kernels.hpp"
+
+using namespace plesmg;
+
+namespace {
+
+struct Level {
+    Geo G{};
+    bool f32 = false;  // V-cycle precision of this level
+    double pot = 1.0;
+    double* g64[9] = {};
+    float* g32[9] = {};
+    float* invm = nullptr;
+    float* cpr = nullptr;
+    void* p[2] = {nullptr, nullptr};
+    void* b = nullptr;
+    void* r = nullptr;
+    int cur = 0;
+    long ncell() const { return (long)G.nx * G.ny * G.nz; }
+};
+
+struct Rec {
+    int fam;
+    double bytes;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct ples_mg_handle {
+    int dev = 0;
+    hipStream_t s = nullptr;
+    ples_mg_opts o{};
+    int jx = 0, jy = 0, jz = 0, ip = 0, jp = 1, kp = 0, L = 1;
+    std::vector<Level> lv;
+    double* nodes[3] = {};
+    double* x = nullptr;  // resident iterate (FP64, ghosted)
+    double* bvec = nullptr;
+    double* rvec = nullptr;
+    double* Z[2] = {};
+    double* W[2] = {};
+    double* cs[6] = {};
+    bool have_cs = false;
+    double dt = 1.0;
+    double* S = nullptr;   // device scalars
+    double* Sh = nullptr;  // pinned host mirror
+    double* part = nullptr;
+    int part_n = 0;
+    size_t bytes = 0;
+    std::vector<std::pair<void*, size_t>> allocs;
+    bool timing = false;
+    std::vector<Rec> recs;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    double last_removed = 0.0;
+};
+
+namespace {
+
+using H = ples_mg_handle;
+
+int hip_status(hipError_t e) {
+    if (e == hipSuccess) return PLES_MG_OK;
+    if (e == hipErrorOutOfMemory) return PLES_MG_ERR_OOM;
+    fprintf(stderr, "ples_mg: HIP error %s\n", hipGetErrorString(e));
+    return PLES_MG_ERR_HIP;
+}
+
+#define CK(x)                                  \
+    do {                                       \
+        hipError_t e_ = (x);                   \
+        if (e_ != hipSuccess) return hip_status(e_); \
+    } while (0)
+#define CKS(x)                          \
+    do {                                \
+        int st_ = (x);                  \
+        if (st_ != PLES_MG_OK) return st_; \
+    } while (0)
+
+int dalloc(H* h, void** p, size_t nbytes) {
+    *p = nullptr;
+    if (nbytes == 0) return PLES_MG_OK;
+    hipError_t e = hipMalloc(p, nbytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return PLES_MG_ERR_OOM;
+    }
+    e = hipMemsetAsync(*p, 0, nbytes, h->s);
+    if (e != hipSuccess) return hip_status(e);
+    h->allocs.push_back({*p, nbytes});
+    h->bytes += nbytes;
+    return PLES_MG_OK;
+}
+
+void dfree(H* h, void* p) {
+    if (!p) return;
+    for (auto it = h->allocs.begin(); it != h->allocs.end(); ++it)
+        if (it->first == p) {
+            h->bytes -= it->second;
+            h->allocs.erase(it);
+            break;
+        }
+    (void)hipFree(p);
+}
+
+Geo make_geo(int nx, int ny, int nz, int ip, int jp, int kp) {
+    Geo G;
+    G.nx = nx; G.ny = ny; G.nz = nz;
+    G.ip = ip; G.jp = jp; G.kp = kp;
+    G.sy = nx + 2;
+    G.sz = (long)(nx + 2) * (ny + 2);
+    G.n = G.sz * (nz + 2);
+    return G;
+}
+
+int pow2ceil(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// column launch: block (bx, by) over (i, k)
+struct ColLaunch {
+    dim3 grid, block;
+    int nblocks;
+};
+ColLaunch col_launch(const Geo& G) {
+    ColLaunch c;
+    const int bx = std::min(64, pow2ceil(G.nx));
+    const int by = std::max(1, std::min(256 / bx, pow2ceil(G.nz)));
+    c.block = dim3(bx, by, 1);
+    c.grid = dim3((G.nx + bx - 1) / bx, (G.nz + by - 1) / by, 1);
+    c.nblocks = (int)(c.grid.x * c.grid.y);
+    return c;
+}
+
+// per-launch event timing
+struct KT {
+    H* h;
+    int fam;
+    double bytes;
+    hipEvent_t a = nullptr, b = nullptr;
+    KT(H* h_, int fam_, double bytes_) : h(h_), fam(fam_), bytes(bytes_) {
+        if (h->timing) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, h->s);
+        }
+    }
+    ~KT() {
+        if (h->timing) {
+            (void)hipEventRecord(b, h->s);
+            h->recs.push_back(Rec{fam, bytes, a, b});
+        }
+    }
+};
+
+template <class T>
+Met<T> met(const Level& L);
+template <>
+Met<float> met<float>(const Level& L) {
+    Met<float> m;
+    for (int q = 0; q < 9; ++q) m.g[q] = L.g32[q];
+    return m;
+}
+template <>
+Met<double> met<double>(const Level& L) {
+    Met<double> m;
+    for (int q = 0; q < 9; ++q) m.g[q] = L.g64[q];
+    return m;
+}
+
+// ---------------------------------------------------------------- halo
+template <class T>
+int halo(H* h, const Level& L, T* p, double an_over_dt, bool with_cs, int passes) {
+    const Geo& G = L.G;
+    BCs bc;
+    for (int q = 0; q < 6; ++q) bc.cs[q] = (with_cs && h->have_cs) ? h->cs[q] : nullptr;
+    bc.an_over_dt = an_over_dt;
+    long nf = 2L * G.nx * G.nz;
+    if (G.ip == 1) nf += 2L * G.ny * G.nz;
+    if (G.kp == 1) nf += 2L * G.nx * G.ny;
+    const long NX = G.nx + 2, NY = G.ny + 2, NZ = G.nz + 2;
+    const long ne = 2 * NY * NZ + 2 * NX * NZ + 2 * NX * NY;
+    const double bytes = (double)(nf * 6 * passes + ne * 2) * sizeof(T);
+    KT kt(h, PLES_MG_K_HALO, bytes);
+    for (int ps = 0; ps < passes; ++ps)
+        for (int color = 0; color < 2; ++color)
+            hipLaunchKernelGGL((k_halo_face<T, T>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, h->s, G,
+                               met<T>(L), p, color, bc);
+    hipLaunchKernelGGL((k_halo_edges<T>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, h->s, G, p);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+// exact ghost fill of an FP64 fine-level vector with the outer metrics
+int halo64(H* h, double* p, bool with_bc) {
+    const Level& L0 = h->lv[0];
+    Met<double> m = met<double>(L0);
+    const Geo& G = L0.G;
+    BCs bc;
+    for (int q = 0; q < 6; ++q) bc.cs[q] = (with_bc && h->have_cs) ? h->cs[q] : nullptr;
+    bc.an_over_dt = with_bc ? 1.0 / h->dt : 0.0;
+    long nf = 2L * G.nx * G.nz;
+    if (G.ip == 1) nf += 2L * G.ny * G.nz;
+    if (G.kp == 1) nf += 2L * G.nx * G.ny;
+    const long NX = G.nx + 2, NY = G.ny + 2, NZ = G.nz + 2;
+    const long ne = 2 * NY * NZ + 2 * NX * NZ + 2 * NX * NY;
+    const int passes = std::max(1, h->o.halo_passes);
+    KT kt(h, PLES_MG_K_HALO, (double)(nf * 6 * passes + ne * 2) * 8.0);
+    for (int ps = 0; ps < passes; ++ps)
+        for (int color = 0; color < 2; ++color)
+            hipLaunchKernelGGL((k_halo_face<double, double>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0,
+                               h->s, G, m, p, color, bc);
+    hipLaunchKernelGGL((k_halo_edges<double>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, h->s, G, p);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+// ---------------------------------------------------------------- V-cycle pieces
+template <class T, class TR>
+int smooth(H* h, Level& L, const TR* rhs, int nu, bool zero_start, bool fine) {
+    const ColLaunch cl = col_launch(L.G);
+    const double sm = sizeof(T), sp = sizeof(T), sr = sizeof(TR);
+    const double N = (double)L.ncell();
+    for (int s = 0; s < nu; ++s) {
+        const int zin = (zero_start && s == 0) ? 1 : 0;
+        const double bytes = zin ? N * (sr + sm + 8 + 3 * sp) : N * (9 * sm + 4 * sp + sr + 12);
+        {
+            KT kt(h, fine ? PLES_MG_K_SMOOTH_FINE : PLES_MG_K_SMOOTH, bytes);
+            hipLaunchKernelGGL((k_yline<T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), L.invm, L.cpr,
+                               (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->o.omega,
+                               (T)(1.0 / L.pot), zin);
+        }
+        L.cur ^= 1;
+        if (L.G.ip == 1 || L.G.kp == 1) CKS(halo<T>(h, L, (T*)L.p[L.cur], 0.0, false, 1));
+    }
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+template <class T, class TR>
+int vresid(H* h, Level& L, const TR* rhs, bool fine) {
+    const ColLaunch cl = col_launch(L.G);
+    const double N = (double)L.ncell();
+    KT kt(h, fine ? PLES_MG_K_RESID_FINE : PLES_MG_K_RESID, N * (9 * sizeof(T) + 2 * sizeof(T) + sizeof(TR)));
+    hipLaunchKernelGGL((k_resid<0, T, T, TR, T>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L),
+                       (const T*)L.p[L.cur], rhs, (T*)L.r, (T)(1.0 / L.pot), (const double*)nullptr,
+                       (const double*)nullptr, (double*)nullptr, 0);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+template <class TF, class TC>
+int restrict_to(H* h, Level& F, Level& Cc) {
+    dim3 block(std::min(64, pow2ceil(Cc.G.nx)), 4);
+    dim3 grid((Cc.G.nx + block.x - 1) / block.x, (Cc.G.ny + 3) / 4, Cc.G.nz);
+    KT kt(h, PLES_MG_K_TRANSFER, (double)F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC));
+    hipLaunchKernelGGL((k_restrict<TF, TC>), grid, block, 0, h->s, Cc.G, F.G, (const TF*)F.r, (TC*)Cc.b);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+template <class TC, class TF>
+int prolong_to(H* h, Level& Cc, Level& F) {
+    dim3 block(std::min(64, pow2ceil(F.G.nx + 2)), 4);
+    dim3 grid((F.G.nx + 2 + block.x - 1) / block.x, (F.G.ny + 2 + 3) / 4, F.G.nz + 2);
+    KT kt(h, PLES_MG_K_TRANSFER, 2.0 * F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC));
+    hipLaunchKernelGGL((k_prolong<TC, TF>), grid, block, 0, h->s, Cc.G, F.G, (const TC*)Cc.p[Cc.cur],
+                       (TF*)F.p[F.cur]);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+int lv_smooth(H* h, int n, const double* r0, int nu, bool zero) {
+    Level& L = h->lv[n];
+    if (n == 0) {
+        return L.f32 ? smooth<float, double>(h, L, r0, nu, zero, true)
+                     : smooth<double, double>(h, L, r0, nu, zero, true);
+    }
+    return L.f32 ? smooth<float, float>(h, L, (const float*)L.b, nu, zero, false)
+                 : smooth<double, double>(h, L, (const double*)L.b, nu, zero, false);
+}
+
+int lv_resid(H* h, int n, const double* r0) {
+    Level& L = h->lv[n];
+    if (n == 0) return L.f32 ? vresid<float, double>(h, L, r0, true) : vresid<double, double>(h, L, r0, true);
+    return L.f32 ? vresid<float, float>(h, L, (const float*)L.b, false)
+                 : vresid<double, double>(h, L, (const double*)L.b, false);
+}
+
+int lv_restrict(H* h, int n) {
+    Level &F = h->lv[n], &Cc = h->lv[n + 1];
+    if (F.f32 && Cc.f32) return restrict_to<float, float>(h, F, Cc);
+    if (F.f32 && !Cc.f32) return restrict_to<float, double>(h, F, Cc);
+    if (!F.f32 && Cc.f32) return restrict_to<double, float>(h, F, Cc);
+    return restrict_to<double, double>(h, F, Cc);
+}
+
+int lv_prolong(H* h, int n) {  // n+1 -> n
+    Level &F = h->lv[n], &Cc = h->lv[n + 1];
+    if (F.f32 && Cc.f32) return prolong_to<float, float>(h, Cc, F);
+    if (F.f32 && !Cc.f32) return prolong_to<double, float>(h, Cc, F);
+    if (!F.f32 && Cc.f32) return prolong_to<float, double>(h, Cc, F);
+    return prolong_to<double, double>(h, Cc, F);
+}
+
+// z = V(r0): one V-cycle for A z = r0 from a zero initial guess
+int vcycle(H* h, const double* r0) {
+    const int L = h->L;
+    for (int n = 0; n < L; ++n) {
+        const int nu = (n == L - 1) ? h->o.nu_coarse : h->o.nu_pre;
+        CKS(lv_smooth(h, n, r0, std::max(1, nu), true));
+        if (n < L - 1) {
+            CKS(lv_resid(h, n, r0));
+            CKS(lv_restrict(h, n));
+        }
+    }
+    for (int n = L - 2; n >= 0; --n) {
+        CKS(lv_prolong(h, n));
+        if (h->o.nu_post > 0) CKS(lv_smooth(h, n, r0, h->o.nu_post, false));
+    }
+    return PLES_MG_OK;
+}
+
+// ---------------------------------------------------------------- outer loop
+int read_S(H* h) {
+    CK(hipMemcpyAsync(h->Sh, h->S, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, h->s));
+    CK(hipStreamSynchronize(h->s));
+    return PLES_MG_OK;
+}
+
+int set_S(H* h, int idx, double v) {
+    h->Sh[idx] = v;
+    CK(hipMemcpyAsync(h->S + idx, h->Sh + idx, sizeof(double), hipMemcpyHostToDevice, h->s));
+    return PLES_MG_OK;
+}
+
+// r = b - A x (exact ghosts of x with the boundary fluxes), sums into S
+int outer_residual(H* h, double* x, double* r) {
+    CKS(halo64(h, x, true));
+    const Level& L0 = h->lv[0];
+    const ColLaunch cl = col_launch(L0.G);
+    {
+        KT kt(h, PLES_MG_K_OUTER_APPLY, (double)L0.ncell() * (72 + 24));
+        hipLaunchKernelGGL((k_resid<1, double, double, double, double>), cl.grid, cl.block, 0, h->s, L0.G,
+                           met<double>(L0), (const double*)x, (const double*)h->bvec, r, 1.0,
+                           (const double*)nullptr, (const double*)nullptr, h->part, cl.nblocks);
+    }
+    hipLaunchKernelGGL(k_fin_resid, dim3(1), dim3(1), 0, h->s, (const double*)h->part, cl.nblocks, h->S);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+int solve_core(H* h, double eps, int maxcyc, ples_mg_result* res) {
+    Level& L0 = h->lv[0];
+    const ColLaunch cl = col_launch(L0.G);
+    const double N0 = (double)L0.ncell();
+    CK(hipEventRecord(h->t0, h->s));
+    CKS(outer_residual(h, h->x, h->rvec));
+    CKS(read_S(h));
+    const double bb = h->Sh[S_BB];
+    const double bnorm = std::sqrt(bb) > 0 ? std::sqrt(bb) : 1.0;
+    auto measure = [&](double rr, double rmax) {
+        return (h->o.stop_mode == PLES_MG_STOP_ABS_MAX) ? rmax : std::sqrt(rr) / bnorm;
+    };
+    double rr = h->Sh[S_RR], rmax = h->Sh[S_RMAX];
+    if (!std::isfinite(rr)) return PLES_MG_ERR_NONFINITE;
+    int cycles = 0;
+    int status = PLES_MG_NOT_CONVERGED;
+    CKS(set_S(h, S_HAVEP, 0.0));
+    int q = 0;
+    bool conv = measure(rr, rmax) < eps;
+    if (conv) status = PLES_MG_OK;
+    while (!conv && cycles < maxcyc) {
+        ++cycles;
+        CKS(vcycle(h, h->rvec));
+        double* Zq = h->Z[q];
+        double* Wq = h->W[q];
+        double* Zp = h->Z[1 - q];
+        double* Wp = h->W[1 - q];
+        {
+            const long n = L0.G.n;
+            const int nb = (int)std::min<long>((n + 255) / 256, 8192);
+            if (L0.f32)
+                hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(256), 0, h->s, n,
+                                   (const float*)L0.p[L0.cur], Zq);
+            else
+                hipLaunchKernelGGL((k_convert<double, double>), dim3(nb), dim3(256), 0, h->s, n,
+                                   (const double*)L0.p[L0.cur], Zq);
+        }
+        CKS(halo64(h, Zq, false));
+        const bool havep = h->Sh[S_HAVEP] != 0.0 && h->o.krylov;
+        {
+            KT kt(h, PLES_MG_K_OUTER_APPLY, N0 * (72 + 24 + (havep ? 8 : 0)));
+            hipLaunchKernelGGL((k_resid<2, double, double, double, double>), cl.grid, cl.block, 0, h->s, L0.G,
+                               met<double>(L0), (const double*)Zq, (const double*)nullptr, Wq, 1.0,
+                               (const double*)h->rvec, (const double*)(havep ? Wp : nullptr), h->part,
+                               cl.nblocks);
+        }
+        hipLaunchKernelGGL(k_fin_apply, dim3(1), dim3(1), 0, h->s, (const double*)h->part, cl.nblocks, h->S,
+                           h->o.krylov);
+        {
+            KT kt(h, PLES_MG_K_OUTER_UPDATE, N0 * (48 + (havep ? 32 + 16 : 0)));
+            hipLaunchKernelGGL(k_gcr_update, cl.grid, cl.block, 0, h->s, L0.G, h->x, h->rvec, Zq, Wq,
+                               (const double*)Zp, (const double*)Wp, (const double*)h->S, h->part, cl.nblocks);
+        }
+        hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(1), 0, h->s, (const double*)h->part, cl.nblocks, h->S);
+        CK(hipGetLastError());
+        q = 1 - q;
+        CKS(read_S(h));
+        rr = h->Sh[S_RR];
+        rmax = h->Sh[S_RMAX];
+        if (!std::isfinite(rr)) {
+            status = PLES_MG_ERR_NONFINITE;
+            break;
+        }
+        if (measure(rr, rmax) < eps) {
+            // confirm with the true residual; restart the recurrence if it drifted
+            CKS(outer_residual(h, h->x, h->rvec));
+            CKS(read_S(h));
+            rr = h->Sh[S_RR];
+            rmax = h->Sh[S_RMAX];
+            if (measure(rr, rmax) < eps) {
+                conv = true;
+                status = PLES_MG_OK;
+            } else {
+                CKS(set_S(h, S_HAVEP, 0.0));
+            }
+        }
+    }
+    // leave the iterate with exact ghosts (the reference returns fi with ghosts)
+    CKS(halo64(h, h->x, true));
+    CK(hipEventRecord(h->t1, h->s));
+    CK(hipStreamSynchronize(h->s));
+    float ms = 0.f;
+    CK(hipEventElapsedTime(&ms, h->t0, h->t1));
+    if (res) {
+        res->status = status;
+        res->cycles = cycles;
+        res->rel_res = std::sqrt(rr) / bnorm;
+        res->abs_res = rmax;
+        res->rhs_mean_removed = h->last_removed;
+        res->seconds = ms * 1e-3;
+    }
+    return status;
+}
+
+// ---------------------------------------------------------------- packing
+template <class T>
+int pack_box(H* h, const Geo& G, int i0, int i1, int j0, int j1, int k0, int k1, const T* host, T* dghost,
+             T* staging) {
+    const long ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
+    CK(hipMemcpyAsync(staging, host, sizeof(T) * ni * nj * nk, hipMemcpyHostToDevice, h->s));
+    dim3 block(64, 4);
+    dim3 grid((unsigned)((ni + 63) / 64), (unsigned)((nj + 3) / 4), (unsigned)nk);
+    hipLaunchKernelGGL((k_pack<T>), grid, block, 0, h->s, G, i0, i1, j0, j1, k0, (const T*)staging, dghost);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+template <class T>
+int unpack_box(H* h, const Geo& G, int i0, int i1, int j0, int j1, int k0, int k1, const T* dghost, T* host,
+               T* staging) {
+    const long ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
+    dim3 block(64, 4);
+    dim3 grid((unsigned)((ni + 63) / 64), (unsigned)((nj + 3) / 4), (unsigned)nk);
+    hipLaunchKernelGGL((k_unpack<T>), grid, block, 0, h->s, G, i0, i1, j0, j1, k0, dghost, staging);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(host, staging, sizeof(T) * ni * nj * nk, hipMemcpyDeviceToHost, h->s));
+    CK(hipStreamSynchronize(h->s));
+    return PLES_MG_OK;
+}
+
+// metric face boxes of the reference shapes (dir of metric m = m/3)
+void face_box(const Geo& G, int dir, int& i0, int& i1, int& j0, int& j1, int& k0, int& k1) {
+    i0 = 1; i1 = G.nx; j0 = 1; j1 = G.ny; k0 = 1; k1 = G.nz;
+    if (dir == 0) i0 = 0;
+    if (dir == 1) j0 = 0;
+    if (dir == 2) k0 = 0;
+}
+
+Nodes nodes_of(const H* h) {
+    Nodes N;
+    N.x = h->nodes[0]; N.y = h->nodes[1]; N.z = h->nodes[2];
+    N.nx = h->jx; N.ny = h->jy; N.nz = h->jz;
+    N.ip = h->ip; N.jp = h->jp; N.kp = h->kp;
+    return N;
+}
+
+// FP64 metrics of level n into g[9] (ghosted, zero elsewhere); optional face vectors
+int compute_metrics(H* h, int n, double* const g[9], double* const v[9]) {
+    const Geo& G = h->lv[n].G;
+    const int st = 1 << n;
+    const Nodes N = nodes_of(h);
+    for (int dir = 0; dir < 3; ++dir) {
+        const int na = (dir == 0) ? G.nx + 1 : G.nx;
+        const int nbv = (dir == 1) ? G.ny + 1 : G.ny;
+        const int nc = (dir == 2) ? G.nz + 1 : G.nz;
+        dim3 block(std::min(64, pow2ceil(na)), 4);
+        dim3 grid((na + block.x - 1) / block.x, (nbv + 3) / 4, nc);
+        hipLaunchKernelGGL((k_metrics<double>), grid, block, 0, h->s, N, G, st, dir, g[3 * dir],
+                           g[3 * dir + 1], g[3 * dir + 2], v ? v[3 * dir] : nullptr, v ? v[3 * dir + 1] : nullptr,
+                           v ? v[3 * dir + 2] : nullptr);
+    }
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+int level_factors(H* h, Level& L) {
+    const ColLaunch cl = col_launch(L.G);
+    if (L.f32)
+        hipLaunchKernelGGL((k_line_factors<float>), cl.grid, cl.block, 0, h->s, L.G, met<float>(L), 1.0 / L.pot,
+                           L.invm, L.cpr);
+    else
+        hipLaunchKernelGGL((k_line_factors<double>), cl.grid, cl.block, 0, h->s, L.G, met<double>(L),
+                           1.0 / L.pot, L.invm, L.cpr);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+int check_grid(const ples_mg_grid* g) {
+    if (!g || !g->x || !g->y || !g->z) return PLES_MG_ERR_ARG;
+    if (g->jx < 1 || g->jy < 1 || g->jz < 1 || g->nlevel < 1 || g->nlevel > 16) return PLES_MG_ERR_BAD_DIMS;
+    if (g->jp != 1) return PLES_MG_ERR_UNSUPPORTED;  // the reference supports eta walls only
+    if ((g->ip != 0 && g->ip != 1) || (g->kp != 0 && g->kp != 1)) return PLES_MG_ERR_ARG;
+    const int st = 1 << (g->nlevel - 1);
+    if (g->jx % st || g->jy % st || g->jz % st) return PLES_MG_ERR_BAD_DIMS;
+    const int cx = g->jx / st, cy = g->jy / st, cz = g->jz / st;
+    if (cy < 2 || (g->ip == 1 && cx < 2) || (g->kp == 1 && cz < 2)) return PLES_MG_ERR_BAD_DIMS;
+    return PLES_MG_OK;
+}
+
+size_t estimate(const ples_mg_grid* g, const ples_mg_opts* o) {
+    size_t tot = 0;
+    const bool mixed = o && o->precision == PLES_MG_MIXED;
+    for (int n = 0; n < g->nlevel; ++n) {
+        const int st = 1 << n;
+        const size_t cells = (size_t)(g->jx / st + 2) * (g->jy / st + 2) * (g->jz / st + 2);
+        const bool f32 = mixed && n < g->nlevel - 1;
+        const size_t sp = f32 ? 4 : 8;
+        tot += cells * 9 * sp;                  // V-cycle metrics
+        if (n == 0 && f32) tot += cells * 9 * 8;  // FP64 outer metrics
+        tot += cells * 8;                       // invm, cpr
+        tot += cells * sp * 2;                  // ping-pong
+        if (n > 0) tot += cells * sp;           // rhs
+        if (n < g->nlevel - 1) tot += cells * sp;  // residual
+    }
+    const size_t c0 = (size_t)(g->jx + 2) * (g->jy + 2) * (g->jz + 2);
+    tot += c0 * 8 * 7;                                             // x, b, r, Z[2], W[2]
+    tot += (size_t)(g->jx + 1) * (g->jy + 1) * (g->jz + 1) * 24;   // nodes
+    return tot;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+void ples_mg_default_opts(ples_mg_opts* o) {
+    if (!o) return;
+    o->precision = PLES_MG_FP64;
+    o->nu_pre = 2;
+    o->nu_post = 2;
+    o->nu_coarse = 24;
+    o->omega = 0.8;
+    o->krylov = 1;
+    o->stop_mode = PLES_MG_STOP_REL_L2;
+    o->project_rhs = 1;
+    o->halo_passes = 6;
+    o->device = 0;
+}
+
+const char* ples_mg_status_string(int s) {
+    switch (s) {
+        case PLES_MG_OK: return "ok";
+        case PLES_MG_NOT_CONVERGED: return "not converged";
+        case PLES_MG_ERR_NONFINITE: return "non-finite value";
+        case PLES_MG_ERR_OOM: return "out of device memory";
+        case PLES_MG_ERR_BAD_DIMS: return "bad dimensions";
+        case PLES_MG_ERR_HIP: return "HIP error";
+        case PLES_MG_ERR_ARG: return "bad argument";
+        case PLES_MG_ERR_UNSUPPORTED: return "unsupported configuration";
+        case PLES_MG_ERR_BUSY: return "not admitted";
+        default: return "unknown";
+    }
+}
+
+size_t ples_mg_estimate_bytes(const ples_mg_grid* g, const ples_mg_opts* o) {
+    if (check_grid(g) != PLES_MG_OK) return 0;
+    return estimate(g, o);
+}
+
+int ples_mg_destroy(ples_mg_handle* h) {
+    if (!h) return PLES_MG_OK;
+    (void)hipSetDevice(h->dev);
+    if (h->s) (void)hipStreamSynchronize(h->s);
+    for (auto& r : h->recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto& a : h->allocs) (void)hipFree(a.first);
+    if (h->Sh) (void)hipHostFree(h->Sh);
+    if (h->t0) (void)hipEventDestroy(h->t0);
+    if (h->t1) (void)hipEventDestroy(h->t1);
+    if (h->s) (void)hipStreamDestroy(h->s);
+    delete h;
+    return PLES_MG_OK;
+}
+
+static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
+    ples_mg_opts o;
+    ples_mg_default_opts(&o);
+    if (oin) o = *oin;
+    if (o.precision != PLES_MG_FP64 && o.precision != PLES_MG_MIXED) return PLES_MG_ERR_ARG;
+    if (o.nu_pre < 0 || o.nu_post < 0 || o.nu_coarse < 0 || !(o.omega > 0.0 && o.omega < 2.0))
+        return PLES_MG_ERR_ARG;
+    h->o = o;
+    h->dev = o.device;
+    CK(hipSetDevice(h->dev));
+    CK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    CK(hipEventCreate(&h->t0));
+    CK(hipEventCreate(&h->t1));
+    CK(hipHostMalloc((void**)&h->Sh, S_COUNT * sizeof(double), hipHostMallocDefault));
+    std::memset(h->Sh, 0, S_COUNT * sizeof(double));
+    h->jx = g->jx; h->jy = g->jy; h->jz = g->jz;
+    h->ip = g->ip; h->jp = g->jp; h->kp = g->kp;
+    h->L = g->nlevel;
+    const bool mixed = o.precision == PLES_MG_MIXED;
+    // nodes
+    const size_t nn = (size_t)(g->jx + 1) * (g->jy + 1) * (g->jz + 1);
+    const double* src[3] = {g->x, g->y, g->z};
+    for (int c = 0; c < 3; ++c) {
+        CKS(dalloc(h, (void**)&h->nodes[c], nn * 8));
+        CK(hipMemcpyAsync(h->nodes[c], src[c], nn * 8, hipMemcpyHostToDevice, h->s));
+    }
+    h->lv.resize(h->L);
+    for (int n = 0; n < h->L; ++n) {
+        Level& L = h->lv[n];
+        const int st = 1 << n;
+        L.G = make_geo(g->jx / st, g->jy / st, g->jz / st, g->ip, g->jp, g->kp);
+        L.pot = (double)st;
+        L.f32 = mixed && n < h->L - 1;
+        const size_t cells = (size_t)L.G.n;
+        for (int q = 0; q < 9; ++q) CKS(dalloc(h, (void**)&L.g64[q], cells * 8));
+        CKS(compute_metrics(h, n, L.g64, nullptr));
+        if (n == 0 && g->g[0]) {
+            // caller-provided level-1 metrics in the reference shapes
+            double* staging = nullptr;
+            CKS(dalloc(h, (void**)&staging, cells * 8));
+            for (int q = 0; q < 9; ++q) {
+                if (!g->g[q]) return PLES_MG_ERR_ARG;
+                int i0, i1, j0, j1, k0, k1;
+                face_box(L.G, q / 3, i0, i1, j0, j1, k0, k1);
+                CKS(pack_box<double>(h, L.G, i0, i1, j0, j1, k0, k1, g->g[q], L.g64[q], staging));
+            }
+            CK(hipStreamSynchronize(h->s));
+            dfree(h, staging);
+        }
+        if (L.f32) {
+            for (int q = 0; q < 9; ++q) {
+                CKS(dalloc(h, (void**)&L.g32[q], cells * 4));
+                const int nb = (int)std::min<size_t>((cells + 255) / 256, 8192);
+                hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(256), 0, h->s, (long)cells,
+                                   (const double*)L.g64[q], L.g32[q]);
+            }
+            CK(hipGetLastError());
+            if (n > 0) {
+                CK(hipStreamSynchronize(h->s));
+                for (int q = 0; q < 9; ++q) {
+                    dfree(h, L.g64[q]);
+                    L.g64[q] = nullptr;
+                }
+            }
+        }
+        CKS(dalloc(h, (void**)&L.invm, cells * 4));
+        CKS(dalloc(h, (void**)&L.cpr, cells * 4));
+        CKS(level_factors(h, L));
+        const size_t sp = L.f32 ? 4 : 8;
+        CKS(dalloc(h, &L.p[0], cells * sp));
+        CKS(dalloc(h, &L.p[1], cells * sp));
+        if (n > 0) CKS(dalloc(h, &L.b, cells * sp));
+        if (n < h->L - 1) CKS(dalloc(h, &L.r, cells * sp));
+    }
+    const size_t c0 = (size_t)h->lv[0].G.n;
+    CKS(dalloc(h, (void**)&h->x, c0 * 8));
+    CKS(dalloc(h, (void**)&h->bvec, c0 * 8));
+    CKS(dalloc(h, (void**)&h->rvec, c0 * 8));
+    for (int q = 0; q < 2; ++q) {
+        CKS(dalloc(h, (void**)&h->Z[q], c0 * 8));
+        CKS(dalloc(h, (void**)&h->W[q], c0 * 8));
+    }
+    const size_t csz[6] = {(size_t)g->jy * g->jz, (size_t)g->jy * g->jz, (size_t)g->jx * g->jz,
+                           (size_t)g->jx * g->jz, (size_t)g->jx * g->jy, (size_t)g->jx * g->jy};
+    for (int q = 0; q < 6; ++q) CKS(dalloc(h, (void**)&h->cs[q], csz[q] * 8));
+    CKS(dalloc(h, (void**)&h->S, S_COUNT * sizeof(double)));
+    h->part_n = std::max(col_launch(h->lv[0].G).nblocks, 1);
+    CKS(dalloc(h, (void**)&h->part, (size_t)h->part_n * 4 * sizeof(double)));
+    CK(hipStreamSynchronize(h->s));
+    return PLES_MG_OK;
+}
+
+int ples_mg_create(const ples_mg_grid* g, const ples_mg_opts* o, ples_mg_handle** out) {
+    if (!out) return PLES_MG_ERR_ARG;
+    *out = nullptr;
+    CKS(check_grid(g));
+    H* h = new H();
+    const int st = create_impl(g, o, h);
+    if (st != PLES_MG_OK) {
+        ples_mg_destroy(h);
+        return st;
+    }
+    *out = h;
+    return PLES_MG_OK;
+}
+
+void* ples_mg_stream(ples_mg_handle* h) { return h ? (void*)h->s : nullptr; }
+size_t ples_mg_device_bytes(const ples_mg_handle* h) { return h ? h->bytes : 0; }
+
+// ---------------------------------------------------------------- rhs helpers
+static int project_rhs(H* h, double bflux) {
+    const Level& L0 = h->lv[0];
+    const ColLaunch cl = col_launch(L0.G);
+    hipLaunchKernelGGL(k_sum, cl.grid, cl.block, 0, h->s, L0.G, (const double*)h->bvec, h->part, cl.nblocks);
+    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1), 0, h->s, (const double*)h->part, cl.nblocks, 2, h->S);
+    const double ncell = (double)L0.ncell();
+    if (h->o.project_rhs)
+        hipLaunchKernelGGL(k_sub_mean, cl.grid, cl.block, 0, h->s, L0.G, h->bvec, (const double*)h->S, bflux,
+                           ncell);
+    CK(hipGetLastError());
+    CKS(read_S(h));
+    h->last_removed = h->o.project_rhs ? (h->Sh[S_SUM] - bflux) / ncell : 0.0;
+    return PLES_MG_OK;
+}
+
+static double boundary_flux(const H* h, const double* cs[6], double dt) {
+    double s = 0.0;
+    if (h->ip == 1 && cs[0] && cs[1])
+        for (long q = 0; q < (long)h->jy * h->jz; ++q) s += cs[1][q] - cs[0][q];
+    if (h->jp == 1 && cs[2] && cs[3])
+        for (long q = 0; q < (long)h->jx * h->jz; ++q) s += cs[3][q] - cs[2][q];
+    if (h->kp == 1 && cs[4] && cs[5])
+        for (long q = 0; q < (long)h->jx * h->jy; ++q) s += cs[5][q] - cs[4][q];
+    return s / dt;
+}
+
+static int upload_cs(H* h, const double* cs[6]) {
+    const size_t csz[6] = {(size_t)h->jy * h->jz, (size_t)h->jy * h->jz, (size_t)h->jx * h->jz,
+                           (size_t)h->jx * h->jz, (size_t)h->jx * h->jy, (size_t)h->jx * h->jy};
+    bool any = false;
+    for (int q = 0; q < 6; ++q) {
+        if (cs[q]) {
+            any = true;
+            CK(hipMemcpyAsync(h->cs[q], cs[q], csz[q] * 8, hipMemcpyHostToDevice, h->s));
+        } else {
+            CK(hipMemsetAsync(h->cs[q], 0, csz[q] * 8, h->s));
+        }
+    }
+    h->have_cs = any;
+    return PLES_MG_OK;
+}
+
+static int upload_fi(H* h, const double* fi, int deepl) {
+    const Geo& G = h->lv[0].G;
+    // fi(0:jx+1,0:jy+1,1-deepl:...) -> planes k = 0..jz+1 start at offset (deepl-1) planes
+    CK(hipMemcpyAsync(h->x, fi + (size_t)(deepl - 1) * G.sz, (size_t)G.n * 8, hipMemcpyHostToDevice, h->s));
+    return PLES_MG_OK;
+}
+
+static int download_fi(H* h, double* fi, int deepl) {
+    const Geo& G = h->lv[0].G;
+    CK(hipMemcpyAsync(fi + (size_t)(deepl - 1) * G.sz, h->x, (size_t)G.n * 8, hipMemcpyDeviceToHost, h->s));
+    CK(hipStreamSynchronize(h->s));
+    return PLES_MG_OK;
+}
+
+int ples_mg_solve(ples_mg_handle* h, double* fi, int deepl, int deepr, const double* rhs, const double* cs1,
+                  const double* cs2, const double* cs3, const double* cs4, const double* cs5, const double* cs6,
+                  double dt, double eps, int ficycle, ples_mg_result* res) {
+    if (!h || !fi || !rhs || deepl < 1 || deepr < 1 || !(dt > 0.0) || !(eps > 0.0) || ficycle < 0)
+        return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    const Geo& G = h->lv[0].G;
+    const double* cs[6] = {cs1, cs2, cs3, cs4, cs5, cs6};
+    h->dt = dt;
+    CKS(upload_cs(h, cs));
+    CKS(upload_fi(h, fi, deepl));
+    CKS(pack_box<double>(h, G, 1, G.nx, 1, G.ny, 1, G.nz, rhs, h->bvec, h->W[1]));
+    CKS(project_rhs(h, boundary_flux(h, cs, dt)));
+    const int st = solve_core(h, eps, ficycle, res);
+    if (st != PLES_MG_OK && st != PLES_MG_NOT_CONVERGED && st != PLES_MG_ERR_NONFINITE) return st;
+    CKS(download_fi(h, fi, deepl));
+    return st;
+}
+
+int ples_mg_residual(ples_mg_handle* h, const double* fi, int deepl, int deepr, const double* rhs,
+                     const double* cs1, const double* cs2, const double* cs3, const double* cs4, const double* cs5,
+                     const double* cs6, double dt, double* r) {
+    if (!h || !fi || !rhs || !r || deepl < 1 || deepr < 1 || !(dt > 0.0)) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    const Level& L0 = h->lv[0];
+    const Geo& G = L0.G;
+    const double* cs[6] = {cs1, cs2, cs3, cs4, cs5, cs6};
+    h->dt = dt;
+    CKS(upload_cs(h, cs));
+    double* p = h->Z[0];
+    double* b = h->Z[1];
+    double* out = h->W[0];
+    CK(hipMemcpyAsync(p, fi + (size_t)(deepl - 1) * G.sz, (size_t)G.n * 8, hipMemcpyHostToDevice, h->s));
+    CKS(pack_box<double>(h, G, 1, G.nx, 1, G.ny, 1, G.nz, rhs, b, h->W[1]));
+    CKS(halo64(h, p, true));
+    const ColLaunch cl = col_launch(G);
+    hipLaunchKernelGGL((k_resid<0, double, double, double, double>), cl.grid, cl.block, 0, h->s, G,
+                       met<double>(L0), (const double*)p, (const double*)b, out, 1.0, (const double*)nullptr,
+                       (const double*)nullptr, (double*)nullptr, 0);
+    CK(hipGetLastError());
+    return unpack_box<double>(h, G, 1, G.nx, 1, G.ny, 1, G.nz, out, r, h->W[1]);
+}
+
+int ples_mg_metrica(ples_mg_handle* h, double* const g[9], double* const csv[9], double* giac) {
+    if (!h) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    const Geo& G = h->lv[0].G;
+    const size_t cells = (size_t)G.n;
+    std::vector<double*> tmp;
+    double* gg[9];
+    double* vv[9];
+    for (int q = 0; q < 9; ++q) {
+        CKS(dalloc(h, (void**)&gg[q], cells * 8));
+        CKS(dalloc(h, (void**)&vv[q], cells * 8));
+        tmp.push_back(gg[q]);
+        tmp.push_back(vv[q]);
+    }
+    double* gi = nullptr;
+    CKS(dalloc(h, (void**)&gi, cells * 8));
+    tmp.push_back(gi);
+    CKS(compute_metrics(h, 0, gg, vv));
+    {
+        dim3 block(64, 4);
+        dim3 grid((G.nx + 63) / 64, (G.ny + 3) / 4, G.nz);
+        hipLaunchKernelGGL(k_giac, grid, block, 0, h->s, nodes_of(h), G, gi);
+        CK(hipGetLastError());
+    }
+    int st = PLES_MG_OK;
+    for (int q = 0; q < 9 && st == PLES_MG_OK; ++q) {
+        int i0, i1, j0, j1, k0, k1;
+        face_box(G, q / 3, i0, i1, j0, j1, k0, k1);
+        if (g && g[q]) st = unpack_box<double>(h, G, i0, i1, j0, j1, k0, k1, gg[q], g[q], h->W[1]);
+        if (st == PLES_MG_OK && csv && csv[q])
+            st = unpack_box<double>(h, G, i0, i1, j0, j1, k0, k1, vv[q], csv[q], h->W[1]);
+    }
+    if (st == PLES_MG_OK && giac) st = unpack_box<double>(h, G, 1, G.nx, 1, G.ny, 1, G.nz, gi, giac, h->W[1]);
+    (void)hipStreamSynchronize(h->s);
+    for (double* p : tmp) {
+        dfree(h, p);
+    }
+    return st;
+}
+
+int ples_mg_level_metrics(ples_mg_handle* h, int level, double* const g[9]) {
+    if (!h || !g || level < 1 || level > h->L) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    const int n = level - 1;
+    const Geo& G = h->lv[n].G;
+    const size_t cells = (size_t)G.n;
+    double* gg[9];
+    for (int q = 0; q < 9; ++q) CKS(dalloc(h, (void**)&gg[q], cells * 8));
+    int st = compute_metrics(h, n, gg, nullptr);
+    if (st == PLES_MG_OK && n == 0 && h->lv[0].g64[0]) {
+        for (int q = 0; q < 9; ++q)
+            (void)hipMemcpyAsync(gg[q], h->lv[0].g64[q], cells * 8, hipMemcpyDeviceToDevice, h->s);
+    }
+    for (int q = 0; q < 9 && st == PLES_MG_OK; ++q) {
+        if (!g[q]) continue;
+        if (hipMemcpyAsync(g[q], gg[q], cells * 8, hipMemcpyDeviceToHost, h->s) != hipSuccess) st = PLES_MG_ERR_HIP;
+    }
+    (void)hipStreamSynchronize(h->s);
+    for (int q = 0; q < 9; ++q) {
+        dfree(h, gg[q]);
+    }
+    return st;
+}
+
+int ples_mg_diver(ples_mg_handle* h, const double* uc, const double* vc, const double* wc, double dt, double* rhs) {
+    if (!h || !uc || !vc || !wc || !rhs || !(dt > 0.0)) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    const Geo& G = h->lv[0].G;
+    double* du = h->Z[0];
+    double* dv = h->Z[1];
+    double* dw = h->W[0];
+    double* out = h->rvec;
+    CKS(pack_box<double>(h, G, 0, G.nx, 1, G.ny, 1, G.nz, uc, du, h->W[1]));
+    CK(hipStreamSynchronize(h->s));
+    CKS(pack_box<double>(h, G, 1, G.nx, 0, G.ny, 1, G.nz, vc, dv, h->W[1]));
+    CK(hipStreamSynchronize(h->s));
+    CKS(pack_box<double>(h, G, 1, G.nx, 1, G.ny, 0, G.nz, wc, dw, h->W[1]));
+    const ColLaunch cl = col_launch(G);
+    hipLaunchKernelGGL(k_diver, cl.grid, cl.block, 0, h->s, G, (const double*)du, (const double*)dv,
+                       (const double*)dw, dt, out);
+    CK(hipGetLastError());
+    return unpack_box<double>(h, G, 1, G.nx, 1, G.ny, 1, G.nz, out, rhs, h->W[1]);
+}
+
+int ples_mg_gradie(ples_mg_handle* h, const double* fi, int deepl, int deepr, double dt, double* gra1,
+                   double* gra2, double* gra3, double* cgra1, double* cgra2, double* cgra3) {
+    if (!h || !fi || deepl < 1 || deepr < 1) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    const Level& L0 = h->lv[0];
+    const Geo& G = L0.G;
+    const size_t cells = (size_t)G.n;
+    std::vector<double*> tmp;
+    double* gg[9];
+    double* vv[9];
+    double* gi = nullptr;
+    double* outs[6];
+    for (int q = 0; q < 9; ++q) {
+        CKS(dalloc(h, (void**)&gg[q], cells * 8));
+        CKS(dalloc(h, (void**)&vv[q], cells * 8));
+        tmp.push_back(gg[q]);
+        tmp.push_back(vv[q]);
+    }
+    CKS(dalloc(h, (void**)&gi, cells * 8));
+    tmp.push_back(gi);
+    for (int q = 0; q < 6; ++q) {
+        CKS(dalloc(h, (void**)&outs[q], cells * 8));
+        tmp.push_back(outs[q]);
+    }
+    CKS(compute_metrics(h, 0, gg, vv));
+    {
+        dim3 block(64, 4);
+        dim3 grid((G.nx + 63) / 64, (G.ny + 3) / 4, G.nz);
+        hipLaunchKernelGGL(k_giac, grid, block, 0, h->s, nodes_of(h), G, gi);
+    }
+    // gradie works with the metrics the solver uses on level 1
+    Met<double> m = met<double>(L0);
+    double* p = h->Z[0];
+    CK(hipMemcpyAsync(p, fi + (size_t)(deepl - 1) * G.sz, cells * 8, hipMemcpyHostToDevice, h->s));
+    CsvP V;
+    for (int q = 0; q < 9; ++q) V.v[q] = vv[q];
+    {
+        dim3 block(64, 4);
+        dim3 grid((G.nx + 1 + 63) / 64, (G.ny + 1 + 3) / 4, G.nz + 1);
+        hipLaunchKernelGGL(k_gradie, grid, block, 0, h->s, G, (const double*)p, V, (const double*)gi, m, dt,
+                           outs[0], outs[1], outs[2], outs[3], outs[4], outs[5]);
+        CK(hipGetLastError());
+    }
+    int st = PLES_MG_OK;
+    double* dst[6] = {gra1, gra2, gra3, cgra1, cgra2, cgra3};
+    for (int q = 0; q < 6 && st == PLES_MG_OK; ++q) {
+        if (!dst[q]) continue;
+        int i0 = 1, i1 = G.nx, j0 = 1, j1 = G.ny, k0 = 1, k1 = G.nz;
+        if (q == 3) i0 = 0;
+        if (q == 4) j0 = 0;
+        if (q == 5) k0 = 0;
+        st = unpack_box<double>(h, G, i0, i1, j0, j1, k0, k1, outs[q], dst[q], h->W[1]);
+    }
+    (void)hipStreamSynchronize(h->s);
+    for (double* t : tmp) {
+        dfree(h, t);
+    }
+    return st;
+}
+
+int ples_mg_case_rhs(ples_mg_handle* h, uint64_t seed, double dt) {
+    if (!h || !(dt > 0.0)) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    const Geo& G = h->lv[0].G;
+    const size_t cells = (size_t)G.n;
+    double* gg[9];
+    double* vv[9];
+    for (int q = 0; q < 9; ++q) {
+        CKS(dalloc(h, (void**)&gg[q], cells * 8));
+        CKS(dalloc(h, (void**)&vv[q], cells * 8));
+    }
+    int st = compute_metrics(h, 0, gg, vv);
+    CsvP V;
+    for (int q = 0; q < 9; ++q) V.v[q] = vv[q];
+    const ColLaunch cl = col_launch(G);
+    if (st == PLES_MG_OK) {
+        hipLaunchKernelGGL(k_case_rhs, cl.grid, cl.block, 0, h->s, G, V, seed, dt, h->bvec, h->part, cl.nblocks);
+        if (hipGetLastError() != hipSuccess) st = PLES_MG_ERR_HIP;
+    }
+    h->dt = dt;
+    const double* none[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (st == PLES_MG_OK) st = upload_cs(h, none);
+    if (st == PLES_MG_OK) st = project_rhs(h, 0.0);
+    (void)hipStreamSynchronize(h->s);
+    for (int q = 0; q < 9; ++q) {
+        dfree(h, gg[q]);
+        dfree(h, vv[q]);
+    }
+    return st;
+}
+
+int ples_mg_zero_fi(ples_mg_handle* h) {
+    if (!h) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    CK(hipMemsetAsync(h->x, 0, (size_t)h->lv[0].G.n * 8, h->s));
+    CK(hipStreamSynchronize(h->s));
+    return PLES_MG_OK;
+}
+
+int ples_mg_perturb_fi(ples_mg_handle* h, uint64_t seed, double rel) {
+    if (!h) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    const Level& L0 = h->lv[0];
+    const ColLaunch cl = col_launch(L0.G);
+    hipLaunchKernelGGL(k_sum, cl.grid, cl.block, 0, h->s, L0.G, (const double*)h->x, h->part, cl.nblocks);
+    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1), 0, h->s, (const double*)h->part, cl.nblocks, 2, h->S);
+    hipLaunchKernelGGL(k_perturb, cl.grid, cl.block, 0, h->s, L0.G, h->x, (const double*)h->S, seed, rel,
+                       (double)L0.ncell());
+    CK(hipGetLastError());
+    CK(hipStreamSynchronize(h->s));
+    return PLES_MG_OK;
+}
+
+int ples_mg_solve_resident(ples_mg_handle* h, double eps, int maxcycles, ples_mg_result* res) {
+    if (!h || !(eps > 0.0) || maxcycles < 0) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    return solve_core(h, eps, maxcycles, res);
+}
+
+int ples_mg_get_fi(ples_mg_handle* h, double* fi) {
+    if (!h || !fi) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    return download_fi(h, fi, 1);
+}
+
+int ples_mg_set_fi(ples_mg_handle* h, const double* fi) {
+    if (!h || !fi) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    CKS(upload_fi(h, fi, 1));
+    CK(hipStreamSynchronize(h->s));
+    return PLES_MG_OK;
+}
+
+int ples_mg_get_rhs(ples_mg_handle* h, double* rhs) {
+    if (!h || !rhs) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    const Geo& G = h->lv[0].G;
+    return unpack_box<double>(h, G, 1, G.nx, 1, G.ny, 1, G.nz, h->bvec, rhs, h->W[1]);
+}
+
+int ples_mg_set_rhs(ples_mg_handle* h, const double* rhs) {
+    if (!h || !rhs) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    const Geo& G = h->lv[0].G;
+    CKS(pack_box<double>(h, G, 1, G.nx, 1, G.ny, 1, G.nz, rhs, h->bvec, h->W[1]));
+    CK(hipStreamSynchronize(h->s));
+    return PLES_MG_OK;
+}
+
+int ples_mg_set_timing(ples_mg_handle* h, int on) {
+    if (!h) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    CK(hipStreamSynchronize(h->s));
+    for (auto& r : h->recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    h->recs.clear();
+    h->timing = on != 0;
+    return PLES_MG_OK;
+}
+
+int ples_mg_kernel_stats(ples_mg_handle* h, int fam, long* launches, double* ms, double* bytes) {
+    if (!h || fam < 0 || fam >= PLES_MG_K_COUNT) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    CK(hipStreamSynchronize(h->s));
+    long n = 0;
+    double t = 0.0, by = 0.0;
+    for (auto& r : h->recs) {
+        const bool match = r.fam == fam || (fam == PLES_MG_K_SMOOTH && r.fam == PLES_MG_K_SMOOTH_FINE) ||
+                           (fam == PLES_MG_K_RESID && r.fam == PLES_MG_K_RESID_FINE);
+        if (!match) continue;
+        float e = 0.f;
+        if (hipEventElapsedTime(&e, r.a, r.b) == hipSuccess) t += e;
+        by += r.bytes;
+        ++n;
+    }
+    if (launches) *launches = n;
+    if (ms) *ms = t;
+    if (bytes) *bytes = by;
+    return PLES_MG_OK;
+}
+
+}  // extern "C"

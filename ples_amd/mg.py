@@ -1,0 +1,327 @@
+"""Python host binding of libples_mg.so (the MI355X pressure multigrid).
+
+Mirrors the reference's solver interface: ``Solver.multi(fi, rhs, cs, dt,
+eps, ficycle)`` plays the role of ``call multi(eps, ficycle, nlevel, ...)``
+(fsrc/multigrid.f03:42-48, called at fsrc/stratiParticle.f03:2881-2887):
+the pressure ``fi`` is the warm start and is overwritten with the solution
+including its ghost layers.
+
+Arrays are numpy float64 in the reference's Fortran memory order, seen from
+numpy as C-order arrays with reversed axes:
+  fi   (jz+2, jy+2, jx+2)    == fi(0:jx+1, 0:jy+1, 0:jz+1)
+  rhs  (jz, jy, jx)          == rhs(1:jx, 1:jy, 1:jz)
+  g11..g13 (jz, jy, jx+1), g21..g23 (jz, jy+1, jx), g31..g33 (jz+1, jy, jx)
+  cs1, cs2 (jz, jy); cs3, cs4 (jz, jx); cs5, cs6 (jy, jx)
+  nodes x, y, z (jz+1, jy+1, jx+1)
+
+There is no CPU fallback: if the HIP library is missing or no GPU is
+present, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libples_mg.so")
+
+OK = 0
+NOT_CONVERGED = 1
+FP64 = 0
+MIXED = 1
+STOP_REL_L2 = 0
+STOP_ABS_MAX = 1
+
+K_SMOOTH, K_SMOOTH_FINE, K_RESID, K_RESID_FINE, K_OUTER_APPLY, K_OUTER_UPDATE, K_TRANSFER, K_HALO = range(8)
+KERNEL_FAMILIES = {
+    "smooth": K_SMOOTH, "smooth_fine": K_SMOOTH_FINE, "resid": K_RESID, "resid_fine": K_RESID_FINE,
+    "outer_apply": K_OUTER_APPLY, "outer_update": K_OUTER_UPDATE, "transfer": K_TRANSFER, "halo": K_HALO,
+}
+
+_dp = C.POINTER(C.c_double)
+
+
+class _Grid(C.Structure):
+    _fields_ = [("jx", C.c_int), ("jy", C.c_int), ("jz", C.c_int),
+                ("ip", C.c_int), ("jp", C.c_int), ("kp", C.c_int), ("nlevel", C.c_int),
+                ("x", _dp), ("y", _dp), ("z", _dp), ("g", _dp * 9)]
+
+
+class _Opts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("nu_pre", C.c_int), ("nu_post", C.c_int),
+                ("nu_coarse", C.c_int), ("omega", C.c_double), ("krylov", C.c_int),
+                ("stop_mode", C.c_int), ("project_rhs", C.c_int), ("halo_passes", C.c_int),
+                ("device", C.c_int)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("status", C.c_int), ("cycles", C.c_int), ("rel_res", C.c_double),
+                ("abs_res", C.c_double), ("rhs_mean_removed", C.c_double), ("seconds", C.c_double)]
+
+
+@dataclass
+class Result:
+    status: int
+    cycles: int
+    rel_res: float
+    abs_res: float
+    rhs_mean_removed: float
+    seconds: float
+
+    @property
+    def ok(self) -> bool:
+        return self.status == OK
+
+
+class PlesMGError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        super().__init__(f"{what}: {status_string(status)} ({status})")
+
+
+_lib = None
+
+
+def lib():
+    """Load libples_mg.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m ples_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.ples_mg_default_opts.argtypes = [C.POINTER(_Opts)]
+        L.ples_mg_status_string.argtypes = [C.c_int]
+        L.ples_mg_status_string.restype = C.c_char_p
+        L.ples_mg_create.argtypes = [C.POINTER(_Grid), C.POINTER(_Opts), C.POINTER(C.c_void_p)]
+        L.ples_mg_destroy.argtypes = [C.c_void_p]
+        L.ples_mg_solve.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, _dp] + [_dp] * 6 + \
+            [C.c_double, C.c_double, C.c_int, C.POINTER(_Result)]
+        L.ples_mg_residual.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, _dp] + [_dp] * 6 + \
+            [C.c_double, _dp]
+        L.ples_mg_metrica.argtypes = [C.c_void_p, C.POINTER(_dp), C.POINTER(_dp), _dp]
+        L.ples_mg_level_metrics.argtypes = [C.c_void_p, C.c_int, C.POINTER(_dp)]
+        L.ples_mg_diver.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, _dp]
+        L.ples_mg_gradie.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_double] + [_dp] * 6
+        L.ples_mg_case_rhs.argtypes = [C.c_void_p, C.c_uint64, C.c_double]
+        L.ples_mg_zero_fi.argtypes = [C.c_void_p]
+        L.ples_mg_perturb_fi.argtypes = [C.c_void_p, C.c_uint64, C.c_double]
+        L.ples_mg_solve_resident.argtypes = [C.c_void_p, C.c_double, C.c_int, C.POINTER(_Result)]
+        for f in ("ples_mg_get_fi", "ples_mg_set_fi", "ples_mg_get_rhs", "ples_mg_set_rhs"):
+            getattr(L, f).argtypes = [C.c_void_p, _dp]
+        L.ples_mg_stream.argtypes = [C.c_void_p]
+        L.ples_mg_stream.restype = C.c_void_p
+        L.ples_mg_device_bytes.argtypes = [C.c_void_p]
+        L.ples_mg_device_bytes.restype = C.c_size_t
+        L.ples_mg_estimate_bytes.argtypes = [C.POINTER(_Grid), C.POINTER(_Opts)]
+        L.ples_mg_estimate_bytes.restype = C.c_size_t
+        L.ples_mg_set_timing.argtypes = [C.c_void_p, C.c_int]
+        L.ples_mg_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def status_string(s: int) -> str:
+    return lib().ples_mg_status_string(s).decode()
+
+
+def _p(a):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("arrays must be C-contiguous float64")
+    return a.ctypes.data_as(_dp)
+
+
+def _check(st: int, what: str):
+    if st != OK:
+        raise PlesMGError(st, what)
+
+
+def default_opts() -> dict:
+    o = _Opts()
+    lib().ples_mg_default_opts(C.byref(o))
+    return {f: getattr(o, f) for f, _ in _Opts._fields_}
+
+
+def _opts(precision="fp64", **kw) -> _Opts:
+    o = _Opts()
+    lib().ples_mg_default_opts(C.byref(o))
+    o.precision = {"fp64": FP64, "mixed": MIXED}[precision] if isinstance(precision, str) else int(precision)
+    for k, v in kw.items():
+        if k == "stop_mode" and isinstance(v, str):
+            v = {"rel_l2": STOP_REL_L2, "abs_max": STOP_ABS_MAX}[v]
+        setattr(o, k, v)
+    return o
+
+
+def _grid(x, y, z, ip, jp, kp, nlevel, metrics=None):
+    jz, jy, jx = (s - 1 for s in x.shape)
+    g = _Grid(jx, jy, jz, ip, jp, kp, nlevel, _p(x), _p(y), _p(z))
+    if metrics is not None:
+        for q in range(9):
+            g.g[q] = _p(metrics[q])
+    return g
+
+
+def estimate_bytes(x, y, z, ip=0, jp=1, kp=0, nlevel=4, precision="fp64", **kw) -> int:
+    g = _grid(x, y, z, ip, jp, kp, nlevel)
+    return int(lib().ples_mg_estimate_bytes(C.byref(g), C.byref(_opts(precision, **kw))))
+
+
+class Solver:
+    """Device-resident multigrid for one grid (one handle, one HIP stream)."""
+
+    def __init__(self, x, y, z, ip=0, jp=1, kp=0, nlevel=4, metrics=None, precision="fp64", **opts):
+        self._x, self._y, self._z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+        self.jz, self.jy, self.jx = (s - 1 for s in self._x.shape)
+        self.ip, self.jp, self.kp, self.nlevel = ip, jp, kp, nlevel
+        self._metrics = None if metrics is None else [np.ascontiguousarray(m, np.float64) for m in metrics]
+        g = _grid(self._x, self._y, self._z, ip, jp, kp, nlevel, self._metrics)
+        o = _opts(precision, **opts)
+        h = C.c_void_p()
+        _check(lib().ples_mg_create(C.byref(g), C.byref(o), C.byref(h)), "ples_mg_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ples_mg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- shapes
+    @property
+    def fi_shape(self):
+        return (self.jz + 2, self.jy + 2, self.jx + 2)
+
+    @property
+    def rhs_shape(self):
+        return (self.jz, self.jy, self.jx)
+
+    def cs_shapes(self):
+        return [(self.jz, self.jy)] * 2 + [(self.jz, self.jx)] * 2 + [(self.jy, self.jx)] * 2
+
+    def _cs(self, cs):
+        if cs is None:
+            return [None] * 6
+        return [None if c is None else np.ascontiguousarray(c, np.float64) for c in cs]
+
+    # ---- drop-in
+    def multi(self, fi, rhs, cs=None, dt=1.0, eps=1e-8, ficycle=1000) -> Result:
+        """Solve in place (fi = warm start -> solution with ghosts)."""
+        assert fi.shape == self.fi_shape and rhs.shape == self.rhs_shape
+        cs = self._cs(cs)
+        r = _Result()
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        st = lib().ples_mg_solve(self._h, _p(fi), 1, 1, _p(rhs), *[_p(c) for c in cs], dt, eps,
+                                 ficycle, C.byref(r))
+        if st not in (OK, NOT_CONVERGED):
+            raise PlesMGError(st, "ples_mg_solve")
+        return Result(r.status, r.cycles, r.rel_res, r.abs_res, r.rhs_mean_removed, r.seconds)
+
+    def residual(self, fi, rhs, cs=None, dt=1.0):
+        cs = self._cs(cs)
+        out = np.zeros(self.rhs_shape)
+        _check(lib().ples_mg_residual(self._h, _p(np.ascontiguousarray(fi)), 1, 1,
+                                      _p(np.ascontiguousarray(rhs, np.float64)),
+                                      *[_p(c) for c in cs], dt, _p(out)), "ples_mg_residual")
+        return out
+
+    def metrica(self):
+        """Level-1 metrics g[9], face vectors csv[9], giac in the reference shapes."""
+        shapes = [(self.jz, self.jy, self.jx + 1)] * 3 + [(self.jz, self.jy + 1, self.jx)] * 3 + \
+                 [(self.jz + 1, self.jy, self.jx)] * 3
+        g = [np.zeros(s) for s in shapes]
+        v = [np.zeros(s) for s in shapes]
+        giac = np.zeros(self.rhs_shape)
+        _check(lib().ples_mg_metrica(self._h, (_dp * 9)(*[_p(a) for a in g]),
+                                     (_dp * 9)(*[_p(a) for a in v]), _p(giac)), "ples_mg_metrica")
+        return g, v, giac
+
+    def level_metrics(self, level: int):
+        """FP64 metrics of level `level` (1-based) in the ghosted layout."""
+        st = 1 << (level - 1)
+        sh = (self.jz // st + 2, self.jy // st + 2, self.jx // st + 2)
+        g = [np.zeros(sh) for _ in range(9)]
+        _check(lib().ples_mg_level_metrics(self._h, level, (_dp * 9)(*[_p(a) for a in g])),
+               "ples_mg_level_metrics")
+        return g
+
+    def diver(self, uc, vc, wc, dt=1.0):
+        out = np.zeros(self.rhs_shape)
+        _check(lib().ples_mg_diver(self._h, _p(np.ascontiguousarray(uc)), _p(np.ascontiguousarray(vc)),
+                                   _p(np.ascontiguousarray(wc)), dt, _p(out)), "ples_mg_diver")
+        return out
+
+    def gradie(self, fi, dt=1.0):
+        jx, jy, jz = self.jx, self.jy, self.jz
+        outs = [np.zeros((jz, jy, jx)) for _ in range(3)] + \
+               [np.zeros((jz, jy, jx + 1)), np.zeros((jz, jy + 1, jx)), np.zeros((jz + 1, jy, jx))]
+        _check(lib().ples_mg_gradie(self._h, _p(np.ascontiguousarray(fi)), 1, 1, dt,
+                                    *[_p(a) for a in outs]), "ples_mg_gradie")
+        return outs
+
+    # ---- device-resident
+    def case_rhs(self, seed: int, dt: float = 1.0):
+        _check(lib().ples_mg_case_rhs(self._h, C.c_uint64(seed), dt), "ples_mg_case_rhs")
+
+    def zero_fi(self):
+        _check(lib().ples_mg_zero_fi(self._h), "ples_mg_zero_fi")
+
+    def perturb_fi(self, seed: int, rel: float):
+        _check(lib().ples_mg_perturb_fi(self._h, C.c_uint64(seed), rel), "ples_mg_perturb_fi")
+
+    def solve_resident(self, eps=1e-10, maxcycles=200) -> Result:
+        r = _Result()
+        st = lib().ples_mg_solve_resident(self._h, eps, maxcycles, C.byref(r))
+        if st not in (OK, NOT_CONVERGED):
+            raise PlesMGError(st, "ples_mg_solve_resident")
+        return Result(r.status, r.cycles, r.rel_res, r.abs_res, r.rhs_mean_removed, r.seconds)
+
+    def get_fi(self):
+        a = np.zeros(self.fi_shape)
+        _check(lib().ples_mg_get_fi(self._h, _p(a)), "ples_mg_get_fi")
+        return a
+
+    def set_fi(self, fi):
+        _check(lib().ples_mg_set_fi(self._h, _p(np.ascontiguousarray(fi, np.float64))), "ples_mg_set_fi")
+
+    def get_rhs(self):
+        a = np.zeros(self.rhs_shape)
+        _check(lib().ples_mg_get_rhs(self._h, _p(a)), "ples_mg_get_rhs")
+        return a
+
+    def set_rhs(self, rhs):
+        _check(lib().ples_mg_set_rhs(self._h, _p(np.ascontiguousarray(rhs, np.float64))), "ples_mg_set_rhs")
+
+    @property
+    def stream(self) -> int:
+        return lib().ples_mg_stream(self._h)
+
+    @property
+    def device_bytes(self) -> int:
+        return int(lib().ples_mg_device_bytes(self._h))
+
+    def set_timing(self, on: bool):
+        _check(lib().ples_mg_set_timing(self._h, int(bool(on))), "ples_mg_set_timing")
+
+    def kernel_stats(self, family) -> tuple[int, float, float]:
+        fam = KERNEL_FAMILIES[family] if isinstance(family, str) else int(family)
+        n, ms, by = C.c_long(0), C.c_double(0), C.c_double(0)
+        _check(lib().ples_mg_kernel_stats(self._h, fam, C.byref(n), C.byref(ms), C.byref(by)),
+               "ples_mg_kernel_stats")
+        return n.value, ms.value, by.value

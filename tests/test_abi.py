@@ -1,0 +1,35 @@
+"""The C-ABI library loads and exports every entry point include/*.h
+declares (no compute calls: this runs without a GPU)."""
+import ctypes
+import glob
+import os
+import re
+
+from ples_amd import mg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    syms = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(ples_\w+)\s*\(", text):
+            syms.add(m.group(1))
+    return syms
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(mg.LIB_PATH)
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in sorted(syms) if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_status_strings_and_defaults():
+    assert mg.status_string(0) == "ok"
+    assert "dimensions" in mg.status_string(4)
+    o = mg.default_opts()
+    assert o["precision"] == mg.FP64 and o["krylov"] == 1 and o["nu_pre"] == 2
