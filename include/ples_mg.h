@@ -73,7 +73,8 @@ typedef struct ples_mg_opts {
     int krylov;      /* 1 = GCR acceleration of the refinement loop (default), 0 = plain */
     int stop_mode;   /* PLES_MG_STOP_REL_L2 (default) or PLES_MG_STOP_ABS_MAX */
     int project_rhs; /* 1 = FP64 compatibility projection of the rhs (default) */
-    int halo_passes; /* red/black passes of the exact wall-ghost solve (default 6) */
+    int halo_passes; /* red/black passes of the exact wall-ghost solve (0 = auto:
+                        6 with eta walls only, 24 with xi/zeta walls) */
     int device;      /* HIP device ordinal (default 0) */
 } ples_mg_opts;
 
@@ -174,8 +175,10 @@ size_t ples_mg_estimate_bytes(const ples_mg_grid* grid, const ples_mg_opts* opts
 #define PLES_MG_K_TRANSFER 6 /* restriction + prolongation */
 #define PLES_MG_K_HALO 7
 #define PLES_MG_K_COUNT 8
-/* Enable (1) / disable (0) event timing of every launch; resets the stats. */
-int ples_mg_set_timing(ples_mg_handle* h, int on);
+#define PLES_MG_TIME_ALL 0xFF
+/* Event timing of every launch of the families whose bit is set in mask
+ * (bit f = family f; PLES_MG_TIME_ALL, 0 = off); resets the stats. */
+int ples_mg_set_timing(ples_mg_handle* h, int mask);
 /* Launch count, summed device milliseconds and algorithmic bytes per kernel
  * family since timing was enabled. */
 int ples_mg_kernel_stats(ples_mg_handle* h, int family, long* launches, double* ms,

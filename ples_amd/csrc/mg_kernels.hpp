@@ -499,9 +499,13 @@ __device__ inline int wrapk(const Geo& G, int k) {
     return k;
 }
 
+// one launch per face set (0 eta, 1 xi, 2 zeta) and colour, so that no
+// ghost read in a launch is written by the same launch
 template <class T, class TM>
-__global__ void k_halo_face(Geo G, Met<TM> M, T* __restrict__ p, int color, BCs bc) {
+__global__ void k_halo_face(Geo G, Met<TM> M, T* __restrict__ p, int color, BCs bc, int set) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (set == 1) t += 2L * G.nx * G.nz;
+    if (set == 2) t += 2L * G.nx * G.nz + (G.ip == 1 ? 2L * G.ny * G.nz : 0L);
     const double an = bc.an_over_dt;
     const T q = (T)0.25;
 #define PV(a, b, c) ((T)p[G.idx(wrapi(G, a), b, wrapk(G, c))])
@@ -535,6 +539,7 @@ __global__ void k_halo_face(Geo G, Met<TM> M, T* __restrict__ p, int color, BCs 
             }
             return;
         }
+        if (set == 0) return;
         t -= nf;
     }
     if (G.ip == 1) {
@@ -566,6 +571,7 @@ __global__ void k_halo_face(Geo G, Met<TM> M, T* __restrict__ p, int color, BCs 
             }
             return;
         }
+        if (set == 1) return;
         t -= nf;
     }
     if (G.kp == 1) {

@@ -60,7 +60,7 @@ struct ples_mg_handle {
     int part_n = 0;
     size_t bytes = 0;
     std::vector<std::pair<void*, size_t>> allocs;
-    bool timing = false;
+    unsigned timing = 0;  // bit f: time kernel family f
     std::vector<Rec> recs;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     double last_removed = 0.0;
@@ -152,15 +152,16 @@ struct KT {
     int fam;
     double bytes;
     hipEvent_t a = nullptr, b = nullptr;
-    KT(H* h_, int fam_, double bytes_) : h(h_), fam(fam_), bytes(bytes_) {
-        if (h->timing) {
+    bool on;
+    KT(H* h_, int fam_, double bytes_) : h(h_), fam(fam_), bytes(bytes_), on(((h_->timing >> fam_) & 1u) != 0) {
+        if (on) {
             (void)hipEventCreate(&a);
             (void)hipEventCreate(&b);
             (void)hipEventRecord(a, h->s);
         }
     }
     ~KT() {
-        if (h->timing) {
+        if (on) {
             (void)hipEventRecord(b, h->s);
             h->recs.push_back(Rec{fam, bytes, a, b});
         }
@@ -183,6 +184,12 @@ Met<double> met<double>(const Level& L) {
 }
 
 // ---------------------------------------------------------------- halo
+long face_set_size(const Geo& G, int set) {
+    if (set == 0) return 2L * G.nx * G.nz;
+    if (set == 1) return G.ip == 1 ? 2L * G.ny * G.nz : 0L;
+    return G.kp == 1 ? 2L * G.nx * G.ny : 0L;
+}
+
 template <class T>
 int halo(H* h, const Level& L, T* p, double an_over_dt, bool with_cs, int passes) {
     const Geo& G = L.G;
@@ -194,13 +201,19 @@ int halo(H* h, const Level& L, T* p, double an_over_dt, bool with_cs, int passes
     if (G.kp == 1) nf += 2L * G.nx * G.ny;
     const long NX = G.nx + 2, NY = G.ny + 2, NZ = G.nz + 2;
     const long ne = 2 * NY * NZ + 2 * NX * NZ + 2 * NX * NY;
-    const double bytes = (double)(nf * 6 * passes + ne * 2) * sizeof(T);
+    const double bytes = (double)(nf * 6 + ne * 2) * passes * sizeof(T);
     KT kt(h, PLES_MG_K_HALO, bytes);
-    for (int ps = 0; ps < passes; ++ps)
-        for (int color = 0; color < 2; ++color)
-            hipLaunchKernelGGL((k_halo_face<T, T>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, h->s, G,
-                               met<T>(L), p, color, bc);
-    hipLaunchKernelGGL((k_halo_edges<T>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, h->s, G, p);
+    // faces (red, black), then the edges that the next pass reads
+    for (int ps = 0; ps < passes; ++ps) {
+        for (int set = 0; set < 3; ++set) {
+            const long nset = face_set_size(G, set);
+            if (nset == 0) continue;
+            for (int color = 0; color < 2; ++color)
+                hipLaunchKernelGGL((k_halo_face<T, T>), dim3((unsigned)((nset + 255) / 256)), dim3(256), 0, h->s,
+                                   G, met<T>(L), p, color, bc, set);
+        }
+        hipLaunchKernelGGL((k_halo_edges<T>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, h->s, G, p);
+    }
     CK(hipGetLastError());
     return PLES_MG_OK;
 }
@@ -219,12 +232,17 @@ int halo64(H* h, double* p, bool with_bc) {
     const long NX = G.nx + 2, NY = G.ny + 2, NZ = G.nz + 2;
     const long ne = 2 * NY * NZ + 2 * NX * NZ + 2 * NX * NY;
     const int passes = std::max(1, h->o.halo_passes);
-    KT kt(h, PLES_MG_K_HALO, (double)(nf * 6 * passes + ne * 2) * 8.0);
-    for (int ps = 0; ps < passes; ++ps)
-        for (int color = 0; color < 2; ++color)
-            hipLaunchKernelGGL((k_halo_face<double, double>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0,
-                               h->s, G, m, p, color, bc);
-    hipLaunchKernelGGL((k_halo_edges<double>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, h->s, G, p);
+    KT kt(h, PLES_MG_K_HALO, (double)(nf * 6 + ne * 2) * passes * 8.0);
+    for (int ps = 0; ps < passes; ++ps) {
+        for (int set = 0; set < 3; ++set) {
+            const long nset = face_set_size(G, set);
+            if (nset == 0) continue;
+            for (int color = 0; color < 2; ++color)
+                hipLaunchKernelGGL((k_halo_face<double, double>), dim3((unsigned)((nset + 255) / 256)), dim3(256),
+                                   0, h->s, G, m, p, color, bc, set);
+        }
+        hipLaunchKernelGGL((k_halo_edges<double>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, h->s, G, p);
+    }
     CK(hipGetLastError());
     return PLES_MG_OK;
 }
@@ -579,7 +597,7 @@ void ples_mg_default_opts(ples_mg_opts* o) {
     o->krylov = 1;
     o->stop_mode = PLES_MG_STOP_REL_L2;
     o->project_rhs = 1;
-    o->halo_passes = 6;
+    o->halo_passes = 0;  // auto
     o->device = 0;
 }
 
@@ -627,6 +645,7 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
     if (o.precision != PLES_MG_FP64 && o.precision != PLES_MG_MIXED) return PLES_MG_ERR_ARG;
     if (o.nu_pre < 0 || o.nu_post < 0 || o.nu_coarse < 0 || !(o.omega > 0.0 && o.omega < 2.0))
         return PLES_MG_ERR_ARG;
+    if (o.halo_passes <= 0) o.halo_passes = (g->ip == 1 || g->kp == 1) ? 24 : 6;
     h->o = o;
     h->dev = o.device;
     CK(hipSetDevice(h->dev));
@@ -1062,7 +1081,7 @@ int ples_mg_set_rhs(ples_mg_handle* h, const double* rhs) {
     return PLES_MG_OK;
 }
 
-int ples_mg_set_timing(ples_mg_handle* h, int on) {
+int ples_mg_set_timing(ples_mg_handle* h, int mask) {
     if (!h) return PLES_MG_ERR_ARG;
     CK(hipSetDevice(h->dev));
     CK(hipStreamSynchronize(h->s));
@@ -1071,7 +1090,7 @@ int ples_mg_set_timing(ples_mg_handle* h, int on) {
         (void)hipEventDestroy(r.b);
     }
     h->recs.clear();
-    h->timing = on != 0;
+    h->timing = (unsigned)mask;
     return PLES_MG_OK;
 }
 

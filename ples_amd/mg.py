@@ -316,8 +316,18 @@ class Solver:
     def device_bytes(self) -> int:
         return int(lib().ples_mg_device_bytes(self._h))
 
-    def set_timing(self, on: bool):
-        _check(lib().ples_mg_set_timing(self._h, int(bool(on))), "ples_mg_set_timing")
+    def set_timing(self, families=None):
+        """Time the given kernel families (names or ids); None/False = off,
+        True = all."""
+        if families is True:
+            mask = 0xFF
+        elif not families:
+            mask = 0
+        else:
+            mask = 0
+            for f in families:
+                mask |= 1 << (KERNEL_FAMILIES[f] if isinstance(f, str) else int(f))
+        _check(lib().ples_mg_set_timing(self._h, mask), "ples_mg_set_timing")
 
     def kernel_stats(self, family) -> tuple[int, float, float]:
         fam = KERNEL_FAMILIES[family] if isinstance(family, str) else int(family)

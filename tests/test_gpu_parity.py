@@ -68,9 +68,12 @@ def test_metrica_vectors_and_giac():
     assert np.allclose(J, cases.interior(giac), rtol=1e-12)
 
 
-@pytest.mark.parametrize("ip,kp", [(0, 0), (1, 1), (1, 0)])
-def test_residual_matches_oracle(ip, kp):
-    g = cases.grid(32, 16, 16, gamma=2.0, amp=0.2, ip=ip, kp=kp)
+@pytest.mark.parametrize("ip,kp,gamma,amp", [(0, 0, 2.0, 0.2), (1, 1, 1.0, 0.1), (1, 0, 1.5, 0.1)])
+def test_residual_matches_oracle(ip, kp, gamma, amp):
+    # xi walls on a strongly warped and stretched grid make the reference's
+    # ghost iteration diverge (|g12/g11| >> 1 on thin wall cells); the cases
+    # here have a ghost fixed point, checked first.
+    g = cases.grid(32, 16, 16, gamma=gamma, amp=amp, ip=ip, kp=kp)
     rng = np.random.default_rng(5)
     fi = rng.standard_normal(g.gshape())
     rhs = rng.standard_normal((g.nz, g.ny, g.nx))
@@ -78,9 +81,12 @@ def test_residual_matches_oracle(ip, kp):
     dt = 0.37
     gm, _, _ = O.metrics(g, 0)
     p = fi.copy()
-    O.ghost_fill(g, 0, p, gm, cs, 1.0, dt, 80)  # converged ghost fixed point
+    O.ghost_fill(g, 0, p, gm, cs, 1.0, dt, 80)
+    p2 = p.copy()
+    O.ghost_fill(g, 0, p2, gm, cs, 1.0, dt, 20)
+    assert np.abs(p2 - p).max() < 1e-13  # converged ghost fixed point
     r_ref = O.resid(g, 0, p, cases.ghosted(g, rhs), gm)
-    with _solver(g, 2, halo_passes=20) as s:
+    with _solver(g, 2, halo_passes=60) as s:
         r = s.residual(fi, rhs, cs, dt)
     ref = cases.interior(r_ref)
     err = np.abs(r - ref).max() / np.abs(ref).max()
