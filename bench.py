@@ -1,0 +1,207 @@
+"""Benchmark: Poisson solves/s to 1e-10 relative residual on one MI355X.
+
+Workload (default): the terrain-following 256x128x256 grid (tanh 1.5
+stretching, terrain amplitude 0.1 H), 6 multigrid levels, seeded Gaussian
+divergence right-hand side, solved from zero to ||b - A p||/||b|| < 1e-10.
+One step = one complete solve.  `value` is the mixed-precision solver (FP32
+V-cycles inside the FP64 GCR refinement loop); the FP64 multigrid is timed on
+the same steps and reported beside it.  The 512x256x512 warm-start solve is
+reported in the same line under "c3".
+
+The roofline entry is the fine-level eta-line smoother of the timed mixed
+solves: algorithmic bytes per launch (the library's per-launch byte count)
+over the launch duration measured with HIP events on the solver's stream,
+inside the timed region.
+
+The cpu_baseline entry times the CPU oracle (the reference algorithm: SOR
+1.6, 5/10 sweeps per level visit, ghost fill 5x per sweep) on a bounded
+sample: three V-cycles of the same 256x128x256 solve on one core; solves/s
+extrapolate the measured time per cycle to the cycles needed for 1e-10 at
+the contraction measured over the sampled cycles.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X, AMD published
+
+
+def solver_for(cfg, precision, **kw):
+    from ples_amd import mg
+
+    x, y, z = cfg.nodes()
+    return mg.Solver(x, y, z, 0, 1, 0, cfg.nlevel, precision=precision, **kw)
+
+
+def time_solves(s, steps, warmup, tol, prepare, timing_fams=None):
+    """prepare(s) sets up the start of a step (device work, inside the step)."""
+    import ctypes
+
+    res = None
+    for _ in range(warmup):
+        prepare(s)
+        res = s.solve_resident(tol, 400)
+    if timing_fams:
+        s.set_timing(timing_fams)
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    cycles = []
+    for _ in range(steps):
+        prepare(s)
+        res = s.solve_resident(tol, 400)
+        cycles.append(res.cycles)
+        if not res.ok:
+            raise RuntimeError(f"solve did not converge: {res}")
+    hip.hipDeviceSynchronize()
+    dt = time.perf_counter() - t0
+    return dt, res, cycles
+
+
+def cpu_baseline(cfg):
+    import numpy as np
+
+    from oracle import oracle as O
+
+    O.build()
+    x, y, z = cfg.nodes()
+    g = O.Grid(cfg.nx, cfg.ny, cfg.nz, 0, 1, 0, x, y, z)
+    rhs = O.case_rhs(g, cfg.seed)
+    b0 = float(np.abs(rhs[1:-1, 1:-1, 1:-1]).max())
+    t = time.perf_counter()
+    O.multi(g, cfg.nlevel, g.zeros(), rhs, eps=1e-300, ficycle=0)
+    t_setup = time.perf_counter() - t
+    ncyc = 3
+    t = time.perf_counter()
+    _, _, hist = O.multi(g, cfg.nlevel, g.zeros(), rhs, eps=1e-300, ficycle=ncyc)
+    t_cyc = (time.perf_counter() - t - t_setup) / ncyc
+    rho = float(hist[-1] / hist[-2])
+    need = math.log(1e-10 * b0 / hist[-1]) / math.log(rho) if rho < 1 else float("inf")
+    cycles = ncyc + max(0.0, need)
+    return {
+        "value": 1.0 / (t_cyc * cycles),
+        "unit": "solves/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"3 V-cycles of the reference algorithm (CPU oracle: SOR 1.6, 5/10 sweeps, "
+                   f"5 ghost passes per sweep, L={cfg.nlevel}) on the {cfg.nx}x{cfg.ny}x{cfg.nz} case: "
+                   f"{t_cyc:.2f} s/cycle, contraction {rho:.3f}, {cycles:.0f} cycles to 1e-10 extrapolated"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--no-c3", action="store_true", help="skip the 512x256x512 warm-start line")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU slab decomposition is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    from ples_amd import cases as PC
+
+    cfg = PC.CONFIGS[args.config]
+
+    def from_zero(s):
+        s.zero_fi()
+
+    out = {}
+    for prec in ("mixed", "fp64"):
+        with solver_for(cfg, prec) as s:
+            s.case_rhs(cfg.seed)
+            fams = ["smooth_fine", "resid_fine"] if prec == "mixed" else ["smooth_fine"]
+            dt, res, cyc = time_solves(s, args.steps, args.warmup, cfg.tol, from_zero, fams)
+            stats = {f: s.kernel_stats(f) for f in fams}
+            out[prec] = dict(seconds=dt, res=res, cycles=cyc, stats=stats)
+            print(f"[bench] {cfg.name} {prec}: {args.steps / dt:.2f} solves/s, {1e3 * dt / args.steps:.2f} ms/solve, "
+                  f"cycles {cyc[-1]}, rel_res {res.rel_res:.2e}", file=sys.stderr, flush=True)
+
+    mixed, fp64 = out["mixed"], out["fp64"]
+    n, ms, by = mixed["stats"]["smooth_fine"]
+    avg_ms = ms / max(n, 1)
+    achieved = (by / max(n, 1)) / (avg_ms * 1e-3) / 1e9 if n else None
+    rn, rms, rby = mixed["stats"]["resid_fine"]
+    resid_gbs = (rby / max(rn, 1)) / ((rms / max(rn, 1)) * 1e-3) / 1e9 if rn else None
+
+    c3 = None
+    if not args.no_c3 and args.config != "c3":
+        c3cfg = PC.CONFIGS["c3"]
+        c3 = {}
+        for prec in ("mixed", "fp64"):
+            with solver_for(c3cfg, prec) as s:
+                s.case_rhs(c3cfg.seed)
+                s.zero_fi()
+                base = s.solve_resident(c3cfg.tol, 400)  # p_prev
+
+                def warm(sv):
+                    sv.perturb_fi(1004, 1e-3)
+
+                steps3 = max(2, args.steps // 2)
+                dt, res, cyc = time_solves(s, steps3, 1, c3cfg.tol, warm, ["smooth_fine"])
+                n3, ms3, by3 = s.kernel_stats("smooth_fine")
+                c3[prec] = {"solves_per_s": steps3 / dt, "ms_per_solve": 1e3 * dt / steps3,
+                            "cycles": cyc[-1], "cycles_from_zero": base.cycles, "rel_res": res.rel_res,
+                            "smoother_fine_GBs": (by3 / n3) / ((ms3 / n3) * 1e-3) / 1e9 if n3 else None}
+                print(f"[bench] c3 {prec}: {c3[prec]}", file=sys.stderr, flush=True)
+        c3["speedup_mixed_vs_fp64"] = c3["mixed"]["solves_per_s"] / c3["fp64"]["solves_per_s"]
+
+    cpu = None
+    if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(cfg)
+        print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
+
+    value = args.steps / mixed["seconds"]
+    line = {
+        "metric": "Poisson solves/s to 1e-10 rel-res (mixed vs FP64 MG, 1 GPU)",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * mixed["seconds"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic (seeded Gaussian divergence rhs generated on device)",
+        "config": {"workload": cfg.text, "cells": cfg.cells, "levels": cfg.nlevel, "tolerance": cfg.tol,
+                   "precision": "FP32 V-cycles, FP64 GCR refinement and outer residual"},
+        "fp64_solves_per_s": args.steps / fp64["seconds"],
+        "speedup_mixed_vs_fp64": fp64["seconds"] / mixed["seconds"],
+        "cycles": {"mixed": mixed["cycles"][-1], "fp64": fp64["cycles"][-1]},
+        "rel_res": {"mixed": mixed["res"].rel_res, "fp64": fp64["res"].rel_res},
+        "roofline": {
+            "kernel": "k_yline<float> fine level (eta-line smoother)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS if achieved else None,
+            "traffic": None,
+            "launches": n,
+            "avg_us": 1e3 * avg_ms,
+            "bytes_per_launch": by / max(n, 1),
+            "resid_fine_GBs": resid_gbs,
+        },
+        "cpu_baseline": cpu,
+        "c3": c3,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
