@@ -185,118 +185,238 @@ __device__ inline void write_images(const Geo& G, T* __restrict__ p, int i, int 
 }
 
 // ------------------------------------------------------------------------
-// eta-line Jacobi smoother:  p_new = p + omega * M_line^{-1} (A p - b)
-// M_line = tridiag(-s^2 g22(j-1), den_j, -s^2 g22(j)) with den the sum of
-// the s^2-scaled normal metrics of the non-wall faces.  Factors invm = 1/m_j
-// and cpr = c'_j of its Thomas factorisation are precomputed per level.
-// Forward pass stores e_j = p_j + omega d'_j + c'_j p_{j+1}; backward pass
-// p_new_j = e_j - c'_j p_new_{j+1}.  Afterwards the column writes its eta-wall
-// ghosts (homogeneous flux condition, neighbours lagged) and its periodic
-// images.  zero_in: the input iterate is identically zero.
+// Column segments: the eta-columns are split into P equal segments, one
+// thread per (column, segment); a block holds bx columns x P segments of one
+// zeta plane k.  P = 1 gives one thread per column.
 // ------------------------------------------------------------------------
-template <class T, class TM, class TR>
-__global__ void __launch_bounds__(256) k_yline(Geo G, Met<TM> M, const float* __restrict__ invm,
-                                               const float* __restrict__ cpr,
-                                               const T* __restrict__ pin, T* __restrict__ pout,
-                                               const TR* __restrict__ b, T omega, T inv_pot,
-                                               int zero_in) {
-    int i, k;
-    if (!column_of(G, i, k)) return;
+struct Seg {
+    int i, k, s, j0, j1;
+};
+
+__device__ inline bool seg_of(const Geo& G, int P, Seg& q) {
+    q.i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    q.k = blockIdx.y + 1;
+    q.s = threadIdx.y;
+    const int m = G.ny / P;
+    q.j0 = q.s * m + 1;
+    q.j1 = q.j0 + m - 1;
+    return q.i <= G.nx;
+}
+
+// Factors of the segmented eta-line solve, all metric-only (per level):
+// invm, cpr: Thomas factorisation of the segment block M_s (the coupling to
+// the neighbouring segments removed), v = M_s^-1 (a_j0 e_first) and
+// w = M_s^-1 (c_j1 e_last), the responses to the neighbouring segments' end
+// values.
+struct LineF {
+    const float* invm;
+    const float* cpr;
+    const float* v;
+    const float* w;
+};
+
+// ------------------------------------------------------------------------
+// eta-line Jacobi smoother:  p_new = p + omega * M_line^{-1} (b - A p)
+// M_line = tridiag(-s^2 g22(j-1), den_j, -s^2 g22(j)) over the whole column,
+// den_j the s^2-scaled sum of the normal metrics of the non-wall faces.
+// Partitioned solve: each segment solves its block locally (forward pass
+// fused with the 19-point residual, storing e_j = p_j + omega d'_j + c'_j
+// p_{j+1}; backward pass q_j = e_j - c'_j q_{j+1} = p_j + local solution),
+// the 2P end values of a column are coupled through a block-tridiagonal 2x2
+// system solved in LDS by every thread of the column, and a last pass adds
+// the responses v, w to the neighbours' end values.  The column then writes
+// its eta-wall ghosts (homogeneous flux condition, neighbours lagged) and
+// its periodic images.  zero_in: the input iterate is identically zero.
+// ------------------------------------------------------------------------
+template <int P, class T>
+__device__ inline void reduced_solve(T (&red)[6][8][64], int tx, int s, T& uLm, T& uFp) {
+    // unknowns x_t = (F_t, L_t): x_t + A_t x_{t-1} + C_t x_{t+1} = Y_t,
+    // A_t = [[0, vF_t],[0, vL_t]], C_t = [[wF_t, 0],[wL_t, 0]]
+    T D[P][4], Gv[P][2];
+#pragma unroll
+    for (int t = 0; t < P; ++t) {
+        const T yF = red[0][t][tx], yL = red[1][t][tx];
+        const T vF = red[2][t][tx], vL = red[3][t][tx];
+        T d00 = 1, d01 = 0, d10 = 0, d11 = 1, g0 = yF, g1 = yL;
+        if (t > 0) {
+            // M = A_t * Dinv_{t-1}; A_t has only a second column (vF, vL)
+            const T m00 = vF * D[t - 1][2], m01 = vF * D[t - 1][3];
+            const T m10 = vL * D[t - 1][2], m11 = vL * D[t - 1][3];
+            const T wFp = red[4][t - 1][tx], wLp = red[5][t - 1][tx];
+            // D_t = I - M C_{t-1}; C_{t-1} = [[wF, 0],[wL, 0]]
+            d00 = 1 - (m00 * wFp + m01 * wLp);
+            d10 = -(m10 * wFp + m11 * wLp);
+            g0 = yF - (m00 * Gv[t - 1][0] + m01 * Gv[t - 1][1]);
+            g1 = yL - (m10 * Gv[t - 1][0] + m11 * Gv[t - 1][1]);
+        }
+        const T det = d00 * d11 - d01 * d10;
+        D[t][0] = d11 / det;
+        D[t][1] = -d01 / det;
+        D[t][2] = -d10 / det;
+        D[t][3] = d00 / det;
+        Gv[t][0] = g0;
+        Gv[t][1] = g1;
+    }
+    T xF = 0, xL = 0;  // x_{t+1}
+    uLm = 0;
+    uFp = 0;
+#pragma unroll
+    for (int t = P - 1; t >= 0; --t) {
+        T g0 = Gv[t][0], g1 = Gv[t][1];
+        if (t < P - 1) {
+            const T wF = red[4][t][tx], wL = red[5][t][tx];
+            g0 -= wF * xF;
+            g1 -= wL * xF;
+        }
+        const T nF = D[t][0] * g0 + D[t][1] * g1;
+        const T nL = D[t][2] * g0 + D[t][3] * g1;
+        if (t == s + 1) uFp = nF;
+        if (t == s - 1) uLm = nL;
+        xF = nF;
+        xL = nL;
+    }
+    (void)xL;
+}
+
+template <int P, class T, class TM, class TR>
+__global__ void __launch_bounds__(512) k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin,
+                                               T* __restrict__ pout, const TR* __restrict__ b, T omega,
+                                               T inv_pot, int zero_in) {
+    __shared__ T red[6][8][64];
+    Seg q;
+    const bool act = seg_of(G, P, q);
+    const int tx = threadIdx.x, s = q.s;
     const long sy = G.sy, sz = G.sz;
     const T s2 = inv_pot * inv_pot;
-    const long c0 = G.idx(i, 0, k);
-    T dprev = 0;
-    if (zero_in) {
-        T g22m = (T)M.g[4][c0];
-        for (int j = 1; j <= G.ny; ++j) {
-            const long c = c0 + sy * j;
-            const T r = -(T)b[c];
-            const T a = (j > 1) ? -s2 * g22m : (T)0;
-            const T d = (r - a * dprev) * (T)invm[c];
-            pout[c] = omega * d;
-            dprev = d;
-            g22m = (T)M.g[4][c];
+    const long cb = act ? G.idx(q.i, q.j0 - 1, q.k) : 0;  // row j0-1
+    T yF = 0, yL = 0, vF = 0, vL = 0, wF = 0, wL = 0;
+    if (act) {
+        T dprev = 0, pfirst = 0, plast = 0;
+        if (zero_in) {
+            T g22m = (T)M.g[4][cb];
+            for (int j = q.j0; j <= q.j1; ++j) {
+                const long c = cb + sy * (j - q.j0 + 1);
+                const T r = -(T)b[c];
+                const T a = (j > q.j0) ? -s2 * g22m : (T)0;
+                const T d = (r - a * dprev) * (T)F.invm[c];
+                pout[c] = omega * d;
+                dprev = d;
+                g22m = (T)M.g[4][c];
+            }
+        } else {
+            Win<T> w;
+            win_init(w, pin, cb, sy, sz);
+            G2<T> gm = load_g2<T>(M, cb);
+            pfirst = w.p0;
+            for (int j = q.j0; j <= q.j1; ++j) {
+                const long c = cb + sy * (j - q.j0 + 1);
+                win_load(w, pin, c, sy, sz);
+                const G2<T> g0 = load_g2<T>(M, c);
+                const T r = stencil(w, M, c, sz, gm, g0, inv_pot) - (T)b[c];
+                const T a = (j > q.j0) ? -s2 * gm.g22 : (T)0;
+                const T d = (r - a * dprev) * (T)F.invm[c];
+                pout[c] = w.p0 + omega * d + (T)F.cpr[c] * w.pp;
+                dprev = d;
+                plast = w.p0;
+                gm = g0;
+                win_shift(w);
+            }
         }
-    } else {
-        Win<T> w;
-        win_init(w, pin, c0, sy, sz);
-        G2<T> gm = load_g2<T>(M, c0);
-        for (int j = 1; j <= G.ny; ++j) {
-            const long c = c0 + sy * j;
-            win_load(w, pin, c, sy, sz);
-            const G2<T> g0 = load_g2<T>(M, c);
-            const T r = stencil(w, M, c, sz, gm, g0, inv_pot) - (T)b[c];
-            const T a = (j > 1) ? -s2 * gm.g22 : (T)0;
-            const T d = (r - a * dprev) * (T)invm[c];
-            pout[c] = w.p0 + omega * d + (T)cpr[c] * w.pp;
-            dprev = d;
-            gm = g0;
-            win_shift(w);
+        T qn = 0;
+        for (int j = q.j1; j >= q.j0; --j) {
+            const long c = cb + sy * (j - q.j0 + 1);
+            qn = pout[c] - (T)F.cpr[c] * qn;
+            pout[c] = qn;
+            if (j == q.j1) yL = qn - plast;
+        }
+        yF = qn - pfirst;
+        if (P > 1) {
+            const long cF = cb + sy, cL = cb + sy * (q.j1 - q.j0 + 1);
+            vF = (T)F.v[cF];
+            vL = (T)F.v[cL];
+            wF = (T)F.w[cF];
+            wL = (T)F.w[cL];
         }
     }
-    // backward substitution
-    T pn = 0, pn1 = 0, pnn = 0;
-    for (int j = G.ny; j >= 1; --j) {
-        const long c = c0 + sy * j;
-        pn = pout[c] - (T)cpr[c] * pn;
+    T uLm = 0, uFp = 0;
+    if (P > 1) {
+        red[0][s][tx] = yF;
+        red[1][s][tx] = yL;
+        red[2][s][tx] = vF;
+        red[3][s][tx] = vL;
+        red[4][s][tx] = wF;
+        red[5][s][tx] = wL;
+        __syncthreads();
+        reduced_solve<P, T>(red, tx, s, uLm, uFp);
+    }
+    if (!act) return;
+    const int i = q.i, k = q.k;
+    T pn1 = 0, pnn = 0;
+    for (int j = q.j0; j <= q.j1; ++j) {
+        const long c = cb + sy * (j - q.j0 + 1);
+        T pn = pout[c];
+        if (P > 1) pn -= uLm * (T)F.v[c] + uFp * (T)F.w[c];
         pout[c] = pn;
         write_images(G, pout, i, j, k, pn);
+        if (j == 1) pn1 = pn;
         if (j == G.ny) pnn = pn;
     }
-    pn1 = pn;
     // eta-wall ghosts, fsrc/multigrid.f03:1166-1186 with an = 0; the
     // neighbouring columns enter with their input (lagged) values.
-    {
-        const T q = (T)0.25;
-        T lo, hi;
-        if (zero_in) {
-            lo = pn1;
-            hi = pnn;
-        } else {
-            const long cb = c0;  // (i,0,k)
-            const T inv = (T)1 / (T)M.g[4][cb];
-            lo = pn1 + q * inv *
-                           ((T)M.g[3][cb] * ((T)pin[cb + 1 + sy] + (T)pin[cb + 1] - (T)pin[cb - 1 + sy] -
-                                             (T)pin[cb - 1]) +
-                            (T)M.g[5][cb] * ((T)pin[cb + sy + sz] + (T)pin[cb + sz] - (T)pin[cb + sy - sz] -
-                                             (T)pin[cb - sz]));
-            const long ct = c0 + sy * G.ny;  // (i,ny,k)
+    const T qq = (T)0.25;
+    if (q.j0 == 1) {
+        T lo = pn1;
+        const long c0 = G.idx(i, 0, k);
+        if (!zero_in) {
+            const T inv = (T)1 / (T)M.g[4][c0];
+            lo = pn1 + qq * inv *
+                           ((T)M.g[3][c0] * ((T)pin[c0 + 1 + sy] + (T)pin[c0 + 1] - (T)pin[c0 - 1 + sy] -
+                                             (T)pin[c0 - 1]) +
+                            (T)M.g[5][c0] * ((T)pin[c0 + sy + sz] + (T)pin[c0 + sz] - (T)pin[c0 + sy - sz] -
+                                             (T)pin[c0 - sz]));
+        }
+        pout[c0] = lo;
+        write_images(G, pout, i, 0, k, lo);
+    }
+    if (q.j1 == G.ny) {
+        T hi = pnn;
+        const long ct = G.idx(i, G.ny, k);
+        if (!zero_in) {
             const T invt = (T)1 / (T)M.g[4][ct];
-            hi = pnn - q * invt *
+            hi = pnn - qq * invt *
                            ((T)M.g[3][ct] * ((T)pin[ct + 1 + sy] + (T)pin[ct + 1] - (T)pin[ct - 1 + sy] -
                                              (T)pin[ct - 1]) +
                             (T)M.g[5][ct] * ((T)pin[ct + sy + sz] + (T)pin[ct + sz] - (T)pin[ct + sy - sz] -
                                              (T)pin[ct - sz]));
         }
-        pout[c0] = lo;
-        write_images(G, pout, i, 0, k, lo);
-        pout[c0 + sy * (G.ny + 1)] = hi;
+        pout[ct + sy] = hi;
         write_images(G, pout, i, G.ny + 1, k, hi);
     }
 }
 
 // ------------------------------------------------------------------------
-// Residual family.  MODE 0: r = A p - b           (V-cycle residual)
-//                   MODE 1: r = b - A p, partial sums of r.r, max|r|, b.b
-//                   MODE 2: w = A p, partial sums of w.w, w.wp, r.w, r.wp
+// Residual family (segmented columns).
+//   MODE 0: r = A p - b                     (V-cycle residual)
+//   MODE 1: r = b - A p, partial sums of r.r, max|r|, b.b
+//   MODE 2: w = A p, partial sums of w.w, w.wp, r.w, r.wp
 // ------------------------------------------------------------------------
 template <int MODE, class T, class TM, class TR, class TO>
-__global__ void __launch_bounds__(256) k_resid(Geo G, Met<TM> M, const T* __restrict__ p,
+__global__ void __launch_bounds__(512) k_resid(Geo G, Met<TM> M, const T* __restrict__ p,
                                                const TR* __restrict__ b, TO* __restrict__ out,
                                                T inv_pot, const double* __restrict__ rvec,
                                                const double* __restrict__ wp, double* partials,
-                                               int nblocks) {
-    int i, k;
-    const bool act = column_of(G, i, k);
+                                               int nblocks, int P) {
+    Seg q;
+    const bool act = seg_of(G, P, q);
     double acc[4] = {0, 0, 0, 0};
     if (act) {
         const long sy = G.sy, sz = G.sz;
-        const long c0 = G.idx(i, 0, k);
+        const long cb = G.idx(q.i, q.j0 - 1, q.k);
         Win<T> w;
-        win_init(w, p, c0, sy, sz);
-        G2<T> gm = load_g2<T>(M, c0);
-        for (int j = 1; j <= G.ny; ++j) {
-            const long c = c0 + sy * j;
+        win_init(w, p, cb, sy, sz);
+        G2<T> gm = load_g2<T>(M, cb);
+        for (int j = q.j0; j <= q.j1; ++j) {
+            const long c = cb + sy * (j - q.j0 + 1);
             win_load(w, p, c, sy, sz);
             const G2<T> g0 = load_g2<T>(M, c);
             const T ap = stencil(w, M, c, sz, gm, g0, inv_pot);
@@ -338,42 +458,58 @@ __global__ void __launch_bounds__(256) k_resid(Geo G, Met<TM> M, const T* __rest
 
 // ------------------------------------------------------------------------
 // Outer-loop scalar logic on the device (no host round trip per iteration).
-// S layout (doubles):
-//  0 rr   1 rmax  2 bb   3 ww   4 wwp  5 rw   6 rwp  7 wpwp  8 alpha
-//  9 beta 10 have_prev 11 rr0
 // ------------------------------------------------------------------------
 enum { S_RR = 0, S_RMAX, S_BB, S_WW, S_WWP, S_RW, S_RWP, S_WPWP, S_ALPHA, S_BETA, S_HAVEP, S_RR0,
        S_SUM, S_COUNT = 16 };
 
-// finalize the partial sums written by k_resid MODE 1 into S[0..2]
-__global__ void k_fin_resid(const double* __restrict__ part, int nb, double* S) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double rr = 0, rm = 0, bb = 0;
-    for (int q = 0; q < nb; ++q) {
-        rr += part[q];
-        rm = fmax(rm, part[nb + q]);
-        bb += part[2 * nb + q];
+// sum (or max) of nb partials per quantity, one block of 256 threads,
+// fixed order: strided per-thread sums then a tree
+template <int NV>
+__device__ inline void fin_reduce(const double* __restrict__ part, int nb, const bool (&mx)[NV],
+                                  double (&out)[NV]) {
+    __shared__ double sh[NV][256];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double a = 0.0;
+        for (int u = t; u < nb; u += 256) a = mx[q] ? fmax(a, part[(long)q * nb + u]) : a + part[(long)q * nb + u];
+        sh[q][t] = a;
     }
-    S[S_RR] = rr;
-    S[S_RMAX] = rm;
-    S[S_BB] = bb;
+    __syncthreads();
+    for (int wdt = 128; wdt > 0; wdt >>= 1) {
+        if (t < wdt)
+#pragma unroll
+            for (int q = 0; q < NV; ++q) sh[q][t] = mx[q] ? fmax(sh[q][t], sh[q][t + wdt]) : sh[q][t] + sh[q][t + wdt];
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] = sh[q][0];
+}
+
+// finalize the partial sums written by k_resid MODE 1 into S[0..2]
+__global__ void __launch_bounds__(256) k_fin_resid(const double* __restrict__ part, int nb, double* S) {
+    const bool mx[3] = {false, true, false};
+    double o[3];
+    fin_reduce<3>(part, nb, mx, o);
+    if (threadIdx.x == 0) {
+        S[S_RR] = o[0];
+        S[S_RMAX] = o[1];
+        S[S_BB] = o[2];
+    }
 }
 
 // finalize k_resid MODE 2 sums and compute the GCR step (beta, alpha)
-__global__ void k_fin_apply(const double* __restrict__ part, int nb, double* S, int krylov) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double ww = 0, wwp = 0, rw = 0, rwp = 0;
-    for (int q = 0; q < nb; ++q) {
-        ww += part[q];
-        wwp += part[nb + q];
-        rw += part[2 * nb + q];
-        rwp += part[3 * nb + q];
-    }
+__global__ void __launch_bounds__(256) k_fin_apply(const double* __restrict__ part, int nb, double* S, int krylov) {
+    const bool mx[4] = {false, false, false, false};
+    double o[4];
+    fin_reduce<4>(part, nb, mx, o);
+    if (threadIdx.x != 0) return;
+    const double ww = o[0], wwp = o[1], rw = o[2], rwp = o[3];
     double beta = 0.0;
     if (krylov && S[S_HAVEP] != 0.0 && S[S_WPWP] > 0.0) beta = wwp / S[S_WPWP];
     const double wwn = ww - 2.0 * beta * wwp + beta * beta * S[S_WPWP];
     const double rwn = rw - beta * rwp;
-    double alpha = krylov ? ((wwn > 0.0) ? rwn / wwn : 0.0) : 1.0;
+    const double alpha = krylov ? ((wwn > 0.0) ? rwn / wwn : 0.0) : 1.0;
     S[S_WW] = ww;
     S[S_WWP] = wwp;
     S[S_RW] = rw;
@@ -385,20 +521,20 @@ __global__ void k_fin_apply(const double* __restrict__ part, int nb, double* S, 
 }
 
 // w <- w - beta wp ; z <- z - beta zp ; x += alpha z ; r -= alpha w ;
-// partial sums of r.r and max|r| (interior cells, one thread per column)
-__global__ void __launch_bounds__(256) k_gcr_update(Geo G, double* __restrict__ x, double* __restrict__ r,
+// partial sums of r.r and max|r| (interior cells, segmented columns)
+__global__ void __launch_bounds__(512) k_gcr_update(Geo G, double* __restrict__ x, double* __restrict__ r,
                                                     double* __restrict__ z, double* __restrict__ w,
                                                     const double* __restrict__ zp,
                                                     const double* __restrict__ wp, const double* S,
-                                                    double* partials, int nblocks) {
-    int i, k;
-    const bool act = column_of(G, i, k);
+                                                    double* partials, int nblocks, int P) {
+    Seg q;
+    const bool act = seg_of(G, P, q);
     double acc[2] = {0, 0};
     if (act) {
         const double alpha = S[S_ALPHA], beta = S[S_BETA];
-        const long c0 = G.idx(i, 0, k);
-        for (int j = 1; j <= G.ny; ++j) {
-            const long c = c0 + G.sy * j;
+        const long cb = G.idx(q.i, q.j0 - 1, q.k);
+        for (int j = q.j0; j <= q.j1; ++j) {
+            const long c = cb + G.sy * (j - q.j0 + 1);
             double zv = z[c], wv = w[c];
             if (beta != 0.0) {
                 zv -= beta * zp[c];
@@ -418,15 +554,14 @@ __global__ void __launch_bounds__(256) k_gcr_update(Geo G, double* __restrict__ 
     block_reduce_store<2>(v, partials, nblocks, mx);
 }
 
-__global__ void k_fin_update(const double* __restrict__ part, int nb, double* S) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double rr = 0, rm = 0;
-    for (int q = 0; q < nb; ++q) {
-        rr += part[q];
-        rm = fmax(rm, part[nb + q]);
+__global__ void __launch_bounds__(256) k_fin_update(const double* __restrict__ part, int nb, double* S) {
+    const bool mx[2] = {false, true};
+    double o[2];
+    fin_reduce<2>(part, nb, mx, o);
+    if (threadIdx.x == 0) {
+        S[S_RR] = o[0];
+        S[S_RMAX] = o[1];
     }
-    S[S_RR] = rr;
-    S[S_RMAX] = rm;
 }
 
 // ------------------------------------------------------------------------
@@ -502,7 +637,7 @@ __device__ inline int wrapk(const Geo& G, int k) {
 // one launch per face set (0 eta, 1 xi, 2 zeta) and colour, so that no
 // ghost read in a launch is written by the same launch
 template <class T, class TM>
-__global__ void k_halo_face(Geo G, Met<TM> M, T* __restrict__ p, int color, BCs bc, int set) {
+__global__ void k_halo_face(Geo G, Met<TM> M, T* __restrict__ p, int color, BCs bc, int set, int images) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (set == 1) t += 2L * G.nx * G.nz;
     if (set == 2) t += 2L * G.nx * G.nz + (G.ip == 1 ? 2L * G.ny * G.nz : 0L);
@@ -521,21 +656,25 @@ __global__ void k_halo_face(Geo G, Met<TM> M, T* __restrict__ p, int color, BCs 
                 const long cb = G.idx(i, 0, k);
                 const T inv = (T)1 / (T)M.g[4][cb];
                 const T csv = bc.cs[2] ? (T)bc.cs[2][(i - 1) + (long)G.nx * (k - 1)] : (T)0;
-                p[cb] = PV(i, 1, k) - (T)an * csv * inv +
+                const T val = PV(i, 1, k) - (T)an * csv * inv +
                         q * inv *
                             ((T)M.g[3][cb] * (PV(i + 1, 1, k) + PV(i + 1, 0, k) - PV(i - 1, 1, k) - PV(i - 1, 0, k)) +
                              (T)M.g[5][cb] * (PV(i, 1, k + 1) + PV(i, 0, k + 1) - PV(i, 1, k - 1) - PV(i, 0, k - 1)));
+                p[cb] = val;
+                if (images) write_images(G, p, i, 0, k, val);
             } else {
                 const int ny = G.ny;
                 const long ct = G.idx(i, ny, k);
                 const T inv = (T)1 / (T)M.g[4][ct];
                 const T csv = bc.cs[3] ? (T)bc.cs[3][(i - 1) + (long)G.nx * (k - 1)] : (T)0;
-                p[ct + G.sy] = PV(i, ny, k) + (T)an * csv * inv -
+                const T val = PV(i, ny, k) + (T)an * csv * inv -
                                q * inv *
                                    ((T)M.g[3][ct] * (PV(i + 1, ny + 1, k) + PV(i + 1, ny, k) - PV(i - 1, ny + 1, k) -
                                                      PV(i - 1, ny, k)) +
                                     (T)M.g[5][ct] * (PV(i, ny + 1, k + 1) + PV(i, ny, k + 1) - PV(i, ny + 1, k - 1) -
                                                      PV(i, ny, k - 1)));
+                p[ct + G.sy] = val;
+                if (images) write_images(G, p, i, ny + 1, k, val);
             }
             return;
         }
@@ -676,16 +815,18 @@ __global__ void k_halo_edges(Geo G, T* __restrict__ p) {
 }
 
 // ------------------------------------------------------------------------
-// Smoother factors: Thomas factorisation of M_line per column.
+// Smoother factors of the segmented eta-line solve (see LineF), computed in
+// FP64 and stored in FP32; the v and w solves use the stored factors so the
+// partitioned solve is consistent with the smoother's local solves.
 // ------------------------------------------------------------------------
 template <class TM>
-__global__ void k_line_factors(Geo G, Met<TM> M, double inv_pot, float* __restrict__ invm,
-                               float* __restrict__ cpr) {
-    int i, k;
-    if (!column_of(G, i, k)) return;
+__global__ void k_line_factors(Geo G, Met<TM> M, double inv_pot, int P, float* __restrict__ invm,
+                               float* __restrict__ cpr, float* __restrict__ v, float* __restrict__ w) {
+    Seg q;
+    if (!seg_of(G, P, q)) return;
+    const int i = q.i, k = q.k;
     const double s2 = inv_pot * inv_pot;
-    double cprev = 0.0;
-    for (int j = 1; j <= G.ny; ++j) {
+    auto den_at = [&](int j) {
         const long c = G.idx(i, j, k);
         const double g11p = (G.ip == 1 && i == G.nx) ? 0.0 : (double)M.g[0][c];
         const double g11m = (G.ip == 1 && i == 1) ? 0.0 : (double)M.g[0][c - 1];
@@ -693,13 +834,42 @@ __global__ void k_line_factors(Geo G, Met<TM> M, double inv_pot, float* __restri
         const double g22m = (j == 1) ? 0.0 : (double)M.g[4][c - G.sy];
         const double g33p = (G.kp == 1 && k == G.nz) ? 0.0 : (double)M.g[8][c];
         const double g33m = (G.kp == 1 && k == 1) ? 0.0 : (double)M.g[8][c - G.sz];
-        const double den = s2 * (g11p + g11m + g22p + g22m + g33p + g33m);
-        const double a = -s2 * g22m, cc = -s2 * g22p;
-        const double m = den - a * cprev;
+        return s2 * (g11p + g11m + g22p + g22m + g33p + g33m);
+    };
+    // couplings of the whole column: a_j to j-1, c_j to j+1 (zero at the walls)
+    auto a_at = [&](int j) { return (j == 1) ? 0.0 : -s2 * (double)M.g[4][G.idx(i, j - 1, k)]; };
+    auto c_at = [&](int j) { return (j == G.ny) ? 0.0 : -s2 * (double)M.g[4][G.idx(i, j, k)]; };
+    double cprev = 0.0;
+    for (int j = q.j0; j <= q.j1; ++j) {
+        const long c = G.idx(i, j, k);
+        const double a = (j > q.j0) ? a_at(j) : 0.0;
+        const double cc = (j < q.j1) ? c_at(j) : 0.0;
+        const double m = den_at(j) - a * cprev;
         const double cp = cc / m;
         invm[c] = (float)(1.0 / m);
         cpr[c] = (float)cp;
-        cprev = cp;
+        cprev = (double)(float)cp;
+    }
+    // v = M_s^-1 (a_j0 e_first), w = M_s^-1 (c_j1 e_last), forward then back;
+    // a single segment has no neighbours (and no v, w arrays)
+    if (P == 1) return;
+    const double aj0 = a_at(q.j0), cj1 = c_at(q.j1);
+    double dv = 0.0, dw = 0.0;
+    for (int j = q.j0; j <= q.j1; ++j) {
+        const long c = G.idx(i, j, k);
+        const double a = (j > q.j0) ? a_at(j) : 0.0;
+        dv = ((j == q.j0 ? aj0 : 0.0) - a * dv) * (double)invm[c];
+        dw = ((j == q.j1 ? cj1 : 0.0) - a * dw) * (double)invm[c];
+        v[c] = (float)dv;
+        w[c] = (float)dw;
+    }
+    double vn = 0.0, wn = 0.0;
+    for (int j = q.j1; j >= q.j0; --j) {
+        const long c = G.idx(i, j, k);
+        vn = (double)v[c] - (double)cpr[c] * vn;
+        wn = (double)w[c] - (double)cpr[c] * wn;
+        v[c] = (float)vn;
+        w[c] = (float)wn;
     }
 }
 

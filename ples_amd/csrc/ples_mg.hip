@@ -24,6 +24,9 @@ struct Level {
     float* g32[9] = {};
     float* invm = nullptr;
     float* cpr = nullptr;
+    float* lv_v = nullptr;  // segmented line-solve responses
+    float* lv_w = nullptr;
+    int P = 1;              // eta segments per column
     void* p[2] = {nullptr, nullptr};
     void* b = nullptr;
     void* r = nullptr;
@@ -146,6 +149,25 @@ ColLaunch col_launch(const Geo& G) {
     return c;
 }
 
+// segmented-column launch: block (bx, P) over (i, segment), one zeta plane per block row
+ColLaunch seg_launch(const Geo& G, int P) {
+    ColLaunch c;
+    const int bx = std::min(64, pow2ceil(G.nx));
+    c.block = dim3(bx, P, 1);
+    c.grid = dim3((G.nx + bx - 1) / bx, G.nz, 1);
+    c.nblocks = (int)(c.grid.x * c.grid.y);
+    return c;
+}
+
+// eta segments per column: up to 8, at least 16 rows each, dividing ny
+int choose_segments(int ny) {
+    int P = 1;
+    while (P < 8 && ny % (2 * P) == 0 && ny / (2 * P) >= 16) P *= 2;
+    return P;
+}
+
+LineF line_f(const Level& L) { return LineF{L.invm, L.cpr, L.lv_v, L.lv_w}; }
+
 // per-launch event timing
 struct KT {
     H* h;
@@ -210,7 +232,7 @@ int halo(H* h, const Level& L, T* p, double an_over_dt, bool with_cs, int passes
             if (nset == 0) continue;
             for (int color = 0; color < 2; ++color)
                 hipLaunchKernelGGL((k_halo_face<T, T>), dim3((unsigned)((nset + 255) / 256)), dim3(256), 0, h->s,
-                                   G, met<T>(L), p, color, bc, set);
+                                   G, met<T>(L), p, color, bc, set, 0);
         }
         hipLaunchKernelGGL((k_halo_edges<T>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, h->s, G, p);
     }
@@ -218,8 +240,11 @@ int halo(H* h, const Level& L, T* p, double an_over_dt, bool with_cs, int passes
     return PLES_MG_OK;
 }
 
-// exact ghost fill of an FP64 fine-level vector with the outer metrics
-int halo64(H* h, double* p, bool with_bc) {
+// exact ghost fill of an FP64 fine-level vector with the outer metrics.
+// fast: periodic xi and zeta with valid interior images -- only the eta-wall
+// ghosts are iterated (neighbours read through the periodic wrap) and each
+// pass writes their images, no edge launch.
+int halo64(H* h, double* p, bool with_bc, bool fast = false) {
     const Level& L0 = h->lv[0];
     Met<double> m = met<double>(L0);
     const Geo& G = L0.G;
@@ -232,35 +257,50 @@ int halo64(H* h, double* p, bool with_bc) {
     const long NX = G.nx + 2, NY = G.ny + 2, NZ = G.nz + 2;
     const long ne = 2 * NY * NZ + 2 * NX * NZ + 2 * NX * NY;
     const int passes = std::max(1, h->o.halo_passes);
-    KT kt(h, PLES_MG_K_HALO, (double)(nf * 6 + ne * 2) * passes * 8.0);
+    fast = fast && G.ip == 0 && G.kp == 0;
+    KT kt(h, PLES_MG_K_HALO, (double)(nf * 6 + (fast ? 0 : ne * 2)) * passes * 8.0);
     for (int ps = 0; ps < passes; ++ps) {
         for (int set = 0; set < 3; ++set) {
             const long nset = face_set_size(G, set);
             if (nset == 0) continue;
             for (int color = 0; color < 2; ++color)
                 hipLaunchKernelGGL((k_halo_face<double, double>), dim3((unsigned)((nset + 255) / 256)), dim3(256),
-                                   0, h->s, G, m, p, color, bc, set);
+                                   0, h->s, G, m, p, color, bc, set, fast ? 1 : 0);
         }
-        hipLaunchKernelGGL((k_halo_edges<double>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, h->s, G, p);
+        if (!fast)
+            hipLaunchKernelGGL((k_halo_edges<double>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, h->s, G,
+                               p);
     }
     CK(hipGetLastError());
     return PLES_MG_OK;
 }
 
 // ---------------------------------------------------------------- V-cycle pieces
+template <int P, class T, class TR>
+void launch_yline(H* h, Level& L, const TR* rhs, int zin) {
+    const ColLaunch cl = seg_launch(L.G, P);
+    hipLaunchKernelGGL((k_yline<P, T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), line_f(L),
+                       (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->o.omega, (T)(1.0 / L.pot), zin);
+}
+
 template <class T, class TR>
 int smooth(H* h, Level& L, const TR* rhs, int nu, bool zero_start, bool fine) {
-    const ColLaunch cl = col_launch(L.G);
     const double sm = sizeof(T), sp = sizeof(T), sr = sizeof(TR);
     const double N = (double)L.ncell();
+    const double seg = L.P > 1 ? 8.0 : 0.0;  // v, w reads of the last pass
     for (int s = 0; s < nu; ++s) {
         const int zin = (zero_start && s == 0) ? 1 : 0;
-        const double bytes = zin ? N * (sr + sm + 8 + 3 * sp) : N * (9 * sm + 4 * sp + sr + 12);
+        // forward (rhs, metrics, iterate, factors, e) + backward (e, cpr, q) + last pass (q, v, w, p)
+        const double bytes = zin ? N * (sr + sm + 8 + 4 * sp + 4 + seg)
+                                 : N * (9 * sm + sp + sr + 8 + sp + sp + 4 + sp + sp + sp + seg);
         {
             KT kt(h, fine ? PLES_MG_K_SMOOTH_FINE : PLES_MG_K_SMOOTH, bytes);
-            hipLaunchKernelGGL((k_yline<T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), L.invm, L.cpr,
-                               (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->o.omega,
-                               (T)(1.0 / L.pot), zin);
+            switch (L.P) {
+                case 8: launch_yline<8, T, TR>(h, L, rhs, zin); break;
+                case 4: launch_yline<4, T, TR>(h, L, rhs, zin); break;
+                case 2: launch_yline<2, T, TR>(h, L, rhs, zin); break;
+                default: launch_yline<1, T, TR>(h, L, rhs, zin); break;
+            }
         }
         L.cur ^= 1;
         if (L.G.ip == 1 || L.G.kp == 1) CKS(halo<T>(h, L, (T*)L.p[L.cur], 0.0, false, 1));
@@ -271,12 +311,12 @@ int smooth(H* h, Level& L, const TR* rhs, int nu, bool zero_start, bool fine) {
 
 template <class T, class TR>
 int vresid(H* h, Level& L, const TR* rhs, bool fine) {
-    const ColLaunch cl = col_launch(L.G);
+    const ColLaunch cl = seg_launch(L.G, L.P);
     const double N = (double)L.ncell();
     KT kt(h, fine ? PLES_MG_K_RESID_FINE : PLES_MG_K_RESID, N * (9 * sizeof(T) + 2 * sizeof(T) + sizeof(TR)));
     hipLaunchKernelGGL((k_resid<0, T, T, TR, T>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L),
                        (const T*)L.p[L.cur], rhs, (T*)L.r, (T)(1.0 / L.pot), (const double*)nullptr,
-                       (const double*)nullptr, (double*)nullptr, 0);
+                       (const double*)nullptr, (double*)nullptr, 0, L.P);
     CK(hipGetLastError());
     return PLES_MG_OK;
 }
@@ -370,21 +410,21 @@ int set_S(H* h, int idx, double v) {
 int outer_residual(H* h, double* x, double* r) {
     CKS(halo64(h, x, true));
     const Level& L0 = h->lv[0];
-    const ColLaunch cl = col_launch(L0.G);
+    const ColLaunch cl = seg_launch(L0.G, L0.P);
     {
         KT kt(h, PLES_MG_K_OUTER_APPLY, (double)L0.ncell() * (72 + 24));
         hipLaunchKernelGGL((k_resid<1, double, double, double, double>), cl.grid, cl.block, 0, h->s, L0.G,
                            met<double>(L0), (const double*)x, (const double*)h->bvec, r, 1.0,
-                           (const double*)nullptr, (const double*)nullptr, h->part, cl.nblocks);
+                           (const double*)nullptr, (const double*)nullptr, h->part, cl.nblocks, L0.P);
     }
-    hipLaunchKernelGGL(k_fin_resid, dim3(1), dim3(1), 0, h->s, (const double*)h->part, cl.nblocks, h->S);
+    hipLaunchKernelGGL(k_fin_resid, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S);
     CK(hipGetLastError());
     return PLES_MG_OK;
 }
 
 int solve_core(H* h, double eps, int maxcyc, ples_mg_result* res) {
     Level& L0 = h->lv[0];
-    const ColLaunch cl = col_launch(L0.G);
+    const ColLaunch cl = seg_launch(L0.G, L0.P);
     const double N0 = (double)L0.ncell();
     CK(hipEventRecord(h->t0, h->s));
     CKS(outer_residual(h, h->x, h->rvec));
@@ -419,23 +459,24 @@ int solve_core(H* h, double eps, int maxcyc, ples_mg_result* res) {
                 hipLaunchKernelGGL((k_convert<double, double>), dim3(nb), dim3(256), 0, h->s, n,
                                    (const double*)L0.p[L0.cur], Zq);
         }
-        CKS(halo64(h, Zq, false));
+        CKS(halo64(h, Zq, false, true));
         const bool havep = h->Sh[S_HAVEP] != 0.0 && h->o.krylov;
         {
             KT kt(h, PLES_MG_K_OUTER_APPLY, N0 * (72 + 24 + (havep ? 8 : 0)));
             hipLaunchKernelGGL((k_resid<2, double, double, double, double>), cl.grid, cl.block, 0, h->s, L0.G,
                                met<double>(L0), (const double*)Zq, (const double*)nullptr, Wq, 1.0,
                                (const double*)h->rvec, (const double*)(havep ? Wp : nullptr), h->part,
-                               cl.nblocks);
+                               cl.nblocks, L0.P);
         }
-        hipLaunchKernelGGL(k_fin_apply, dim3(1), dim3(1), 0, h->s, (const double*)h->part, cl.nblocks, h->S,
+        hipLaunchKernelGGL(k_fin_apply, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S,
                            h->o.krylov);
         {
             KT kt(h, PLES_MG_K_OUTER_UPDATE, N0 * (48 + (havep ? 32 + 16 : 0)));
             hipLaunchKernelGGL(k_gcr_update, cl.grid, cl.block, 0, h->s, L0.G, h->x, h->rvec, Zq, Wq,
-                               (const double*)Zp, (const double*)Wp, (const double*)h->S, h->part, cl.nblocks);
+                               (const double*)Zp, (const double*)Wp, (const double*)h->S, h->part, cl.nblocks,
+                               L0.P);
         }
-        hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(1), 0, h->s, (const double*)h->part, cl.nblocks, h->S);
+        hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S);
         CK(hipGetLastError());
         q = 1 - q;
         CKS(read_S(h));
@@ -538,13 +579,15 @@ int compute_metrics(H* h, int n, double* const g[9], double* const v[9]) {
 }
 
 int level_factors(H* h, Level& L) {
-    const ColLaunch cl = col_launch(L.G);
+    if ((L.P > 1) != (L.lv_v != nullptr && L.lv_w != nullptr)) return PLES_MG_ERR_ARG;
+    if (L.G.ny % L.P != 0 || L.P > 8) return PLES_MG_ERR_ARG;
+    const ColLaunch cl = seg_launch(L.G, L.P);
     if (L.f32)
         hipLaunchKernelGGL((k_line_factors<float>), cl.grid, cl.block, 0, h->s, L.G, met<float>(L), 1.0 / L.pot,
-                           L.invm, L.cpr);
+                           L.P, L.invm, L.cpr, L.lv_v, L.lv_w);
     else
         hipLaunchKernelGGL((k_line_factors<double>), cl.grid, cl.block, 0, h->s, L.G, met<double>(L),
-                           1.0 / L.pot, L.invm, L.cpr);
+                           1.0 / L.pot, L.P, L.invm, L.cpr, L.lv_v, L.lv_w);
     CK(hipGetLastError());
     return PLES_MG_OK;
 }
@@ -571,7 +614,7 @@ size_t estimate(const ples_mg_grid* g, const ples_mg_opts* o) {
         const size_t sp = f32 ? 4 : 8;
         tot += cells * 9 * sp;                  // V-cycle metrics
         if (n == 0 && f32) tot += cells * 9 * 8;  // FP64 outer metrics
-        tot += cells * 8;                       // invm, cpr
+        tot += cells * 16;                      // invm, cpr, v, w
         tot += cells * sp * 2;                  // ping-pong
         if (n > 0) tot += cells * sp;           // rhs
         if (n < g->nlevel - 1) tot += cells * sp;  // residual
@@ -704,8 +747,13 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
                 }
             }
         }
+        L.P = choose_segments(L.G.ny);
         CKS(dalloc(h, (void**)&L.invm, cells * 4));
         CKS(dalloc(h, (void**)&L.cpr, cells * 4));
+        if (L.P > 1) {
+            CKS(dalloc(h, (void**)&L.lv_v, cells * 4));
+            CKS(dalloc(h, (void**)&L.lv_w, cells * 4));
+        }
         CKS(level_factors(h, L));
         const size_t sp = L.f32 ? 4 : 8;
         CKS(dalloc(h, &L.p[0], cells * sp));
@@ -725,7 +773,7 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
                            (size_t)g->jx * g->jz, (size_t)g->jx * g->jy, (size_t)g->jx * g->jy};
     for (int q = 0; q < 6; ++q) CKS(dalloc(h, (void**)&h->cs[q], csz[q] * 8));
     CKS(dalloc(h, (void**)&h->S, S_COUNT * sizeof(double)));
-    h->part_n = std::max(col_launch(h->lv[0].G).nblocks, 1);
+    h->part_n = std::max(std::max(col_launch(h->lv[0].G).nblocks, seg_launch(h->lv[0].G, h->lv[0].P).nblocks), 1);
     CKS(dalloc(h, (void**)&h->part, (size_t)h->part_n * 4 * sizeof(double)));
     CK(hipStreamSynchronize(h->s));
     return PLES_MG_OK;
@@ -840,10 +888,10 @@ int ples_mg_residual(ples_mg_handle* h, const double* fi, int deepl, int deepr, 
     CK(hipMemcpyAsync(p, fi + (size_t)(deepl - 1) * G.sz, (size_t)G.n * 8, hipMemcpyHostToDevice, h->s));
     CKS(pack_box<double>(h, G, 1, G.nx, 1, G.ny, 1, G.nz, rhs, b, h->W[1]));
     CKS(halo64(h, p, true));
-    const ColLaunch cl = col_launch(G);
+    const ColLaunch cl = seg_launch(G, L0.P);
     hipLaunchKernelGGL((k_resid<0, double, double, double, double>), cl.grid, cl.block, 0, h->s, G,
                        met<double>(L0), (const double*)p, (const double*)b, out, 1.0, (const double*)nullptr,
-                       (const double*)nullptr, (double*)nullptr, 0);
+                       (const double*)nullptr, (double*)nullptr, 0, L0.P);
     CK(hipGetLastError());
     return unpack_box<double>(h, G, 1, G.nx, 1, G.ny, 1, G.nz, out, r, h->W[1]);
 }
