@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
     ap.add_argument("--no-c3", action="store_true", help="skip the 512x256x512 warm-start line")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-live-timing", action="store_true",
+                    help="no per-kernel HIP events in the timed region (roofline fields null)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -123,8 +125,10 @@ def main():
         with solver_for(cfg, prec) as s:
             s.case_rhs(cfg.seed)
             fams = ["smooth_fine", "resid_fine"] if prec == "mixed" else ["smooth_fine"]
+            if args.no_live_timing:
+                fams = []
             dt, res, cyc = time_solves(s, args.steps, args.warmup, cfg.tol, from_zero, fams)
-            stats = {f: s.kernel_stats(f) for f in fams}
+            stats = {f: s.kernel_stats(f) for f in ("smooth_fine", "resid_fine")}
             out[prec] = dict(seconds=dt, res=res, cycles=cyc, stats=stats)
             print(f"[bench] {cfg.name} {prec}: {args.steps / dt:.2f} solves/s, {1e3 * dt / args.steps:.2f} ms/solve, "
                   f"cycles {cyc[-1]}, rel_res {res.rel_res:.2e}", file=sys.stderr, flush=True)
@@ -150,7 +154,8 @@ def main():
                     sv.perturb_fi(1004, 1e-3)
 
                 steps3 = max(2, args.steps // 2)
-                dt, res, cyc = time_solves(s, steps3, 1, c3cfg.tol, warm, ["smooth_fine"])
+                dt, res, cyc = time_solves(s, steps3, 1, c3cfg.tol, warm,
+                                           [] if args.no_live_timing else ["smooth_fine"])
                 n3, ms3, by3 = s.kernel_stats("smooth_fine")
                 c3[prec] = {"solves_per_s": steps3 / dt, "ms_per_solve": 1e3 * dt / steps3,
                             "cycles": cyc[-1], "cycles_from_zero": base.cycles, "rel_res": res.rel_res,

@@ -68,7 +68,7 @@ typedef struct ples_mg_opts {
     int precision;   /* PLES_MG_FP64 or PLES_MG_MIXED */
     int nu_pre;      /* smoothing sweeps per level on the way down (default 2) */
     int nu_post;     /* ... on the way up (default 2) */
-    int nu_coarse;   /* sweeps on the coarsest level (default 24) */
+    int nu_coarse;   /* sweeps on the coarsest level (default 16) */
     double omega;    /* damping of the eta-line Jacobi smoother (default 0.8) */
     int krylov;      /* 1 = GCR acceleration of the refinement loop (default), 0 = plain */
     int stop_mode;   /* PLES_MG_STOP_REL_L2 (default) or PLES_MG_STOP_ABS_MAX */

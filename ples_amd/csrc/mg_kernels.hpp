@@ -193,9 +193,24 @@ struct Seg {
     int i, k, s, j0, j1;
 };
 
+// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2):
+// linear block b runs on XCD b % 8.  Remap so that XCD x works through the
+// contiguous tile range [x T/8, (x+1) T/8) in order, i.e. neighbouring zeta
+// planes (which share the k-1/k+1 stencil data) meet in the same L2.
+__device__ inline void xcd_tile(int& bx, int& bk) {
+    const int gx = gridDim.x, T = gridDim.x * gridDim.y;
+    const int b = blockIdx.x + gx * blockIdx.y;
+    int t = b;
+    if ((T & 7) == 0) t = (b & 7) * (T >> 3) + (b >> 3);
+    bx = t % gx;
+    bk = t / gx;
+}
+
 __device__ inline bool seg_of(const Geo& G, int P, Seg& q) {
-    q.i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    q.k = blockIdx.y + 1;
+    int bxi, bk;
+    xcd_tile(bxi, bk);
+    q.i = bxi * blockDim.x + threadIdx.x + 1;
+    q.k = bk + 1;
     q.s = threadIdx.y;
     const int m = G.ny / P;
     q.j0 = q.s * m + 1;
@@ -228,67 +243,309 @@ struct LineF {
 // its eta-wall ghosts (homogeneous flux condition, neighbours lagged) and
 // its periodic images.  zero_in: the input iterate is identically zero.
 // ------------------------------------------------------------------------
-template <int P, class T>
-__device__ inline void reduced_solve(T (&red)[6][8][64], int tx, int s, T& uLm, T& uFp) {
+template <int P, int BX, class T>
+__device__ inline void reduced_solve(T (&red)[8][P][BX], int tx, int s, T& uLm, T& uFp) {
     // unknowns x_t = (F_t, L_t): x_t + A_t x_{t-1} + C_t x_{t+1} = Y_t,
-    // A_t = [[0, vF_t],[0, vL_t]], C_t = [[wF_t, 0],[wL_t, 0]]
-    T D[P][4], Gv[P][2];
-#pragma unroll
-    for (int t = 0; t < P; ++t) {
-        const T yF = red[0][t][tx], yL = red[1][t][tx];
-        const T vF = red[2][t][tx], vL = red[3][t][tx];
-        T d00 = 1, d01 = 0, d10 = 0, d11 = 1, g0 = yF, g1 = yL;
-        if (t > 0) {
-            // M = A_t * Dinv_{t-1}; A_t has only a second column (vF, vL)
-            const T m00 = vF * D[t - 1][2], m01 = vF * D[t - 1][3];
-            const T m10 = vL * D[t - 1][2], m11 = vL * D[t - 1][3];
-            const T wFp = red[4][t - 1][tx], wLp = red[5][t - 1][tx];
-            // D_t = I - M C_{t-1}; C_{t-1} = [[wF, 0],[wL, 0]]
-            d00 = 1 - (m00 * wFp + m01 * wLp);
-            d10 = -(m10 * wFp + m11 * wLp);
-            g0 = yF - (m00 * Gv[t - 1][0] + m01 * Gv[t - 1][1]);
-            g1 = yL - (m10 * Gv[t - 1][0] + m11 * Gv[t - 1][1]);
+    // A_t = [[0, vF_t],[0, vL_t]], C_t = [[wF_t, 0],[wL_t, 0]].
+    // Block Thomas by the column's first thread, in place in LDS:
+    // red[0..1][t] (Y) -> G_t -> x_t, red[2..3][t] (vF, vL) and red[6..7][t]
+    // -> D_t^{-1}; red[4..5][t] keep (wF, wL).
+    if (s == 0) {
+        for (int t = 0; t < P; ++t) {
+            const T yF = red[0][t][tx], yL = red[1][t][tx];
+            const T vF = red[2][t][tx], vL = red[3][t][tx];
+            T d00 = 1, d01 = 0, d10 = 0, d11 = 1, g0 = yF, g1 = yL;
+            if (t > 0) {
+                const T i10 = red[6][t - 1][tx], i11 = red[7][t - 1][tx];
+                const T m00 = vF * i10, m01 = vF * i11, m10 = vL * i10, m11 = vL * i11;
+                const T wFp = red[4][t - 1][tx], wLp = red[5][t - 1][tx];
+                const T gp0 = red[0][t - 1][tx], gp1 = red[1][t - 1][tx];
+                d00 = 1 - (m00 * wFp + m01 * wLp);
+                d10 = -(m10 * wFp + m11 * wLp);
+                g0 = yF - (m00 * gp0 + m01 * gp1);
+                g1 = yL - (m10 * gp0 + m11 * gp1);
+            }
+            const T det = d00 * d11 - d01 * d10;
+            red[0][t][tx] = g0;
+            red[1][t][tx] = g1;
+            red[2][t][tx] = d11 / det;   // Dinv00
+            red[3][t][tx] = -d01 / det;  // Dinv01
+            red[6][t][tx] = -d10 / det;  // Dinv10
+            red[7][t][tx] = d00 / det;   // Dinv11
         }
-        const T det = d00 * d11 - d01 * d10;
-        D[t][0] = d11 / det;
-        D[t][1] = -d01 / det;
-        D[t][2] = -d10 / det;
-        D[t][3] = d00 / det;
-        Gv[t][0] = g0;
-        Gv[t][1] = g1;
-    }
-    T xF = 0, xL = 0;  // x_{t+1}
-    uLm = 0;
-    uFp = 0;
-#pragma unroll
-    for (int t = P - 1; t >= 0; --t) {
-        T g0 = Gv[t][0], g1 = Gv[t][1];
-        if (t < P - 1) {
-            const T wF = red[4][t][tx], wL = red[5][t][tx];
-            g0 -= wF * xF;
-            g1 -= wL * xF;
+        T xF = 0;
+        for (int t = P - 1; t >= 0; --t) {
+            T g0 = red[0][t][tx], g1 = red[1][t][tx];
+            if (t < P - 1) {
+                g0 -= red[4][t][tx] * xF;
+                g1 -= red[5][t][tx] * xF;
+            }
+            const T nF = red[2][t][tx] * g0 + red[3][t][tx] * g1;
+            const T nL = red[6][t][tx] * g0 + red[7][t][tx] * g1;
+            red[0][t][tx] = nF;
+            red[1][t][tx] = nL;
+            xF = nF;
         }
-        const T nF = D[t][0] * g0 + D[t][1] * g1;
-        const T nL = D[t][2] * g0 + D[t][3] * g1;
-        if (t == s + 1) uFp = nF;
-        if (t == s - 1) uLm = nL;
-        xF = nF;
-        xL = nL;
     }
-    (void)xL;
+    __syncthreads();
+    uLm = (s > 0) ? red[1][s - 1][tx] : (T)0;
+    uFp = (s < P - 1) ? red[0][s + 1][tx] : (T)0;
 }
 
-template <int P, class T, class TM, class TR>
+// lanes of a segmented-smoother block along xi for P segments
+template <int P>
+struct SegBX {
+    static constexpr int value = (256 / P < 32) ? 32 : ((256 / P > 128) ? 128 : 256 / P);
+};
+
+// eta-wall ghosts of column (i,k) after its new values pn1 = p(i,1,k),
+// pnn = p(i,ny,k): fsrc/multigrid.f03:1166-1186 with an = 0, neighbouring
+// columns with their input (lagged) values; writes the ghosts' images too.
+template <class T, class TM>
+__device__ inline void yline_ghosts(const Geo& G, const Met<TM>& M, const T* __restrict__ pin,
+                                    T* __restrict__ pout, int i, int k, bool lo_side, bool hi_side, T pn1,
+                                    T pnn, int zero_in) {
+    const T qq = (T)0.25;
+    const long sy = G.sy, sz = G.sz;
+    if (lo_side) {
+        T lo = pn1;
+        const long c0 = G.idx(i, 0, k);
+        if (!zero_in) {
+            const T inv = (T)1 / (T)M.g[4][c0];
+            lo = pn1 + qq * inv *
+                           ((T)M.g[3][c0] * ((T)pin[c0 + 1 + sy] + (T)pin[c0 + 1] - (T)pin[c0 - 1 + sy] -
+                                             (T)pin[c0 - 1]) +
+                            (T)M.g[5][c0] * ((T)pin[c0 + sy + sz] + (T)pin[c0 + sz] - (T)pin[c0 + sy - sz] -
+                                             (T)pin[c0 - sz]));
+        }
+        pout[c0] = lo;
+        write_images(G, pout, i, 0, k, lo);
+    }
+    if (hi_side) {
+        T hi = pnn;
+        const long ct = G.idx(i, G.ny, k);
+        if (!zero_in) {
+            const T invt = (T)1 / (T)M.g[4][ct];
+            hi = pnn - qq * invt *
+                           ((T)M.g[3][ct] * ((T)pin[ct + 1 + sy] + (T)pin[ct + 1] - (T)pin[ct - 1 + sy] -
+                                             (T)pin[ct - 1]) +
+                            (T)M.g[5][ct] * ((T)pin[ct + sy + sz] + (T)pin[ct + sz] - (T)pin[ct + sy - sz] -
+                                             (T)pin[ct - sz]));
+        }
+        pout[ct + sy] = hi;
+        write_images(G, pout, i, G.ny + 1, k, hi);
+    }
+}
+
+// one whole column (a single segment), the local solve staged through pout
+template <class T, class TM, class TR>
+__device__ inline void yline_column(const Geo& G, const Met<TM>& M, const LineF& F, const T* __restrict__ pin,
+                                    T* __restrict__ pout, const TR* __restrict__ b, T omega, T inv_pot,
+                                    int zero_in, int i, int k) {
+    const long sy = G.sy, sz = G.sz;
+    const T s2 = inv_pot * inv_pot;
+    const long cb = G.idx(i, 0, k);
+    T dprev = 0;
+    if (zero_in) {
+        T g22m = (T)M.g[4][cb];
+        for (int j = 1; j <= G.ny; ++j) {
+            const long c = cb + sy * j;
+            const T a = (j > 1) ? -s2 * g22m : (T)0;
+            const T d = (-(T)b[c] - a * dprev) * (T)F.invm[c];
+            pout[c] = omega * d;
+            dprev = d;
+            g22m = (T)M.g[4][c];
+        }
+    } else {
+        Win<T> w;
+        win_init(w, pin, cb, sy, sz);
+        G2<T> gm = load_g2<T>(M, cb);
+        for (int j = 1; j <= G.ny; ++j) {
+            const long c = cb + sy * j;
+            win_load(w, pin, c, sy, sz);
+            const G2<T> g0 = load_g2<T>(M, c);
+            const T r = stencil(w, M, c, sz, gm, g0, inv_pot) - (T)b[c];
+            const T a = (j > 1) ? -s2 * gm.g22 : (T)0;
+            const T d = (r - a * dprev) * (T)F.invm[c];
+            pout[c] = w.p0 + omega * d + (T)F.cpr[c] * w.pp;
+            dprev = d;
+            gm = g0;
+            win_shift(w);
+        }
+    }
+    T qn = 0, pnn = 0;
+    for (int j = G.ny; j >= 1; --j) {
+        const long c = cb + sy * j;
+        qn = pout[c] - (T)F.cpr[c] * qn;
+        pout[c] = qn;
+        write_images(G, pout, i, j, k, qn);
+        if (j == G.ny) pnn = qn;
+    }
+    yline_ghosts(G, M, pin, pout, i, k, true, true, qn, pnn, zero_in);
+}
+
+// Coarsest level: nu sweeps in one launch of one workgroup (the level fits
+// in a block's columns), ping-pong p0 -> p1 -> p0 ... with a block barrier
+// between sweeps.  The first sweep starts from zero.
+template <class T, class TM>
+__global__ void __launch_bounds__(1024) k_coarse(Geo G, Met<TM> M, LineF F, T* __restrict__ p0,
+                                                 T* __restrict__ p1, const T* __restrict__ b, T omega,
+                                                 T inv_pot, int nu) {
+    const int ncol = G.nx * G.nz;
+    for (int s = 0; s < nu; ++s) {
+        const T* pin = (s & 1) ? p1 : p0;
+        T* pout = (s & 1) ? p0 : p1;
+        for (int t = threadIdx.x; t < ncol; t += blockDim.x)
+            yline_column(G, M, F, pin, pout, b, omega, inv_pot, s == 0 ? 1 : 0, t % G.nx + 1, t / G.nx + 1);
+        __syncthreads();
+    }
+}
+
+// Coarsest level held in LDS: the iterate pair, rhs, metrics and line
+// factors are staged into the workgroup's LDS once, the nu sweeps run on LDS
+// (no global-memory latency per row), the result is written back.  Layout of
+// the dynamic LDS: 12 double arrays of n = ghosted cells (p0, p1, b, g11..g33),
+// then 2 float arrays (invm, cpr).
+template <class T>
+__global__ void __launch_bounds__(256) k_coarse_lds(Geo G, Met<T> M, LineF F, T* __restrict__ pout_g,
+                                                    const T* __restrict__ b_g, T omega, T inv_pot, int nu) {
+    extern __shared__ double smem[];
+    const long n = G.n;
+    T* base = reinterpret_cast<T*>(smem);
+    T* p0 = base;
+    T* p1 = base + n;
+    T* bb = base + 2 * n;
+    Met<T> ML;
+    for (int q = 0; q < 9; ++q) ML.g[q] = base + (3 + q) * n;
+    float* fl = reinterpret_cast<float*>(base + 12 * n);
+    LineF FL{fl, fl + n, nullptr, nullptr};
+    for (long t = threadIdx.x; t < n; t += blockDim.x) {
+        p0[t] = 0;
+        p1[t] = 0;
+        bb[t] = b_g[t];
+        for (int q = 0; q < 9; ++q) const_cast<T*>(ML.g[q])[t] = M.g[q][t];
+        fl[t] = F.invm[t];
+        fl[n + t] = F.cpr[t];
+    }
+    __syncthreads();
+    const int ncol = G.nx * G.nz;
+    for (int s = 0; s < nu; ++s) {
+        const T* pin = (s & 1) ? p1 : p0;
+        T* pout = (s & 1) ? p0 : p1;
+        for (int t = threadIdx.x; t < ncol; t += blockDim.x)
+            yline_column(G, ML, FL, pin, pout, bb, omega, inv_pot, s == 0 ? 1 : 0, t % G.nx + 1, t / G.nx + 1);
+        __syncthreads();
+    }
+    const T* res = (nu & 1) ? p1 : p0;
+    for (long t = threadIdx.x; t < n; t += blockDim.x) pout_g[t] = res[t];
+}
+
+// Segmented smoother with the segment held in registers (M rows per segment
+// known at compile time): forward pass fused with the 19-point residual,
+// local backward substitution in registers, reduced 2x2 block system of the
+// column in LDS, final pass adds the neighbour responses and writes once.
+template <int P, int MR, class T, class TM, class TR>
 __global__ void __launch_bounds__(512) k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin,
                                                T* __restrict__ pout, const TR* __restrict__ b, T omega,
                                                T inv_pot, int zero_in) {
-    __shared__ T red[6][8][64];
+    constexpr int BX = SegBX<P>::value;
+    __shared__ T red[8][P][BX];
     Seg q;
     const bool act = seg_of(G, P, q);
     const int tx = threadIdx.x, s = q.s;
     const long sy = G.sy, sz = G.sz;
     const T s2 = inv_pot * inv_pot;
     const long cb = act ? G.idx(q.i, q.j0 - 1, q.k) : 0;  // row j0-1
+    T e[MR];
+    T yF = 0, yL = 0, vF = 0, vL = 0, wF = 0, wL = 0;
+    if (act) {
+        T dprev = 0, pfirst = 0, plast = 0;
+        if (zero_in) {
+            T g22m = (T)M.g[4][cb];
+#pragma unroll
+            for (int jj = 0; jj < MR; ++jj) {
+                const long c = cb + sy * (jj + 1);
+                const T a = (jj > 0) ? -s2 * g22m : (T)0;
+                const T d = (-(T)b[c] - a * dprev) * (T)F.invm[c];
+                e[jj] = omega * d;
+                dprev = d;
+                g22m = (T)M.g[4][c];
+            }
+        } else {
+            Win<T> w;
+            win_init(w, pin, cb, sy, sz);
+            G2<T> gm = load_g2<T>(M, cb);
+            pfirst = w.p0;
+#pragma unroll
+            for (int jj = 0; jj < MR; ++jj) {
+                const long c = cb + sy * (jj + 1);
+                win_load(w, pin, c, sy, sz);
+                const G2<T> g0 = load_g2<T>(M, c);
+                const T r = stencil(w, M, c, sz, gm, g0, inv_pot) - (T)b[c];
+                const T a = (jj > 0) ? -s2 * gm.g22 : (T)0;
+                const T d = (r - a * dprev) * (T)F.invm[c];
+                e[jj] = w.p0 + omega * d + (T)F.cpr[c] * w.pp;
+                dprev = d;
+                plast = w.p0;
+                gm = g0;
+                win_shift(w);
+            }
+        }
+        T qn = 0;
+#pragma unroll
+        for (int jj = MR - 1; jj >= 0; --jj) {
+            qn = e[jj] - (T)F.cpr[cb + sy * (jj + 1)] * qn;  // re-read: cache hit, saves registers
+            e[jj] = qn;
+        }
+        yF = e[0] - pfirst;
+        yL = e[MR - 1] - plast;
+        if (P > 1) {
+            const long cF = cb + sy, cL = cb + sy * MR;
+            vF = (T)F.v[cF];
+            vL = (T)F.v[cL];
+            wF = (T)F.w[cF];
+            wL = (T)F.w[cL];
+        }
+    }
+    T uLm = 0, uFp = 0;
+    if (P > 1) {
+        red[0][s][tx] = yF;
+        red[1][s][tx] = yL;
+        red[2][s][tx] = vF;
+        red[3][s][tx] = vL;
+        red[4][s][tx] = wF;
+        red[5][s][tx] = wL;
+        __syncthreads();
+        reduced_solve<P, BX, T>(red, tx, s, uLm, uFp);
+    }
+    if (!act) return;
+    const int i = q.i, k = q.k;
+#pragma unroll
+    for (int jj = 0; jj < MR; ++jj) {
+        const long c = cb + sy * (jj + 1);
+        T pn = e[jj];
+        if (P > 1) pn -= uLm * (T)F.v[c] + uFp * (T)F.w[c];
+        e[jj] = pn;
+        pout[c] = pn;
+        write_images(G, pout, i, q.j0 + jj, k, pn);
+    }
+    yline_ghosts(G, M, pin, pout, i, k, q.j0 == 1, q.j1 == G.ny, e[0], e[MR - 1], zero_in);
+}
+
+// Segmented smoother for segment lengths without a register variant: the
+// local solve is staged through pout (e, then q), same algorithm.
+template <int P, class T, class TM, class TR>
+__global__ void __launch_bounds__(512) k_yline_staged(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin,
+                                                      T* __restrict__ pout, const TR* __restrict__ b, T omega,
+                                                      T inv_pot, int zero_in) {
+    constexpr int BX = SegBX<P>::value;
+    __shared__ T red[8][P][BX];
+    Seg q;
+    const bool act = seg_of(G, P, q);
+    const int tx = threadIdx.x, s = q.s;
+    const long sy = G.sy, sz = G.sz;
+    const T s2 = inv_pot * inv_pot;
+    const long cb = act ? G.idx(q.i, q.j0 - 1, q.k) : 0;
     T yF = 0, yL = 0, vF = 0, vL = 0, wF = 0, wL = 0;
     if (act) {
         T dprev = 0, pfirst = 0, plast = 0;
@@ -296,9 +553,8 @@ __global__ void __launch_bounds__(512) k_yline(Geo G, Met<TM> M, LineF F, const 
             T g22m = (T)M.g[4][cb];
             for (int j = q.j0; j <= q.j1; ++j) {
                 const long c = cb + sy * (j - q.j0 + 1);
-                const T r = -(T)b[c];
                 const T a = (j > q.j0) ? -s2 * g22m : (T)0;
-                const T d = (r - a * dprev) * (T)F.invm[c];
+                const T d = (-(T)b[c] - a * dprev) * (T)F.invm[c];
                 pout[c] = omega * d;
                 dprev = d;
                 g22m = (T)M.g[4][c];
@@ -347,7 +603,7 @@ __global__ void __launch_bounds__(512) k_yline(Geo G, Met<TM> M, LineF F, const 
         red[4][s][tx] = wF;
         red[5][s][tx] = wL;
         __syncthreads();
-        reduced_solve<P, T>(red, tx, s, uLm, uFp);
+        reduced_solve<P, BX, T>(red, tx, s, uLm, uFp);
     }
     if (!act) return;
     const int i = q.i, k = q.k;
@@ -358,40 +614,10 @@ __global__ void __launch_bounds__(512) k_yline(Geo G, Met<TM> M, LineF F, const 
         if (P > 1) pn -= uLm * (T)F.v[c] + uFp * (T)F.w[c];
         pout[c] = pn;
         write_images(G, pout, i, j, k, pn);
-        if (j == 1) pn1 = pn;
-        if (j == G.ny) pnn = pn;
+        if (j == q.j0) pn1 = pn;
+        if (j == q.j1) pnn = pn;
     }
-    // eta-wall ghosts, fsrc/multigrid.f03:1166-1186 with an = 0; the
-    // neighbouring columns enter with their input (lagged) values.
-    const T qq = (T)0.25;
-    if (q.j0 == 1) {
-        T lo = pn1;
-        const long c0 = G.idx(i, 0, k);
-        if (!zero_in) {
-            const T inv = (T)1 / (T)M.g[4][c0];
-            lo = pn1 + qq * inv *
-                           ((T)M.g[3][c0] * ((T)pin[c0 + 1 + sy] + (T)pin[c0 + 1] - (T)pin[c0 - 1 + sy] -
-                                             (T)pin[c0 - 1]) +
-                            (T)M.g[5][c0] * ((T)pin[c0 + sy + sz] + (T)pin[c0 + sz] - (T)pin[c0 + sy - sz] -
-                                             (T)pin[c0 - sz]));
-        }
-        pout[c0] = lo;
-        write_images(G, pout, i, 0, k, lo);
-    }
-    if (q.j1 == G.ny) {
-        T hi = pnn;
-        const long ct = G.idx(i, G.ny, k);
-        if (!zero_in) {
-            const T invt = (T)1 / (T)M.g[4][ct];
-            hi = pnn - qq * invt *
-                           ((T)M.g[3][ct] * ((T)pin[ct + 1 + sy] + (T)pin[ct + 1] - (T)pin[ct - 1 + sy] -
-                                             (T)pin[ct - 1]) +
-                            (T)M.g[5][ct] * ((T)pin[ct + sy + sz] + (T)pin[ct + sz] - (T)pin[ct + sy - sz] -
-                                             (T)pin[ct - sz]));
-        }
-        pout[ct + sy] = hi;
-        write_images(G, pout, i, G.ny + 1, k, hi);
-    }
+    yline_ghosts(G, M, pin, pout, i, k, q.j0 == 1, q.j1 == G.ny, pn1, pnn, zero_in);
 }
 
 // ------------------------------------------------------------------------

@@ -152,17 +152,19 @@ ColLaunch col_launch(const Geo& G) {
 // segmented-column launch: block (bx, P) over (i, segment), one zeta plane per block row
 ColLaunch seg_launch(const Geo& G, int P) {
     ColLaunch c;
-    const int bx = std::min(64, pow2ceil(G.nx));
+    const int bxmax = std::max(32, std::min(128, 256 / P));  // = SegBX<P>
+    const int bx = std::min(bxmax, pow2ceil(G.nx));
     c.block = dim3(bx, P, 1);
     c.grid = dim3((G.nx + bx - 1) / bx, G.nz, 1);
     c.nblocks = (int)(c.grid.x * c.grid.y);
     return c;
 }
 
-// eta segments per column: up to 8, at least 16 rows each, dividing ny
+// eta segments per column: the largest power of two up to 16 that divides ny
+// and leaves at least 16 rows per segment
 int choose_segments(int ny) {
     int P = 1;
-    while (P < 8 && ny % (2 * P) == 0 && ny / (2 * P) >= 16) P *= 2;
+    while (P < 16 && ny % (2 * P) == 0 && ny / (2 * P) >= 16) P *= 2;
     return P;
 }
 
@@ -256,8 +258,10 @@ int halo64(H* h, double* p, bool with_bc, bool fast = false) {
     if (G.kp == 1) nf += 2L * G.nx * G.ny;
     const long NX = G.nx + 2, NY = G.ny + 2, NZ = G.nz + 2;
     const long ne = 2 * NY * NZ + 2 * NX * NZ + 2 * NX * NY;
-    const int passes = std::max(1, h->o.halo_passes);
     fast = fast && G.ip == 0 && G.kp == 0;
+    // every pass counts: the outer GCR needs the ghosts at their fixed point
+    // to machine precision, or the attainable residual stalls
+    const int passes = std::max(1, h->o.halo_passes);
     KT kt(h, PLES_MG_K_HALO, (double)(nf * 6 + (fast ? 0 : ne * 2)) * passes * 8.0);
     for (int ps = 0; ps < passes; ++ps) {
         for (int set = 0; set < 3; ++set) {
@@ -279,9 +283,35 @@ int halo64(H* h, double* p, bool with_bc, bool fast = false) {
 template <int P, class T, class TR>
 void launch_yline(H* h, Level& L, const TR* rhs, int zin) {
     const ColLaunch cl = seg_launch(L.G, P);
-    hipLaunchKernelGGL((k_yline<P, T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), line_f(L),
-                       (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->o.omega, (T)(1.0 / L.pot), zin);
+    const LineF F = line_f(L);
+    const T* pin = (const T*)L.p[L.cur];
+    T* pout = (T*)L.p[1 - L.cur];
+    const T om = (T)h->o.omega, ip = (T)(1.0 / L.pot);
+    switch (L.G.ny / P) {
+        case 4:
+            hipLaunchKernelGGL((k_yline<P, 4, T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), F, pin, pout,
+                               rhs, om, ip, zin);
+            break;
+        case 8:
+            hipLaunchKernelGGL((k_yline<P, 8, T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), F, pin, pout,
+                               rhs, om, ip, zin);
+            break;
+        case 16:
+            hipLaunchKernelGGL((k_yline<P, 16, T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), F, pin,
+                               pout, rhs, om, ip, zin);
+            break;
+        case 32:
+            hipLaunchKernelGGL((k_yline<P, 32, T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), F, pin,
+                               pout, rhs, om, ip, zin);
+            break;
+        default:
+            hipLaunchKernelGGL((k_yline_staged<P, T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), F, pin,
+                               pout, rhs, om, ip, zin);
+    }
 }
+
+// segment lengths with a register-resident smoother variant
+bool reg_rows(int m) { return m == 4 || m == 8 || m == 16 || m == 32; }
 
 template <class T, class TR>
 int smooth(H* h, Level& L, const TR* rhs, int nu, bool zero_start, bool fine) {
@@ -290,12 +320,15 @@ int smooth(H* h, Level& L, const TR* rhs, int nu, bool zero_start, bool fine) {
     const double seg = L.P > 1 ? 8.0 : 0.0;  // v, w reads of the last pass
     for (int s = 0; s < nu; ++s) {
         const int zin = (zero_start && s == 0) ? 1 : 0;
-        // forward (rhs, metrics, iterate, factors, e) + backward (e, cpr, q) + last pass (q, v, w, p)
-        const double bytes = zin ? N * (sr + sm + 8 + 4 * sp + 4 + seg)
-                                 : N * (9 * sm + sp + sr + 8 + sp + sp + 4 + sp + sp + sp + seg);
+        // forward: rhs, metrics, iterate, invm, cpr; last pass: (v, w), write.  The staged variant
+        // adds the e write/read, the cpr re-read and the q write/read.
+        const double stage = reg_rows(L.G.ny / L.P) ? 0.0 : (4 * sp + 4);
+        const double bytes = zin ? N * (sr + sm + 8 + sp + seg + stage)
+                                 : N * (9 * sm + sp + sr + 8 + sp + seg + stage);
         {
             KT kt(h, fine ? PLES_MG_K_SMOOTH_FINE : PLES_MG_K_SMOOTH, bytes);
             switch (L.P) {
+                case 16: launch_yline<16, T, TR>(h, L, rhs, zin); break;
                 case 8: launch_yline<8, T, TR>(h, L, rhs, zin); break;
                 case 4: launch_yline<4, T, TR>(h, L, rhs, zin); break;
                 case 2: launch_yline<2, T, TR>(h, L, rhs, zin); break;
@@ -375,11 +408,44 @@ int lv_prolong(H* h, int n) {  // n+1 -> n
     return prolong_to<double, double>(h, Cc, F);
 }
 
+// coarsest level in one launch of one workgroup when its columns fit a block
+// and it has no xi/zeta walls (whose ghosts need the separate halo launches)
+bool coarse_one_block(const Level& L) {
+    return !L.f32 && L.G.ip == 0 && L.G.kp == 0 && (long)L.G.nx * L.G.nz <= 1024;
+}
+
+size_t coarse_lds_bytes(const Level& L) { return (size_t)L.G.n * (12 * sizeof(double) + 2 * sizeof(float)); }
+
+int coarse_solve(H* h, Level& L, const double* rhs, int nu) {
+    const int ncol = L.G.nx * L.G.nz;
+    KT kt(h, PLES_MG_K_SMOOTH, (double)L.ncell() * (9 * 8 + 8 + 8 + 8 + 8));
+    const size_t lds = coarse_lds_bytes(L);
+    if (lds <= 64 * 1024) {
+        // result lands in p[1 - cur]
+        hipLaunchKernelGGL((k_coarse_lds<double>), dim3(1), dim3(256), lds, h->s, L.G, met<double>(L), line_f(L),
+                           (double*)L.p[1 - L.cur], rhs, (double)h->o.omega, 1.0 / L.pot, nu);
+        CK(hipGetLastError());
+        L.cur ^= 1;
+        return PLES_MG_OK;
+    }
+    const int threads = std::min(1024, ((ncol + 63) / 64) * 64);
+    hipLaunchKernelGGL((k_coarse<double, double>), dim3(1), dim3(threads), 0, h->s, L.G, met<double>(L), line_f(L),
+                       (double*)L.p[L.cur], (double*)L.p[1 - L.cur], rhs, (double)h->o.omega, 1.0 / L.pot, nu);
+    CK(hipGetLastError());
+    if (nu & 1) L.cur ^= 1;
+    return PLES_MG_OK;
+}
+
 // z = V(r0): one V-cycle for A z = r0 from a zero initial guess
 int vcycle(H* h, const double* r0) {
     const int L = h->L;
     for (int n = 0; n < L; ++n) {
         const int nu = (n == L - 1) ? h->o.nu_coarse : h->o.nu_pre;
+        Level& lvn = h->lv[n];
+        if (n == L - 1 && coarse_one_block(lvn)) {
+            CKS(coarse_solve(h, lvn, n == 0 ? r0 : (const double*)lvn.b, std::max(1, nu)));
+            continue;
+        }
         CKS(lv_smooth(h, n, r0, std::max(1, nu), true));
         if (n < L - 1) {
             CKS(lv_resid(h, n, r0));
@@ -580,7 +646,7 @@ int compute_metrics(H* h, int n, double* const g[9], double* const v[9]) {
 
 int level_factors(H* h, Level& L) {
     if ((L.P > 1) != (L.lv_v != nullptr && L.lv_w != nullptr)) return PLES_MG_ERR_ARG;
-    if (L.G.ny % L.P != 0 || L.P > 8) return PLES_MG_ERR_ARG;
+    if (L.G.ny % L.P != 0 || L.P > 16) return PLES_MG_ERR_ARG;
     const ColLaunch cl = seg_launch(L.G, L.P);
     if (L.f32)
         hipLaunchKernelGGL((k_line_factors<float>), cl.grid, cl.block, 0, h->s, L.G, met<float>(L), 1.0 / L.pot,
@@ -635,7 +701,7 @@ void ples_mg_default_opts(ples_mg_opts* o) {
     o->precision = PLES_MG_FP64;
     o->nu_pre = 2;
     o->nu_post = 2;
-    o->nu_coarse = 24;
+    o->nu_coarse = 16;
     o->omega = 0.8;
     o->krylov = 1;
     o->stop_mode = PLES_MG_STOP_REL_L2;
