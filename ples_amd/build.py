@@ -19,6 +19,27 @@ FLAGS = [
 ]
 
 
+FC = os.environ.get("PLES_FC", "/opt/rocm/bin/amdflang")
+FDIR = os.path.join(HERE, "fortran")
+FDRIVER = os.path.join(LIBDIR, "ples_mg_driver")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Fortran binding module + the test driver, linked against libples_mg.so."""
+    mod = os.path.join(LIBDIR, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    cmds = [
+        [FC, "-O2", "-c", os.path.join(FDIR, "ples_mg_f.f90"), "-J", mod, "-o", os.path.join(mod, "ples_mg_f.o")],
+        [FC, "-O2", os.path.join(FDIR, "ples_mg_driver.f90"), os.path.join(mod, "ples_mg_f.o"), "-J", mod,
+         "-I", mod, "-L", LIBDIR, "-lples_mg", "-Wl,-rpath," + LIBDIR, "-o", FDRIVER],
+    ]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    return FDRIVER
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     src = os.path.join(CSRC, "ples_mg.hip")
@@ -28,6 +49,8 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    if os.path.exists(FC):
+        build_fortran(verbose)
     return LIB
 
 

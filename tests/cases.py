@@ -66,3 +66,10 @@ def random_cs(g, seed, scale=1e-2):
         cs[4][:] = scale * rng.standard_normal(cs[4].shape)
         cs[5][:] = scale * rng.standard_normal(cs[5].shape)
     return cs
+
+
+def driver_grid():
+    """Nodes of the Fortran driver's grid (ples_amd/fortran/ples_mg_driver.f90):
+    32x16x32, tanh 1.5 stretching, terrain amplitude 0.2."""
+    g = grid(32, 16, 32, gamma=1.5, amp=0.2)
+    return g.x, g.y, g.z
