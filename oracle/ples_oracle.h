@@ -29,6 +29,15 @@
  *     first index fastest.
  *   - periodic flags follow the reference (Aboundary.in): ip/jp/kp = 1 means
  *     walls (non-periodic), 0 means periodic.
+ *
+ * What pins each part (tests/test_oracle.py): metrics exact on affine maps at
+ * every level and on quadratic maps; operator exact on quadratics; wall ghosts
+ * reproduce an affine pressure (eta walls); restriction/prolongation
+ * invariants; orc_multi equal to the exact FFT/tridiagonal channel solution
+ * and to a dense solve on a warped grid; case rhs = divergence of the face
+ * fluxes; gradie exact on an affine pressure.  Parity unpinned: the
+ * edge/corner ghost copy rules with xi/zeta walls
+ * (fsrc/multigrid.f03:1358-1742) and the per-cycle SOR residual trajectory.
  */
 #ifndef PLES_ORACLE_H
 #define PLES_ORACLE_H
