@@ -140,6 +140,55 @@ int ples_mg_gradie(ples_mg_handle* h, const double* fi, int deepl, int deepr, do
                    double* gra1, double* gra2, double* gra3, double* cgra1, double* cgra2,
                    double* cgra3);
 
+/* ---- the time-loop steps around the solve ----
+ * Cell velocities u, v, w are shaped like fi (0:jx+1, 0:jy+1, 1-deepl:jz+deepr)
+ * (fsrc/iniz.f03:95-97); face fluxes uc (0:jx,1:jy,1:jz), vc (1:jx,0:jy,1:jz),
+ * wc (1:jx,1:jy,0:jz) (fsrc/iniz.f03:103-105); cs1..cs6 as above.
+ * potenziale = 1 is the reference's potential-flow switch.  Walls need at
+ * least three cells across (PLES_MG_ERR_BAD_DIMS otherwise). */
+
+/* update (fsrc/update.f03:2-289): u,v,w += delu,delv,delw in the interior,
+ * then every ghost face (not edge) from the interior: parabolic extrapolation
+ * 1.875/-1.25/0.375 at walls, periodic copies otherwise.  delu.. are
+ * (0:jx+1, 0:jy+1, 0:jz+1) (fsrc/stratiParticle.f03:315); only the interior
+ * is read. */
+int ples_mg_update(ples_mg_handle* h, double* u, double* v, double* w, int deepl, int deepr,
+                   const double* delu, const double* delv, const double* delw);
+
+/* contra (fsrc/contra.f03:2-241): intermediate contravariant fluxes uc,vc,wc
+ * (in/out) from u,v,w, and on wall faces the flux change cs = new - old
+ * (potenziale = 1: cs = -old) written to cs[0..5] = cs1..cs6 for the wall
+ * directions (entries may be NULL).  The handle keeps cs as the boundary
+ * fluxes of its next resident solve. */
+int ples_mg_contra(ples_mg_handle* h, const double* u, const double* v, const double* w, int deepl,
+                   int deepr, double* uc, double* vc, double* wc, double* const cs[6], int potenziale);
+
+/* vel_up (fsrc/vel_up.f03:2-313, freesurface = 0): u,v,w += gra1..3
+ * (1:jx,1:jy,1:jz), periodic ghost copies, uc,vc,wc -= dt*cgra1..3 on the
+ * non-wall faces (cgra in the uc,vc,wc shapes, as ples_mg_gradie returns
+ * them). */
+int ples_mg_vel_up(ples_mg_handle* h, double* u, double* v, double* w, int deepl, int deepr,
+                   double* uc, double* vc, double* wc, const double* const gra[3],
+                   const double* const cgra[3], double dt, int potenziale);
+
+/* check_divergence (fsrc/check_divergence.f03:131-190, bodyforce = 0):
+ * out = {max|div| inside the field (i=2..jx-1, j=2..jy-1, k=2..jz-1),
+ *        max|div/giac| inside, max|div| all cells, max|div/giac| all}. */
+int ples_mg_check_divergence(ples_mg_handle* h, const double* uc, const double* vc,
+                             const double* wc, double out[4]);
+
+/* The projection step of the time loop in one call, on the device:
+ * update -> contra -> diver -> (compatibility projection) -> multi ->
+ * gradie -> vel_up -> check_divergence
+ * (fsrc/stratiParticle.f03:2678-2937, 3526).  All arrays in/out as in the
+ * single-step entry points; fi is the warm start and the result; cs may be
+ * NULL; div (may be NULL) receives the check_divergence maxima. */
+int ples_mg_projection_step(ples_mg_handle* h, double* u, double* v, double* w, int deepl, int deepr,
+                            const double* delu, const double* delv, const double* delw,
+                            double* uc, double* vc, double* wc, double* const cs[6], double* fi,
+                            double dt, double eps, int ficycle, int potenziale,
+                            ples_mg_result* res, double div[4]);
+
 /* ---- device-resident use (inputs stay in HBM between calls) ---- */
 
 /* Synthetic divergence rhs on the device: N(0,1) cell velocity from a

@@ -61,6 +61,11 @@ def lib():
         L.orc_project.restype = C.c_double
         L.orc_gradie.argtypes = [C.POINTER(Dims), _dp, C.POINTER(_dp), _dp, C.POINTER(_dp),
                                  C.c_double] + [_dp] * 6
+        L.orc_update.argtypes = [C.POINTER(Dims), C.POINTER(_dp), C.POINTER(_dp)]
+        L.orc_contra_step.argtypes = [C.POINTER(Dims), C.POINTER(_dp), C.POINTER(_dp), C.POINTER(_dp),
+                                      C.POINTER(_dp), C.c_int]
+        L.orc_vel_up.argtypes = [C.POINTER(Dims)] + [C.POINTER(_dp)] * 4 + [C.c_double, C.c_int]
+        L.orc_check_divergence.argtypes = [C.POINTER(Dims), C.POINTER(_dp), _dp, _dp]
         for f in ("orc_absmax", "orc_l2", "orc_mean"):
             getattr(L, f).argtypes = [C.POINTER(Dims), _dp]
             getattr(L, f).restype = C.c_double
@@ -218,3 +223,27 @@ def l2(g: Grid, r, level=0):
 def mean(g: Grid, p, level=0):
     d = g.dims(level)
     return lib().orc_mean(C.byref(d), _p(p))
+
+
+def update(g: Grid, vel, dvel):
+    """In place on vel (3 ghosted arrays)."""
+    lib().orc_update(C.byref(g.dims(0)), _parr(vel), _parr(dvel))
+
+
+def contra_step(g: Grid, csv, vel, fc, potenziale=0):
+    """In place on fc = [uc, vc, wc] (ghosted face layout); returns cs1..cs6
+    in the reference shapes as numpy (C order: cs1 (nz, ny) ...)."""
+    cs = [np.zeros((g.nz, g.ny)), np.zeros((g.nz, g.ny)), np.zeros((g.nz, g.nx)), np.zeros((g.nz, g.nx)),
+          np.zeros((g.ny, g.nx)), np.zeros((g.ny, g.nx))]
+    lib().orc_contra_step(C.byref(g.dims(0)), _parr(csv), _parr(vel), _parr(fc), _parr(cs), potenziale)
+    return cs
+
+
+def vel_up(g: Grid, vel, fc, gra, cgra, dt=1.0, potenziale=0):
+    lib().orc_vel_up(C.byref(g.dims(0)), _parr(vel), _parr(fc), _parr(gra), _parr(cgra), dt, potenziale)
+
+
+def check_divergence(g: Grid, fc, giac):
+    out = np.zeros(4)
+    lib().orc_check_divergence(C.byref(g.dims(0)), _parr(fc), _p(giac), _p(out))
+    return out

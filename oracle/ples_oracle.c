@@ -924,6 +924,198 @@ void orc_gradie(const orc_dims* d, const double* fi, const double* const csv[9],
 }
 
 /* ------------------------------------------------------------------ */
+/* time-loop neighbours of the solve: update, contra, vel_up, divergence */
+/* ------------------------------------------------------------------ */
+
+/* Parabolic extrapolation to the wall face from the first three cells. */
+static double parab(double a1, double a2, double a3) { return 1.875 * a1 - 1.25 * a2 + .375 * a3; }
+
+void orc_update(const orc_dims* d, double* const vel[3], const double* const del[3]) {
+    /* fsrc/update.f03:29-38 interior increment */
+    for (int c = 0; c < 3; ++c) {
+        double* u = vel[c];
+        const double* du = del[c];
+        for (int k = 1; k <= d->nz; ++k)
+            for (int j = 1; j <= d->ny; ++j)
+                for (int i = 1; i <= d->nx; ++i) A(u, i, j, k) += A(du, i, j, k);
+    }
+    for (int c = 0; c < 3; ++c) {
+        double* u = vel[c];
+        const int nx = d->nx, ny = d->ny, nz = d->nz;
+        /* fsrc/update.f03:43-91: xi sides, walls (ip=1) or periodic copies */
+        for (int k = 1; k <= nz; ++k)
+            for (int j = 1; j <= ny; ++j) {
+                if (d->ip == 1) {
+                    A(u, 0, j, k) = parab(A(u, 1, j, k), A(u, 2, j, k), A(u, 3, j, k));
+                    A(u, nx + 1, j, k) = parab(A(u, nx, j, k), A(u, nx - 1, j, k), A(u, nx - 2, j, k));
+                } else {
+                    A(u, 0, j, k) = A(u, nx, j, k);
+                    A(u, nx + 1, j, k) = A(u, 1, j, k);
+                }
+            }
+        /* fsrc/update.f03:93-139: eta sides */
+        for (int k = 1; k <= nz; ++k)
+            for (int i = 1; i <= nx; ++i) {
+                if (d->jp == 1) {
+                    A(u, i, 0, k) = parab(A(u, i, 1, k), A(u, i, 2, k), A(u, i, 3, k));
+                    A(u, i, ny + 1, k) = parab(A(u, i, ny, k), A(u, i, ny - 1, k), A(u, i, ny - 2, k));
+                } else {
+                    A(u, i, 0, k) = A(u, i, ny, k);
+                    A(u, i, ny + 1, k) = A(u, i, 1, k);
+                }
+            }
+        /* fsrc/update.f03:141-235: zeta sides (the periodic plane exchange
+         * copies i = 1..nx, j = 1..ny) */
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                if (d->kp == 1) {
+                    A(u, i, j, 0) = parab(A(u, i, j, 1), A(u, i, j, 2), A(u, i, j, 3));
+                    A(u, i, j, nz + 1) = parab(A(u, i, j, nz), A(u, i, j, nz - 1), A(u, i, j, nz - 2));
+                } else {
+                    A(u, i, j, 0) = A(u, i, j, nz);
+                    A(u, i, j, nz + 1) = A(u, i, j, 1);
+                }
+            }
+    }
+}
+
+/* wall face flux from the ghost-cell velocity and the face area vector */
+static double wall_flux(const orc_dims* d, const double* const vel[3], const double* const cv[3], int gi,
+                        int gj, int gk, int fi, int fj, int fk) {
+    return A(vel[0], gi, gj, gk) * A(cv[0], fi, fj, fk) + A(vel[1], gi, gj, gk) * A(cv[1], fi, fj, fk) +
+           A(vel[2], gi, gj, gk) * A(cv[2], fi, fj, fk);
+}
+
+void orc_contra_step(const orc_dims* d, const double* const csv[9], const double* const vel[3],
+                     double* const fc[3], double* const cs[6], int potenziale) {
+    const int nx = d->nx, ny = d->ny, nz = d->nz;
+    const double* cx[3] = {csv[0], csv[1], csv[2]};
+    const double* ex[3] = {csv[3], csv[4], csv[5]};
+    const double* zx[3] = {csv[6], csv[7], csv[8]};
+    double *uc = fc[0], *vc = fc[1], *wc = fc[2];
+    /* fsrc/contra.f03:35-65: xi walls; cs1/cs2 (ny,nz) */
+    if (d->ip == 1)
+        for (int k = 1; k <= nz; ++k)
+            for (int j = 1; j <= ny; ++j) {
+                const size_t o = (size_t)(j - 1) + (size_t)ny * (k - 1);
+                double f0 = wall_flux(d, vel, cx, 0, j, k, 0, j, k);
+                cs[0][o] = potenziale ? -A(uc, 0, j, k) : f0 - A(uc, 0, j, k);
+                A(uc, 0, j, k) = f0;
+                double f1 = wall_flux(d, vel, cx, nx + 1, j, k, nx, j, k);
+                cs[1][o] = potenziale ? -A(uc, nx, j, k) : f1 - A(uc, nx, j, k);
+                A(uc, nx, j, k) = f1;
+            }
+    /* fsrc/contra.f03:70-87 */
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = d->ip; i <= nx - d->ip; ++i)
+                A(uc, i, j, k) = .5 * (A(vel[0], i, j, k) + A(vel[0], i + 1, j, k)) * A(cx[0], i, j, k) +
+                                 .5 * (A(vel[1], i, j, k) + A(vel[1], i + 1, j, k)) * A(cx[1], i, j, k) +
+                                 .5 * (A(vel[2], i, j, k) + A(vel[2], i + 1, j, k)) * A(cx[2], i, j, k);
+    /* fsrc/contra.f03:95-127: eta walls; cs3/cs4 (nx,nz) */
+    if (d->jp == 1)
+        for (int k = 1; k <= nz; ++k)
+            for (int i = 1; i <= nx; ++i) {
+                const size_t o = (size_t)(i - 1) + (size_t)nx * (k - 1);
+                double f0 = wall_flux(d, vel, ex, i, 0, k, i, 0, k);
+                cs[2][o] = potenziale ? -A(vc, i, 0, k) : f0 - A(vc, i, 0, k);
+                A(vc, i, 0, k) = f0;
+                double f1 = wall_flux(d, vel, ex, i, ny + 1, k, i, ny, k);
+                cs[3][o] = potenziale ? -A(vc, i, ny, k) : f1 - A(vc, i, ny, k);
+                A(vc, i, ny, k) = f1;
+            }
+    /* fsrc/contra.f03:132-146 */
+    for (int k = 1; k <= nz; ++k)
+        for (int j = d->jp; j <= ny - d->jp; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(vc, i, j, k) = .5 * (A(vel[0], i, j, k) + A(vel[0], i, j + 1, k)) * A(ex[0], i, j, k) +
+                                 .5 * (A(vel[1], i, j, k) + A(vel[1], i, j + 1, k)) * A(ex[1], i, j, k) +
+                                 .5 * (A(vel[2], i, j, k) + A(vel[2], i, j + 1, k)) * A(ex[2], i, j, k);
+    /* fsrc/contra.f03:152-194: zeta walls on the first / last slab; cs5/cs6 (nx,ny) */
+    if (d->kp == 1)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const size_t o = (size_t)(i - 1) + (size_t)nx * (j - 1);
+                double f0 = wall_flux(d, vel, zx, i, j, 0, i, j, 0);
+                cs[4][o] = potenziale ? -A(wc, i, j, 0) : f0 - A(wc, i, j, 0);
+                A(wc, i, j, 0) = f0;
+                double f1 = wall_flux(d, vel, zx, i, j, nz + 1, i, j, nz);
+                cs[5][o] = potenziale ? -A(wc, i, j, nz) : f1 - A(wc, i, j, nz);
+                A(wc, i, j, nz) = f1;
+            }
+    /* fsrc/contra.f03:198-221 (k = kp..nz-kp over the slabs) */
+    for (int k = d->kp; k <= nz - d->kp; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(wc, i, j, k) = .5 * (A(vel[0], i, j, k) + A(vel[0], i, j, k + 1)) * A(zx[0], i, j, k) +
+                                 .5 * (A(vel[1], i, j, k) + A(vel[1], i, j, k + 1)) * A(zx[1], i, j, k) +
+                                 .5 * (A(vel[2], i, j, k) + A(vel[2], i, j, k + 1)) * A(zx[2], i, j, k);
+}
+
+void orc_vel_up(const orc_dims* d, double* const vel[3], double* const fc[3], const double* const gra[3],
+                const double* const cgra[3], double dt, int potenziale) {
+    const int nx = d->nx, ny = d->ny, nz = d->nz;
+    /* fsrc/vel_up.f03:39-80: cell velocities */
+    for (int c = 0; c < 3; ++c)
+        for (int k = 1; k <= nz; ++k)
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 1; i <= nx; ++i)
+                    A(vel[c], i, j, k) = (potenziale ? 0.0 : A(vel[c], i, j, k)) + A(gra[c], i, j, k);
+    /* fsrc/vel_up.f03:85-101: periodic xi copies, j = 0..ny+1 */
+    if (d->ip == 0)
+        for (int c = 0; c < 3; ++c)
+            for (int k = 1; k <= nz; ++k)
+                for (int j = 0; j <= ny + 1; ++j) {
+                    A(vel[c], 0, j, k) = A(vel[c], nx, j, k);
+                    A(vel[c], nx + 1, j, k) = A(vel[c], 1, j, k);
+                }
+    /* fsrc/vel_up.f03:133-186: periodic zeta planes, i = 1..nx, j = 0..ny+1
+     * (eta is never periodic for this solver) */
+    if (d->kp == 0)
+        for (int c = 0; c < 3; ++c)
+            for (int j = 0; j <= ny + 1; ++j)
+                for (int i = 1; i <= nx; ++i) {
+                    A(vel[c], i, j, 0) = A(vel[c], i, j, nz);
+                    A(vel[c], i, j, nz + 1) = A(vel[c], i, j, 1);
+                }
+    /* fsrc/vel_up.f03:191-289: contravariant fluxes on the non-wall faces */
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = d->ip; i <= nx - d->ip; ++i)
+                A(fc[0], i, j, k) = (potenziale ? 0.0 : A(fc[0], i, j, k)) - dt * A(cgra[0], i, j, k);
+    for (int k = 1; k <= nz; ++k)
+        for (int j = d->jp; j <= ny - d->jp; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(fc[1], i, j, k) = (potenziale ? 0.0 : A(fc[1], i, j, k)) - dt * A(cgra[1], i, j, k);
+    for (int k = d->kp; k <= nz - d->kp; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                A(fc[2], i, j, k) = (potenziale ? 0.0 : A(fc[2], i, j, k)) - dt * A(cgra[2], i, j, k);
+}
+
+void orc_check_divergence(const orc_dims* d, const double* const fc[3], const double* giac, double out[4]) {
+    /* fsrc/check_divergence.f03:131-190 (bodyforce = 0): max |div| and
+     * max |div/giac|, first over i=2..nx-1, j=2..ny-1, k=2..nz-1 ("inside
+     * field"), then over all cells */
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 1; k <= d->nz; ++k)
+        for (int j = 1; j <= d->ny; ++j)
+            for (int i = 1; i <= d->nx; ++i) {
+                const double div = A(fc[0], i, j, k) - A(fc[0], i - 1, j, k) + A(fc[1], i, j, k) -
+                                   A(fc[1], i, j - 1, k) + A(fc[2], i, j, k) - A(fc[2], i, j, k - 1);
+                const double a = fabs(div), ag = fabs(div / A(giac, i, j, k));
+                const int inside = i >= 2 && i <= d->nx - 1 && j >= 2 && j <= d->ny - 1 && k != 1 && k != d->nz;
+                if (inside) {
+                    if (a > m[0]) m[0] = a;
+                    if (ag > m[1]) m[1] = ag;
+                }
+                if (a > m[2]) m[2] = a;
+                if (ag > m[3]) m[3] = ag;
+            }
+    for (int q = 0; q < 4; ++q) out[q] = m[q];
+}
+
+/* ------------------------------------------------------------------ */
 /* norms                                                               */
 /* ------------------------------------------------------------------ */
 

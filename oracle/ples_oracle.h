@@ -154,6 +154,32 @@ void orc_gradie(const orc_dims* d, const double* fi, const double* const csv[9],
                 const double* giac, const double* const g[9], double dt, double* gra1,
                 double* gra2, double* gra3, double* cgra1, double* cgra2, double* cgra3);
 
+/* ---- time-loop neighbours of the solve ------------------------------ */
+
+/* update (fsrc/update.f03:29-235): vel[c] += del[c] in the interior, then
+ * the ghost faces (not edges) of each direction: parabolic extrapolation
+ * 1.875/-1.25/0.375 from the first three cells at walls, periodic copies
+ * otherwise.  vel, del ghosted cell layout. */
+void orc_update(const orc_dims* d, double* const vel[3], const double* const del[3]);
+
+/* contra (fsrc/contra.f03:35-221): face fluxes fc = (uc, vc, wc) (ghosted
+ * face layout, in/out).  Wall faces take the ghost-cell velocity dotted
+ * with the face vector and return cs = new - old wall flux (potenziale = 1:
+ * cs = -old) in cs1..cs6 (reference shapes; only wall directions written);
+ * other faces take the average of the two cells. */
+void orc_contra_step(const orc_dims* d, const double* const csv[9], const double* const vel[3],
+                     double* const fc[3], double* const cs[6], int potenziale);
+
+/* vel_up (fsrc/vel_up.f03:39-289, freesurface = 0): vel += gra (= gra if
+ * potenziale), periodic xi / zeta ghost copies, fc -= dt*cgra on the
+ * non-wall faces (fc = -dt*cgra if potenziale). */
+void orc_vel_up(const orc_dims* d, double* const vel[3], double* const fc[3], const double* const gra[3],
+                const double* const cgra[3], double dt, int potenziale);
+
+/* check_divergence (fsrc/check_divergence.f03:131-190, bodyforce = 0):
+ * out = {max|div| inside, max|div/giac| inside, max|div| all, max|div/giac| all}. */
+void orc_check_divergence(const orc_dims* d, const double* const fc[3], const double* giac, double out[4]);
+
 /* ---- norms ------------------------------------------------------------ */
 double orc_absmax(const orc_dims* d, const double* r);
 double orc_l2(const orc_dims* d, const double* r);

@@ -11,6 +11,7 @@
 
 #include "../../include/ples_mg.h"
 #include "mg_kernels.hpp"
+#include "proj_kernels.hpp"
 
 using namespace plesmg;
 
@@ -67,6 +68,17 @@ struct ples_mg_handle {
     std::vector<Rec> recs;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     double last_removed = 0.0;
+    // projection-step state (allocated on first use): level-1 face vectors
+    // and Jacobian, cell velocities, face fluxes, increments, gradients
+    bool pj_ready = false;
+    double* pj_csv[9] = {};
+    double* pj_giac = nullptr;
+    double* pj_vel[3] = {};
+    double* pj_fc[3] = {};
+    double* pj_del[3] = {};
+    double* pj_gra[3] = {};
+    double* pj_cgra[3] = {};
+    double* pj_dmax = nullptr;
 };
 
 namespace {
@@ -693,6 +705,15 @@ size_t estimate(const ples_mg_grid* g, const ples_mg_opts* o) {
 
 }  // namespace
 
+// ---------------------------------------------------------------- projection step
+dim3 box_grid(const Geo& G) { return dim3((unsigned)((G.nx + 2 + 63) / 64), (unsigned)((G.ny + 2 + 3) / 4), (unsigned)(G.nz + 2)); }
+const dim3 kBoxBlock(64, 4);
+
+// wall directions need three interior cells for the parabolic extrapolation
+bool proj_dims_ok(const Geo& G) {
+    return G.ny >= 3 && (G.ip == 0 || G.nx >= 3) && (G.kp == 0 || G.nz >= 3);
+}
+
 // ====================================================================== C ABI
 extern "C" {
 
@@ -761,8 +782,8 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
     CK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
     CK(hipEventCreate(&h->t0));
     CK(hipEventCreate(&h->t1));
-    CK(hipHostMalloc((void**)&h->Sh, S_COUNT * sizeof(double), hipHostMallocDefault));
-    std::memset(h->Sh, 0, S_COUNT * sizeof(double));
+    CK(hipHostMalloc((void**)&h->Sh, (S_COUNT + 4) * sizeof(double), hipHostMallocDefault));
+    std::memset(h->Sh, 0, (S_COUNT + 4) * sizeof(double));
     h->jx = g->jx; h->jy = g->jy; h->jz = g->jz;
     h->ip = g->ip; h->jp = g->jp; h->kp = g->kp;
     h->L = g->nlevel;
@@ -1227,6 +1248,283 @@ int ples_mg_kernel_stats(ples_mg_handle* h, int fam, long* launches, double* ms,
     if (ms) *ms = t;
     if (bytes) *bytes = by;
     return PLES_MG_OK;
+}
+
+
+// ---------------------------------------------------------------- projection step
+static int ensure_proj(H* h) {
+    if (h->pj_ready) return PLES_MG_OK;
+    const Geo& G = h->lv[0].G;
+    if (!proj_dims_ok(G)) return PLES_MG_ERR_BAD_DIMS;
+    const size_t cells = (size_t)G.n;
+    for (int q = 0; q < 9; ++q) CKS(dalloc(h, (void**)&h->pj_csv[q], cells * 8));
+    CKS(dalloc(h, (void**)&h->pj_giac, cells * 8));
+    for (int q = 0; q < 3; ++q) {
+        CKS(dalloc(h, (void**)&h->pj_vel[q], cells * 8));
+        CKS(dalloc(h, (void**)&h->pj_fc[q], cells * 8));
+        CKS(dalloc(h, (void**)&h->pj_del[q], cells * 8));
+        CKS(dalloc(h, (void**)&h->pj_gra[q], cells * 8));
+        CKS(dalloc(h, (void**)&h->pj_cgra[q], cells * 8));
+    }
+    CKS(dalloc(h, (void**)&h->pj_dmax, 4 * sizeof(double)));
+    double* gg[9] = {};
+    int st = PLES_MG_OK;
+    for (int q = 0; q < 9 && st == PLES_MG_OK; ++q) st = dalloc(h, (void**)&gg[q], cells * 8);
+    if (st == PLES_MG_OK) st = compute_metrics(h, 0, gg, h->pj_csv);
+    if (st == PLES_MG_OK) {
+        dim3 block(64, 4);
+        dim3 grid((G.nx + 63) / 64, (G.ny + 3) / 4, G.nz);
+        hipLaunchKernelGGL(k_giac, grid, block, 0, h->s, nodes_of(h), G, h->pj_giac);
+        if (hipGetLastError() != hipSuccess) st = PLES_MG_ERR_HIP;
+    }
+    (void)hipStreamSynchronize(h->s);
+    for (int q = 0; q < 9; ++q)
+        if (gg[q]) dfree(h, gg[q]);
+    if (st == PLES_MG_OK) h->pj_ready = true;
+    return st;
+}
+
+// cell arrays shaped like fi: (0:jx+1, 0:jy+1, 1-deepl:jz+deepr), planes 0..jz+1 used
+static int upload_cells(H* h, double* dst, const double* host, int deepl) {
+    const Geo& G = h->lv[0].G;
+    CK(hipMemcpyAsync(dst, host + (size_t)(deepl - 1) * G.sz, (size_t)G.n * 8, hipMemcpyHostToDevice, h->s));
+    return PLES_MG_OK;
+}
+static int download_cells(H* h, double* host, const double* src, int deepl) {
+    const Geo& G = h->lv[0].G;
+    CK(hipMemcpyAsync(host + (size_t)(deepl - 1) * G.sz, src, (size_t)G.n * 8, hipMemcpyDeviceToHost, h->s));
+    return PLES_MG_OK;
+}
+
+// face flux boxes in the reference shapes: uc (0:jx,1:jy,1:jz), vc (1:jx,0:jy,1:jz), wc (1:jx,1:jy,0:jz)
+static int upload_faces(H* h, double* const dst[3], const double* const fc[3]) {
+    const Geo& G = h->lv[0].G;
+    for (int d = 0; d < 3; ++d) {
+        int i0, i1, j0, j1, k0, k1;
+        face_box(G, d, i0, i1, j0, j1, k0, k1);
+        CK(hipMemsetAsync(dst[d], 0, (size_t)G.n * 8, h->s));
+        CKS(pack_box<double>(h, G, i0, i1, j0, j1, k0, k1, fc[d], dst[d], h->W[1]));
+        CK(hipStreamSynchronize(h->s));  // the staging buffer is reused
+    }
+    return PLES_MG_OK;
+}
+static int download_faces(H* h, double* const fc[3], double* const src[3]) {
+    const Geo& G = h->lv[0].G;
+    for (int d = 0; d < 3; ++d) {
+        int i0, i1, j0, j1, k0, k1;
+        face_box(G, d, i0, i1, j0, j1, k0, k1);
+        CKS(unpack_box<double>(h, G, i0, i1, j0, j1, k0, k1, src[d], fc[d], h->W[1]));
+    }
+    return PLES_MG_OK;
+}
+
+static int launch_update(H* h) {
+    const Geo& G = h->lv[0].G;
+    Vel3 V{{h->pj_vel[0], h->pj_vel[1], h->pj_vel[2]}};
+    CVel3 D{{h->pj_del[0], h->pj_del[1], h->pj_del[2]}};
+    hipLaunchKernelGGL(k_update_cells, box_grid(G), kBoxBlock, 0, h->s, G, V, D);
+    hipLaunchKernelGGL(k_update_ghosts, box_grid(G), kBoxBlock, 0, h->s, G, V);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+// contra into the resident face fluxes; cs into the solver's boundary fluxes
+static int launch_contra(H* h, int potenziale) {
+    const Geo& G = h->lv[0].G;
+    CsvP V;
+    for (int q = 0; q < 9; ++q) V.v[q] = h->pj_csv[q];
+    CVel3 U{{h->pj_vel[0], h->pj_vel[1], h->pj_vel[2]}};
+    Vel3 F{{h->pj_fc[0], h->pj_fc[1], h->pj_fc[2]}};
+    CsOut cs{{h->cs[0], h->cs[1], h->cs[2], h->cs[3], h->cs[4], h->cs[5]}};
+    hipLaunchKernelGGL(k_contra, box_grid(G), kBoxBlock, 0, h->s, G, V, U, F, cs, potenziale);
+    CK(hipGetLastError());
+    h->have_cs = true;
+    return PLES_MG_OK;
+}
+
+static int launch_gradie(H* h, const double* fi, double dt) {
+    const Level& L0 = h->lv[0];
+    const Geo& G = L0.G;
+    CsvP V;
+    for (int q = 0; q < 9; ++q) V.v[q] = h->pj_csv[q];
+    dim3 block(64, 4);
+    dim3 grid((G.nx + 1 + 63) / 64, (G.ny + 1 + 3) / 4, G.nz + 1);
+    hipLaunchKernelGGL(k_gradie, grid, block, 0, h->s, G, fi, V, (const double*)h->pj_giac, met<double>(L0), dt,
+                       h->pj_gra[0], h->pj_gra[1], h->pj_gra[2], h->pj_cgra[0], h->pj_cgra[1], h->pj_cgra[2]);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+static int launch_vel_up(H* h, double dt, int potenziale) {
+    const Geo& G = h->lv[0].G;
+    Vel3 U{{h->pj_vel[0], h->pj_vel[1], h->pj_vel[2]}};
+    Vel3 F{{h->pj_fc[0], h->pj_fc[1], h->pj_fc[2]}};
+    CVel3 gr{{h->pj_gra[0], h->pj_gra[1], h->pj_gra[2]}};
+    CVel3 cg{{h->pj_cgra[0], h->pj_cgra[1], h->pj_cgra[2]}};
+    hipLaunchKernelGGL(k_vel_up, box_grid(G), kBoxBlock, 0, h->s, G, U, F, gr, cg, dt, potenziale);
+    hipLaunchKernelGGL(k_vel_up_images, box_grid(G), kBoxBlock, 0, h->s, G, U);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+static int launch_divmax(H* h, double out[4]) {
+    const Geo& G = h->lv[0].G;
+    const ColLaunch cl = col_launch(G);
+    CVel3 F{{h->pj_fc[0], h->pj_fc[1], h->pj_fc[2]}};
+    hipLaunchKernelGGL(k_divmax, cl.grid, cl.block, 0, h->s, G, F, (const double*)h->pj_giac, h->part, cl.nblocks);
+    hipLaunchKernelGGL(k_fin_max, dim3(1), dim3(1), 0, h->s, (const double*)h->part, cl.nblocks, 4, h->pj_dmax);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(h->Sh + S_COUNT, h->pj_dmax, 4 * sizeof(double), hipMemcpyDeviceToHost, h->s));
+    CK(hipStreamSynchronize(h->s));
+    for (int q = 0; q < 4; ++q) out[q] = h->Sh[S_COUNT + q];
+    return PLES_MG_OK;
+}
+
+static int download_cs(H* h, double* const cs[6]) {
+    const size_t csz[6] = {(size_t)h->jy * h->jz, (size_t)h->jy * h->jz, (size_t)h->jx * h->jz,
+                           (size_t)h->jx * h->jz, (size_t)h->jx * h->jy, (size_t)h->jx * h->jy};
+    const int wall[3] = {h->ip, h->jp, h->kp};
+    for (int q = 0; q < 6; ++q)
+        if (cs && cs[q] && wall[q / 2])
+            CK(hipMemcpyAsync(cs[q], h->cs[q], csz[q] * 8, hipMemcpyDeviceToHost, h->s));
+    CK(hipStreamSynchronize(h->s));
+    return PLES_MG_OK;
+}
+
+int ples_mg_update(ples_mg_handle* h, double* u, double* v, double* w, int deepl, int deepr, const double* delu,
+                   const double* delv, const double* delw) {
+    if (!h || !u || !v || !w || !delu || !delv || !delw || deepl < 1 || deepr < 1) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    CKS(ensure_proj(h));
+    const Geo& G = h->lv[0].G;
+    double* vel[3] = {u, v, w};
+    const double* del[3] = {delu, delv, delw};
+    for (int q = 0; q < 3; ++q) {
+        CKS(upload_cells(h, h->pj_vel[q], vel[q], deepl));
+        CK(hipMemcpyAsync(h->pj_del[q], del[q], (size_t)G.n * 8, hipMemcpyHostToDevice, h->s));
+    }
+    CKS(launch_update(h));
+    for (int q = 0; q < 3; ++q) CKS(download_cells(h, vel[q], h->pj_vel[q], deepl));
+    CK(hipStreamSynchronize(h->s));
+    return PLES_MG_OK;
+}
+
+int ples_mg_contra(ples_mg_handle* h, const double* u, const double* v, const double* w, int deepl, int deepr,
+                   double* uc, double* vc, double* wc, double* const cs[6], int potenziale) {
+    if (!h || !u || !v || !w || !uc || !vc || !wc || deepl < 1 || deepr < 1) return PLES_MG_ERR_ARG;
+    if (potenziale != 0 && potenziale != 1) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    CKS(ensure_proj(h));
+    const double* vel[3] = {u, v, w};
+    for (int q = 0; q < 3; ++q) CKS(upload_cells(h, h->pj_vel[q], vel[q], deepl));
+    const double* fcin[3] = {uc, vc, wc};
+    CKS(upload_faces(h, h->pj_fc, fcin));
+    CKS(launch_contra(h, potenziale));
+    double* fcout[3] = {uc, vc, wc};
+    CKS(download_faces(h, fcout, h->pj_fc));
+    return download_cs(h, cs);
+}
+
+int ples_mg_vel_up(ples_mg_handle* h, double* u, double* v, double* w, int deepl, int deepr, double* uc,
+                   double* vc, double* wc, const double* const gra[3], const double* const cgra[3], double dt,
+                   int potenziale) {
+    if (!h || !u || !v || !w || !uc || !vc || !wc || !gra || !cgra || deepl < 1 || deepr < 1 || !(dt > 0.0))
+        return PLES_MG_ERR_ARG;
+    if (potenziale != 0 && potenziale != 1) return PLES_MG_ERR_ARG;
+    for (int q = 0; q < 3; ++q)
+        if (!gra[q] || !cgra[q]) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    CKS(ensure_proj(h));
+    const Geo& G = h->lv[0].G;
+    double* vel[3] = {u, v, w};
+    for (int q = 0; q < 3; ++q) {
+        CKS(upload_cells(h, h->pj_vel[q], vel[q], deepl));
+        CK(hipMemsetAsync(h->pj_gra[q], 0, (size_t)G.n * 8, h->s));
+        CKS(pack_box<double>(h, G, 1, G.nx, 1, G.ny, 1, G.nz, gra[q], h->pj_gra[q], h->W[1]));
+        CK(hipStreamSynchronize(h->s));
+    }
+    CKS(upload_faces(h, h->pj_cgra, cgra));
+    const double* fcin[3] = {uc, vc, wc};
+    CKS(upload_faces(h, h->pj_fc, fcin));
+    CKS(launch_vel_up(h, dt, potenziale));
+    for (int q = 0; q < 3; ++q) CKS(download_cells(h, vel[q], h->pj_vel[q], deepl));
+    double* fcout[3] = {uc, vc, wc};
+    return download_faces(h, fcout, h->pj_fc);
+}
+
+int ples_mg_check_divergence(ples_mg_handle* h, const double* uc, const double* vc, const double* wc,
+                             double out[4]) {
+    if (!h || !uc || !vc || !wc || !out) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    CKS(ensure_proj(h));
+    const double* fcin[3] = {uc, vc, wc};
+    CKS(upload_faces(h, h->pj_fc, fcin));
+    return launch_divmax(h, out);
+}
+
+int ples_mg_projection_step(ples_mg_handle* h, double* u, double* v, double* w, int deepl, int deepr,
+                            const double* delu, const double* delv, const double* delw, double* uc, double* vc,
+                            double* wc, double* const cs[6], double* fi, double dt, double eps, int ficycle,
+                            int potenziale, ples_mg_result* res, double div[4]) {
+    if (!h || !u || !v || !w || !delu || !delv || !delw || !uc || !vc || !wc || !fi || deepl < 1 || deepr < 1 ||
+        !(dt > 0.0) || !(eps > 0.0) || ficycle < 0)
+        return PLES_MG_ERR_ARG;
+    if (potenziale != 0 && potenziale != 1) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    CKS(ensure_proj(h));
+    const Level& L0 = h->lv[0];
+    const Geo& G = L0.G;
+    double* vel[3] = {u, v, w};
+    const double* del[3] = {delu, delv, delw};
+    for (int q = 0; q < 3; ++q) {
+        CKS(upload_cells(h, h->pj_vel[q], vel[q], deepl));
+        CK(hipMemcpyAsync(h->pj_del[q], del[q], (size_t)G.n * 8, hipMemcpyHostToDevice, h->s));
+    }
+    const double* fcin[3] = {uc, vc, wc};
+    CKS(upload_faces(h, h->pj_fc, fcin));
+    CKS(upload_fi(h, fi, deepl));
+    // update -> contra -> diver (fsrc/stratiParticle.f03:2678-2845)
+    CKS(launch_update(h));
+    CKS(launch_contra(h, potenziale));
+    {
+        const ColLaunch cl = col_launch(G);
+        hipLaunchKernelGGL(k_diver, cl.grid, cl.block, 0, h->s, G, (const double*)h->pj_fc[0],
+                           (const double*)h->pj_fc[1], (const double*)h->pj_fc[2], dt, h->bvec);
+        CK(hipGetLastError());
+    }
+    // boundary flux of the compatibility projection from the new cs
+    std::vector<double> hcs[6];
+    const double* csp[6] = {};
+    {
+        const size_t csz[6] = {(size_t)h->jy * h->jz, (size_t)h->jy * h->jz, (size_t)h->jx * h->jz,
+                               (size_t)h->jx * h->jz, (size_t)h->jx * h->jy, (size_t)h->jx * h->jy};
+        const int wall[3] = {h->ip, h->jp, h->kp};
+        for (int q = 0; q < 6; ++q) {
+            if (!wall[q / 2]) continue;
+            hcs[q].resize(csz[q]);
+            CK(hipMemcpyAsync(hcs[q].data(), h->cs[q], csz[q] * 8, hipMemcpyDeviceToHost, h->s));
+            csp[q] = hcs[q].data();
+        }
+        CK(hipStreamSynchronize(h->s));
+    }
+    h->dt = dt;
+    CKS(project_rhs(h, boundary_flux(h, csp, dt)));
+    // multi (fsrc/stratiParticle.f03:2881-2887)
+    const int st = solve_core(h, eps, ficycle, res);
+    if (st != PLES_MG_OK && st != PLES_MG_NOT_CONVERGED) return st;
+    // gradie -> vel_up -> check_divergence (fsrc/stratiParticle.f03:2932-2937, 3526)
+    CKS(launch_gradie(h, h->x, dt));
+    CKS(launch_vel_up(h, dt, potenziale));
+    double dm[4];
+    CKS(launch_divmax(h, dm));
+    if (div)
+        for (int q = 0; q < 4; ++q) div[q] = dm[q];
+    for (int q = 0; q < 3; ++q) CKS(download_cells(h, vel[q], h->pj_vel[q], deepl));
+    CKS(download_fi(h, fi, deepl));
+    double* fcout[3] = {uc, vc, wc};
+    CKS(download_faces(h, fcout, h->pj_fc));
+    CKS(download_cs(h, cs));
+    return st;
 }
 
 }  // extern "C"

@@ -105,6 +105,15 @@ def lib():
         L.ples_mg_level_metrics.argtypes = [C.c_void_p, C.c_int, C.POINTER(_dp)]
         L.ples_mg_diver.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, _dp]
         L.ples_mg_gradie.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_double] + [_dp] * 6
+        L.ples_mg_update.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.ples_mg_contra.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp,
+                                     C.POINTER(_dp), C.c_int]
+        L.ples_mg_vel_up.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp,
+                                     C.POINTER(_dp), C.POINTER(_dp), C.c_double, C.c_int]
+        L.ples_mg_check_divergence.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.ples_mg_projection_step.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp,
+                                              _dp, _dp, _dp, C.POINTER(_dp), _dp, C.c_double, C.c_double,
+                                              C.c_int, C.c_int, C.POINTER(_Result), _dp]
         L.ples_mg_case_rhs.argtypes = [C.c_void_p, C.c_uint64, C.c_double]
         L.ples_mg_zero_fi.argtypes = [C.c_void_p]
         L.ples_mg_perturb_fi.argtypes = [C.c_void_p, C.c_uint64, C.c_double]
@@ -274,6 +283,57 @@ class Solver:
         _check(lib().ples_mg_gradie(self._h, _p(np.ascontiguousarray(fi)), 1, 1, dt,
                                     *[_p(a) for a in outs]), "ples_mg_gradie")
         return outs
+
+    # ---- time-loop steps around the solve (arrays in C order: reversed reference shapes)
+    def face_shapes(self):
+        jx, jy, jz = self.jx, self.jy, self.jz
+        return [(jz, jy, jx + 1), (jz, jy + 1, jx), (jz + 1, jy, jx)]
+
+    def _cs_out(self):
+        return [np.zeros(s) for s in self.cs_shapes()]
+
+    def update(self, vel, dvel):
+        """In place on vel = [u, v, w] (fi_shape); dvel shaped fi_shape too."""
+        for a in vel:
+            assert a.shape == self.fi_shape and a.flags["C_CONTIGUOUS"] and a.dtype == np.float64
+        dv = [np.ascontiguousarray(a, np.float64) for a in dvel]
+        _check(lib().ples_mg_update(self._h, *[_p(a) for a in vel], 1, 1, *[_p(a) for a in dv]),
+               "ples_mg_update")
+
+    def contra(self, vel, fc, potenziale=0):
+        """In place on fc = [uc, vc, wc]; returns cs1..cs6 (zeros for periodic directions)."""
+        cs = self._cs_out()
+        _check(lib().ples_mg_contra(self._h, *[_p(np.ascontiguousarray(a)) for a in vel], 1, 1,
+                                    *[_p(a) for a in fc], (_dp * 6)(*[_p(a) for a in cs]), potenziale),
+               "ples_mg_contra")
+        return cs
+
+    def vel_up(self, vel, fc, gra, cgra, dt=1.0, potenziale=0):
+        gra = [np.ascontiguousarray(a, np.float64) for a in gra]
+        cgra = [np.ascontiguousarray(a, np.float64) for a in cgra]
+        _check(lib().ples_mg_vel_up(self._h, *[_p(a) for a in vel], 1, 1, *[_p(a) for a in fc],
+                                    (_dp * 3)(*[_p(a) for a in gra]), (_dp * 3)(*[_p(a) for a in cgra]),
+                                    dt, potenziale), "ples_mg_vel_up")
+
+    def check_divergence(self, fc):
+        out = np.zeros(4)
+        _check(lib().ples_mg_check_divergence(self._h, *[_p(np.ascontiguousarray(a)) for a in fc], _p(out)),
+               "ples_mg_check_divergence")
+        return out
+
+    def projection_step(self, vel, dvel, fc, fi, dt=1.0, eps=1e-10, ficycle=200, potenziale=0):
+        """update -> contra -> diver -> multi -> gradie -> vel_up -> check_divergence,
+        in place on vel, fc, fi.  Returns (Result, cs1..cs6, div maxima)."""
+        cs = self._cs_out()
+        dv = [np.ascontiguousarray(a, np.float64) for a in dvel]
+        div = np.zeros(4)
+        r = _Result()
+        st = lib().ples_mg_projection_step(self._h, *[_p(a) for a in vel], 1, 1, *[_p(a) for a in dv],
+                                           *[_p(a) for a in fc], (_dp * 6)(*[_p(a) for a in cs]), _p(fi),
+                                           dt, eps, ficycle, potenziale, C.byref(r), _p(div))
+        if st not in (OK, NOT_CONVERGED):
+            raise PlesMGError(st, "ples_mg_projection_step")
+        return Result(r.status, r.cycles, r.rel_res, r.abs_res, r.rhs_mean_removed, r.seconds), cs, div
 
     # ---- device-resident
     def case_rhs(self, seed: int, dt: float = 1.0):
