@@ -76,6 +76,9 @@ typedef struct ples_mg_opts {
     int halo_passes; /* red/black passes of the exact wall-ghost solve (0 = auto:
                         6 with eta walls only, 24 with xi/zeta walls) */
     int device;      /* HIP device ordinal (default 0) */
+    int graphs;      /* 1 = replay each refinement step from a captured HIP graph
+                        (default), 0 = launch its kernels one by one; steps are
+                        always launched one by one while kernel timing is on */
 } ples_mg_opts;
 
 typedef struct ples_mg_result {

@@ -685,8 +685,11 @@ __global__ void __launch_bounds__(512) k_resid(Geo G, Met<TM> M, const T* __rest
 // ------------------------------------------------------------------------
 // Outer-loop scalar logic on the device (no host round trip per iteration).
 // ------------------------------------------------------------------------
+// S_DONE: set by k_fin_update once the stop test (S_MODE 0: r.r < S_THR,
+// 1: max|r| < S_THR) holds; later refinement steps then leave x and r alone,
+// so the host may enqueue steps ahead of reading the residual.
 enum { S_RR = 0, S_RMAX, S_BB, S_WW, S_WWP, S_RW, S_RWP, S_WPWP, S_ALPHA, S_BETA, S_HAVEP, S_RR0,
-       S_SUM, S_COUNT = 16 };
+       S_SUM, S_SUM2, S_DONE, S_THR, S_MODE, S_COUNT = 20 };
 
 // sum (or max) of nb partials per quantity, one block of 256 threads,
 // fixed order: strided per-thread sums then a tree
@@ -730,6 +733,11 @@ __global__ void __launch_bounds__(256) k_fin_apply(const double* __restrict__ pa
     double o[4];
     fin_reduce<4>(part, nb, mx, o);
     if (threadIdx.x != 0) return;
+    if (S[S_DONE] != 0.0) {
+        S[S_ALPHA] = 0.0;
+        S[S_BETA] = 0.0;
+        return;
+    }
     const double ww = o[0], wwp = o[1], rw = o[2], rwp = o[3];
     double beta = 0.0;
     if (krylov && S[S_HAVEP] != 0.0 && S[S_WPWP] > 0.0) beta = wwp / S[S_WPWP];
@@ -753,6 +761,7 @@ __global__ void __launch_bounds__(512) k_gcr_update(Geo G, double* __restrict__ 
                                                     const double* __restrict__ zp,
                                                     const double* __restrict__ wp, const double* S,
                                                     double* partials, int nblocks, int P) {
+    if (S[S_DONE] != 0.0) return;  // converged: uniform exit, partials unused
     Seg q;
     const bool act = seg_of(G, P, q);
     double acc[2] = {0, 0};
@@ -781,12 +790,15 @@ __global__ void __launch_bounds__(512) k_gcr_update(Geo G, double* __restrict__ 
 }
 
 __global__ void __launch_bounds__(256) k_fin_update(const double* __restrict__ part, int nb, double* S) {
+    if (S[S_DONE] != 0.0) return;  // uniform: r.r and max|r| stay those of the converged step
     const bool mx[2] = {false, true};
     double o[2];
     fin_reduce<2>(part, nb, mx, o);
     if (threadIdx.x == 0) {
         S[S_RR] = o[0];
         S[S_RMAX] = o[1];
+        const double m = (S[S_MODE] != 0.0) ? o[1] : o[0];
+        if (m < S[S_THR]) S[S_DONE] = 1.0;
     }
 }
 

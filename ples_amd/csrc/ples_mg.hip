@@ -67,6 +67,9 @@ struct ples_mg_handle {
     unsigned timing = 0;  // bit f: time kernel family f
     std::vector<Rec> recs;
     hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipEvent_t sev[4] = {};          // read-back ring of the refinement loop
+    hipGraphExec_t gexec[2] = {};    // captured refinement step per (Z, W) pair
+    std::vector<int> gin[2], gout[2];
     double last_removed = 0.0;
     // projection-step state (allocated on first use): level-1 face vectors
     // and Jacobian, cell velocities, face fluxes, increments, gradients
@@ -500,83 +503,194 @@ int outer_residual(H* h, double* x, double* r) {
     return PLES_MG_OK;
 }
 
-int solve_core(H* h, double eps, int maxcyc, ples_mg_result* res) {
+// one refinement step: z = V(r) (FP32 or FP64 V-cycle), w = A z, GCR(2)
+// orthogonalisation against the previous (z, w), x += alpha z, r -= alpha w,
+// r.r and max|r| into S.  Scalars stay on the device; a converged solve
+// (S_DONE) leaves x and r unchanged.  q selects the (Z, W) ping-pong pair.
+int step_body(H* h, int q) {
     Level& L0 = h->lv[0];
     const ColLaunch cl = seg_launch(L0.G, L0.P);
     const double N0 = (double)L0.ncell();
+    CKS(vcycle(h, h->rvec));
+    double* Zq = h->Z[q];
+    double* Wq = h->W[q];
+    double* Zp = h->Z[1 - q];
+    double* Wp = h->W[1 - q];
+    {
+        const long n = L0.G.n;
+        const int nb = (int)std::min<long>((n + 255) / 256, 8192);
+        if (L0.f32)
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(256), 0, h->s, n,
+                               (const float*)L0.p[L0.cur], Zq);
+        else
+            hipLaunchKernelGGL((k_convert<double, double>), dim3(nb), dim3(256), 0, h->s, n,
+                               (const double*)L0.p[L0.cur], Zq);
+    }
+    CKS(halo64(h, Zq, false, true));
+    const bool kry = h->o.krylov != 0;
+    {
+        KT kt(h, PLES_MG_K_OUTER_APPLY, N0 * (72 + 24 + (kry ? 8 : 0)));
+        hipLaunchKernelGGL((k_resid<2, double, double, double, double>), cl.grid, cl.block, 0, h->s, L0.G,
+                           met<double>(L0), (const double*)Zq, (const double*)nullptr, Wq, 1.0,
+                           (const double*)h->rvec, (const double*)(kry ? Wp : nullptr), h->part, cl.nblocks,
+                           L0.P);
+    }
+    hipLaunchKernelGGL(k_fin_apply, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S,
+                       h->o.krylov);
+    {
+        KT kt(h, PLES_MG_K_OUTER_UPDATE, N0 * (48 + (kry ? 32 + 16 : 0)));
+        hipLaunchKernelGGL(k_gcr_update, cl.grid, cl.block, 0, h->s, L0.G, h->x, h->rvec, Zq, Wq,
+                           (const double*)Zp, (const double*)Wp, (const double*)h->S, h->part, cl.nblocks, L0.P);
+    }
+    hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+// host-side state a step depends on (ping-pong selectors of every level)
+std::vector<int> step_state(const H* h) {
+    std::vector<int> v;
+    for (const Level& L : h->lv) v.push_back(L.cur);
+    return v;
+}
+void set_step_state(H* h, const std::vector<int>& v) {
+    for (size_t n = 0; n < h->lv.size(); ++n) h->lv[n].cur = v[n];
+}
+
+void drop_graphs(H* h) {
+    for (int q = 0; q < 2; ++q) {
+        if (h->gexec[q]) (void)hipGraphExecDestroy(h->gexec[q]);
+        h->gexec[q] = nullptr;
+        h->gin[q].clear();
+        h->gout[q].clear();
+    }
+}
+
+// capture step_body(q) into a graph on the handle's stream (not executed)
+int capture_step(H* h, int q) {
+    h->gin[q] = step_state(h);
+    CK(hipStreamBeginCapture(h->s, hipStreamCaptureModeThreadLocal));
+    const int st = step_body(h, q);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->s, &g);
+    if (st != PLES_MG_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    CK(e);
+    const hipError_t ei = hipGraphInstantiate(&h->gexec[q], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    CK(ei);
+    h->gout[q] = step_state(h);
+    return PLES_MG_OK;
+}
+
+// enqueue one refinement step: replay of the captured graph of pair q when
+// graphs are on and no kernel timing is requested, else eager launches
+int enqueue_step(H* h, int q) {
+    if (!h->o.graphs || h->timing != 0) return step_body(h, q);
+    if (h->gexec[q] && step_state(h) != h->gin[q]) drop_graphs(h);  // state moved: recapture
+    if (!h->gexec[q]) {
+        const std::vector<int> before = step_state(h);
+        CKS(capture_step(h, q));
+        set_step_state(h, before);
+    }
+    CK(hipGraphLaunch(h->gexec[q], h->s));
+    set_step_state(h, h->gout[q]);
+    return PLES_MG_OK;
+}
+
+// stop test parameters on the device; clears S_DONE and the GCR history
+int arm_stop(H* h, double thr) {
+    h->Sh[S_DONE] = 0.0;
+    h->Sh[S_THR] = thr;
+    h->Sh[S_MODE] = (h->o.stop_mode == PLES_MG_STOP_ABS_MAX) ? 1.0 : 0.0;
+    CK(hipMemcpyAsync(h->S + S_DONE, h->Sh + S_DONE, 3 * sizeof(double), hipMemcpyHostToDevice, h->s));
+    CKS(set_S(h, S_HAVEP, 0.0));
+    return PLES_MG_OK;
+}
+
+// The refinement loop.  Steps are enqueued ahead of the residual read-back
+// (a ring of pinned snapshots of S, one event each) while the observed
+// contraction predicts that the step in flight will not yet converge, so the
+// device never idles on the host between steps; near convergence the host
+// waits for each step.  The stop test itself runs on the device (S_DONE),
+// which makes any step enqueued past convergence a no-op for x and r.
+int solve_core(H* h, double eps, int maxcyc, ples_mg_result* res) {
+    Level& L0 = h->lv[0];
     CK(hipEventRecord(h->t0, h->s));
     CKS(outer_residual(h, h->x, h->rvec));
     CKS(read_S(h));
     const double bb = h->Sh[S_BB];
     const double bnorm = std::sqrt(bb) > 0 ? std::sqrt(bb) : 1.0;
-    auto measure = [&](double rr, double rmax) {
-        return (h->o.stop_mode == PLES_MG_STOP_ABS_MAX) ? rmax : std::sqrt(rr) / bnorm;
-    };
+    const bool absmode = h->o.stop_mode == PLES_MG_STOP_ABS_MAX;
+    auto measure = [&](double rr, double rmax) { return absmode ? rmax : std::sqrt(rr) / bnorm; };
+    const double thr = absmode ? eps : (eps * bnorm) * (eps * bnorm);
     double rr = h->Sh[S_RR], rmax = h->Sh[S_RMAX];
     if (!std::isfinite(rr)) return PLES_MG_ERR_NONFINITE;
+    (void)L0;
     int cycles = 0;
     int status = PLES_MG_NOT_CONVERGED;
-    CKS(set_S(h, S_HAVEP, 0.0));
-    int q = 0;
     bool conv = measure(rr, rmax) < eps;
     if (conv) status = PLES_MG_OK;
-    while (!conv && cycles < maxcyc) {
-        ++cycles;
-        CKS(vcycle(h, h->rvec));
-        double* Zq = h->Z[q];
-        double* Wq = h->W[q];
-        double* Zp = h->Z[1 - q];
-        double* Wp = h->W[1 - q];
-        {
-            const long n = L0.G.n;
-            const int nb = (int)std::min<long>((n + 255) / 256, 8192);
-            if (L0.f32)
-                hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(256), 0, h->s, n,
-                                   (const float*)L0.p[L0.cur], Zq);
-            else
-                hipLaunchKernelGGL((k_convert<double, double>), dim3(nb), dim3(256), 0, h->s, n,
-                                   (const double*)L0.p[L0.cur], Zq);
+    CKS(arm_stop(h, thr));
+    int q = 0;
+    int enq = 0, seen = 0;        // steps enqueued / read back since arming
+    double last = measure(rr, rmax), rho = 0.5;
+    const int depth = 2;          // at most this many steps in flight
+    while (!conv) {
+        while (cycles + enq < maxcyc && enq - seen < depth) {
+            if (enq > seen && last * std::pow(rho, (double)(enq - seen)) < 4.0 * eps) break;
+            CKS(enqueue_step(h, q));
+            q = 1 - q;
+            double* slot = h->Sh + 2 * S_COUNT + (enq % 4) * S_COUNT;
+            CK(hipMemcpyAsync(slot, h->S, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, h->s));
+            CK(hipEventRecord(h->sev[enq % 4], h->s));
+            ++enq;
         }
-        CKS(halo64(h, Zq, false, true));
-        const bool havep = h->Sh[S_HAVEP] != 0.0 && h->o.krylov;
-        {
-            KT kt(h, PLES_MG_K_OUTER_APPLY, N0 * (72 + 24 + (havep ? 8 : 0)));
-            hipLaunchKernelGGL((k_resid<2, double, double, double, double>), cl.grid, cl.block, 0, h->s, L0.G,
-                               met<double>(L0), (const double*)Zq, (const double*)nullptr, Wq, 1.0,
-                               (const double*)h->rvec, (const double*)(havep ? Wp : nullptr), h->part,
-                               cl.nblocks, L0.P);
+        if (seen == enq) break;  // cycle cap reached
+        CK(hipEventSynchronize(h->sev[seen % 4]));
+        const double* slot = h->Sh + 2 * S_COUNT + (seen % 4) * S_COUNT;
+        ++seen;
+        const double r2 = slot[S_RR], rm = slot[S_RMAX];
+        if (!std::isfinite(r2)) {
+            status = PLES_MG_ERR_NONFINITE;
+            cycles += seen;
+            break;
         }
-        hipLaunchKernelGGL(k_fin_apply, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S,
-                           h->o.krylov);
-        {
-            KT kt(h, PLES_MG_K_OUTER_UPDATE, N0 * (48 + (havep ? 32 + 16 : 0)));
-            hipLaunchKernelGGL(k_gcr_update, cl.grid, cl.block, 0, h->s, L0.G, h->x, h->rvec, Zq, Wq,
-                               (const double*)Zp, (const double*)Wp, (const double*)h->S, h->part, cl.nblocks,
-                               L0.P);
+        if (slot[S_DONE] == 0.0) {
+            const double m = measure(r2, rm);
+            if (last > 0.0 && m > 0.0) rho = std::min(0.95, std::max(1e-3, m / last));
+            last = m;
+            continue;
         }
-        hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S);
-        CK(hipGetLastError());
-        q = 1 - q;
+        // converged at step `seen`; later steps in flight change nothing
+        cycles += seen;
+        CK(hipStreamSynchronize(h->s));
+        rr = r2;
+        rmax = rm;
+        // confirm with the true residual; restart the recurrence if it drifted
+        CKS(outer_residual(h, h->x, h->rvec));
         CKS(read_S(h));
         rr = h->Sh[S_RR];
         rmax = h->Sh[S_RMAX];
-        if (!std::isfinite(rr)) {
-            status = PLES_MG_ERR_NONFINITE;
-            break;
-        }
         if (measure(rr, rmax) < eps) {
-            // confirm with the true residual; restart the recurrence if it drifted
-            CKS(outer_residual(h, h->x, h->rvec));
-            CKS(read_S(h));
-            rr = h->Sh[S_RR];
-            rmax = h->Sh[S_RMAX];
-            if (measure(rr, rmax) < eps) {
-                conv = true;
-                status = PLES_MG_OK;
-            } else {
-                CKS(set_S(h, S_HAVEP, 0.0));
-            }
+            conv = true;
+            status = PLES_MG_OK;
+        } else {
+            CKS(arm_stop(h, thr));
+            enq = seen = 0;
+            last = measure(rr, rmax);
         }
+    }
+    if (!conv && status != PLES_MG_ERR_NONFINITE) {
+        CK(hipStreamSynchronize(h->s));
+        if (seen > 0) {
+            const double* slot = h->Sh + 2 * S_COUNT + ((seen - 1) % 4) * S_COUNT;
+            rr = slot[S_RR];
+            rmax = slot[S_RMAX];
+        }
+        cycles += seen;
     }
     // leave the iterate with exact ghosts (the reference returns fi with ghosts)
     CKS(halo64(h, h->x, true));
@@ -729,6 +843,7 @@ void ples_mg_default_opts(ples_mg_opts* o) {
     o->project_rhs = 1;
     o->halo_passes = 0;  // auto
     o->device = 0;
+    o->graphs = 1;
 }
 
 const char* ples_mg_status_string(int s) {
@@ -760,6 +875,9 @@ int ples_mg_destroy(ples_mg_handle* h) {
         (void)hipEventDestroy(r.b);
     }
     for (auto& a : h->allocs) (void)hipFree(a.first);
+    drop_graphs(h);
+    for (int q = 0; q < 4; ++q)
+        if (h->sev[q]) (void)hipEventDestroy(h->sev[q]);
     if (h->Sh) (void)hipHostFree(h->Sh);
     if (h->t0) (void)hipEventDestroy(h->t0);
     if (h->t1) (void)hipEventDestroy(h->t1);
@@ -782,8 +900,11 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
     CK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
     CK(hipEventCreate(&h->t0));
     CK(hipEventCreate(&h->t1));
-    CK(hipHostMalloc((void**)&h->Sh, (S_COUNT + 4) * sizeof(double), hipHostMallocDefault));
-    std::memset(h->Sh, 0, (S_COUNT + 4) * sizeof(double));
+    // pinned S mirror: [0, S_COUNT) scalars, then 4 divergence maxima, then
+    // the 4-slot read-back ring of the refinement loop
+    CK(hipHostMalloc((void**)&h->Sh, 6 * S_COUNT * sizeof(double), hipHostMallocDefault));
+    std::memset(h->Sh, 0, 6 * S_COUNT * sizeof(double));
+    for (int q = 0; q < 4; ++q) CK(hipEventCreateWithFlags(&h->sev[q], hipEventDisableTiming));
     h->jx = g->jx; h->jy = g->jy; h->jz = g->jz;
     h->ip = g->ip; h->jp = g->jp; h->kp = g->kp;
     h->L = g->nlevel;

@@ -54,7 +54,7 @@ class _Opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("nu_pre", C.c_int), ("nu_post", C.c_int),
                 ("nu_coarse", C.c_int), ("omega", C.c_double), ("krylov", C.c_int),
                 ("stop_mode", C.c_int), ("project_rhs", C.c_int), ("halo_passes", C.c_int),
-                ("device", C.c_int)]
+                ("device", C.c_int), ("graphs", C.c_int)]
 
 
 class _Result(C.Structure):
