@@ -79,6 +79,8 @@ typedef struct ples_mg_opts {
     int graphs;      /* 1 = replay each refinement step from a captured HIP graph
                         (default), 0 = launch its kernels one by one; steps are
                         always launched one by one while kernel timing is on */
+    int inner_cycles; /* V-cycles per refinement step, the first from zero, each
+                        further one continuing from the last (0 = auto = 1) */
 } ples_mg_opts;
 
 typedef struct ples_mg_result {
@@ -209,6 +211,31 @@ int ples_mg_get_fi(ples_mg_handle* h, double* fi);
 int ples_mg_set_fi(ples_mg_handle* h, const double* fi);
 int ples_mg_get_rhs(ples_mg_handle* h, double* rhs);
 int ples_mg_set_rhs(ples_mg_handle* h, const double* rhs);
+
+/* ---- batched solves ----
+ * n independent problems of any sizes (the grids may differ; one opts for
+ * all), each with its own handle and HIP stream.  ples_mg_batch_solve runs
+ * the refinement loops of all of them together: one HIP graph holds
+ * steps_per_launch refinement steps (V-cycle + FP64 update) of every problem,
+ * forked onto the problems' streams so their kernels overlap on the device,
+ * and one launch of it advances the whole batch.  Each problem stops on its
+ * own test (device-side; later steps leave it unchanged), the host reads all
+ * residuals back with one copy per launch.  There is no reference
+ * counterpart: PLES solves one pressure field per time step. */
+typedef struct ples_mg_batch ples_mg_batch;
+int ples_mg_batch_create(int n, const ples_mg_grid* grids, const ples_mg_opts* opts,
+                         ples_mg_batch** out);
+int ples_mg_batch_destroy(ples_mg_batch* b);
+int ples_mg_batch_size(const ples_mg_batch* b);
+/* The handle of problem i (owned by the batch): rhs and iterate set-up and
+ * read-back use the device-resident entry points above. */
+ples_mg_handle* ples_mg_batch_handle(ples_mg_batch* b, int i);
+size_t ples_mg_batch_device_bytes(const ples_mg_batch* b);
+/* Solve every problem from its resident iterate to eps (opts.stop_mode);
+ * res[i] per problem (seconds = the whole batch).  steps_per_launch 0 = 2.
+ * Returns PLES_MG_OK when all converged, else the first failing status. */
+int ples_mg_batch_solve(ples_mg_batch* b, double eps, int maxcycles, int steps_per_launch,
+                        ples_mg_result* res);
 
 /* HIP stream the handle launches on (hipStream_t as void*). */
 void* ples_mg_stream(ples_mg_handle* h);

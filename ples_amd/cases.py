@@ -63,3 +63,21 @@ CONFIGS = {
     "c3": Config("c3", 512, 256, 512, 2.0, 0.1 * H, 7, 1e-10, 1003,
                  "warped 512x256x512, tanh 2.0, L=7, warm start p_prev + 1e-3 std N(0,1), 1e-10"),
 }
+
+
+# Batched workload: 64 independent problems, sizes drawn uniformly from five
+# families, terrain amplitude U[0, 0.15] H, tanh 1.5 stretching, each with its
+# own seeded divergence rhs.  Levels leave at least two wall-normal cells on
+# the coarsest level (32^3 therefore uses 5 levels).
+C4_SIZES = [(32, 32, 32, 5), (64, 64, 64, 6), (96, 48, 96, 5), (128, 64, 128, 6), (192, 96, 192, 5)]
+
+
+def c4_problems(n=64, seed0=2000, sizes=C4_SIZES):
+    probs = []
+    for b in range(n):
+        rng = np.random.default_rng(seed0 + b)
+        nx, ny, nz, nlevel = sizes[int(rng.integers(len(sizes)))]
+        amp = float(rng.uniform(0.0, 0.15)) * H
+        x, y, z = make_nodes(nx, ny, nz, 1.5, amp)
+        probs.append(dict(x=x, y=y, z=z, nlevel=nlevel, seed=seed0 + b, shape=(nx, ny, nz), amp=amp))
+    return probs

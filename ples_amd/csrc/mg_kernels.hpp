@@ -689,7 +689,7 @@ __global__ void __launch_bounds__(512) k_resid(Geo G, Met<TM> M, const T* __rest
 // 1: max|r| < S_THR) holds; later refinement steps then leave x and r alone,
 // so the host may enqueue steps ahead of reading the residual.
 enum { S_RR = 0, S_RMAX, S_BB, S_WW, S_WWP, S_RW, S_RWP, S_WPWP, S_ALPHA, S_BETA, S_HAVEP, S_RR0,
-       S_SUM, S_SUM2, S_DONE, S_THR, S_MODE, S_COUNT = 20 };
+       S_SUM, S_SUM2, S_DONE, S_THR, S_MODE, S_CYC, S_COUNT = 20 };
 
 // sum (or max) of nb partials per quantity, one block of 256 threads,
 // fixed order: strided per-thread sums then a tree
@@ -798,8 +798,19 @@ __global__ void __launch_bounds__(256) k_fin_update(const double* __restrict__ p
         S[S_RR] = o[0];
         S[S_RMAX] = o[1];
         const double m = (S[S_MODE] != 0.0) ? o[1] : o[0];
+        S[S_CYC] += 1.0;  // steps that changed x since the stop test was armed
         if (m < S[S_THR]) S[S_DONE] = 1.0;
     }
+}
+
+// copy the scalars of n solves into one buffer (one read-back per batch)
+struct SPtrs {
+    const double* const* s;
+};
+__global__ void k_gather_S(const double* const* __restrict__ S, int n, double* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * S_COUNT) return;
+    out[t] = S[t / S_COUNT][t % S_COUNT];
 }
 
 // ------------------------------------------------------------------------

@@ -71,6 +71,7 @@ struct ples_mg_handle {
     hipGraphExec_t gexec[2] = {};    // captured refinement step per (Z, W) pair
     std::vector<int> gin[2], gout[2];
     double last_removed = 0.0;
+    int inner = 1;                   // V-cycles per refinement step (resolved from opts)
     // projection-step state (allocated on first use): level-1 face vectors
     // and Jacobian, cell velocities, face fluxes, increments, gradients
     bool pj_ready = false;
@@ -451,8 +452,10 @@ int coarse_solve(H* h, Level& L, const double* rhs, int nu) {
     return PLES_MG_OK;
 }
 
-// z = V(r0): one V-cycle for A z = r0 from a zero initial guess
-int vcycle(H* h, const double* r0) {
+// One V-cycle for A z = r0 on the fine iterate of level 0: from a zero
+// initial guess (fine_zero) or continuing from the current one; the coarse
+// levels always solve for corrections from zero.
+int vcycle(H* h, const double* r0, bool fine_zero = true) {
     const int L = h->L;
     for (int n = 0; n < L; ++n) {
         const int nu = (n == L - 1) ? h->o.nu_coarse : h->o.nu_pre;
@@ -461,7 +464,7 @@ int vcycle(H* h, const double* r0) {
             CKS(coarse_solve(h, lvn, n == 0 ? r0 : (const double*)lvn.b, std::max(1, nu)));
             continue;
         }
-        CKS(lv_smooth(h, n, r0, std::max(1, nu), true));
+        CKS(lv_smooth(h, n, r0, std::max(1, nu), n > 0 || fine_zero));
         if (n < L - 1) {
             CKS(lv_resid(h, n, r0));
             CKS(lv_restrict(h, n));
@@ -511,7 +514,8 @@ int step_body(H* h, int q) {
     Level& L0 = h->lv[0];
     const ColLaunch cl = seg_launch(L0.G, L0.P);
     const double N0 = (double)L0.ncell();
-    CKS(vcycle(h, h->rvec));
+    // z = M r: inner_cycles V-cycles on A z = r (the first from zero)
+    for (int c = 0; c < h->inner; ++c) CKS(vcycle(h, h->rvec, c == 0));
     double* Zq = h->Z[q];
     double* Wq = h->W[q];
     double* Zp = h->Z[1 - q];
@@ -605,7 +609,8 @@ int arm_stop(H* h, double thr) {
     h->Sh[S_DONE] = 0.0;
     h->Sh[S_THR] = thr;
     h->Sh[S_MODE] = (h->o.stop_mode == PLES_MG_STOP_ABS_MAX) ? 1.0 : 0.0;
-    CK(hipMemcpyAsync(h->S + S_DONE, h->Sh + S_DONE, 3 * sizeof(double), hipMemcpyHostToDevice, h->s));
+    h->Sh[S_CYC] = 0.0;
+    CK(hipMemcpyAsync(h->S + S_DONE, h->Sh + S_DONE, 4 * sizeof(double), hipMemcpyHostToDevice, h->s));
     CKS(set_S(h, S_HAVEP, 0.0));
     return PLES_MG_OK;
 }
@@ -844,6 +849,7 @@ void ples_mg_default_opts(ples_mg_opts* o) {
     o->halo_passes = 0;  // auto
     o->device = 0;
     o->graphs = 1;
+    o->inner_cycles = 0;  // auto
 }
 
 const char* ples_mg_status_string(int s) {
@@ -894,7 +900,9 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
     if (o.nu_pre < 0 || o.nu_post < 0 || o.nu_coarse < 0 || !(o.omega > 0.0 && o.omega < 2.0))
         return PLES_MG_ERR_ARG;
     if (o.halo_passes <= 0) o.halo_passes = (g->ip == 1 || g->kp == 1) ? 24 : 6;
+    if (o.inner_cycles < 0 || o.inner_cycles > 8) return PLES_MG_ERR_ARG;
     h->o = o;
+    h->inner = o.inner_cycles > 0 ? o.inner_cycles : 1;
     h->dev = o.device;
     CK(hipSetDevice(h->dev));
     CK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
@@ -1646,6 +1654,380 @@ int ples_mg_projection_step(ples_mg_handle* h, double* u, double* v, double* w, 
     CKS(download_faces(h, fcout, h->pj_fc));
     CKS(download_cs(h, cs));
     return st;
+}
+
+
+// ---------------------------------------------------------------- batched solves
+}  // extern "C"
+
+struct ples_mg_batch {
+    int dev = 0;
+    std::vector<H*> hs;
+    hipStream_t ms = nullptr;  // batch stream: graph launches, gathers, read-backs
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    int K = 2;                              // refinement steps per launch
+    std::vector<int> q;                     // (Z, W) pair of each problem's next step
+    hipGraphExec_t gexec[2] = {};           // K steps of every problem, by start state
+    std::vector<std::vector<int>> gin[2], gout[2];
+    const double** dS = nullptr;            // device array of the problems' S
+    double* Sall = nullptr;                 // gathered S (device)
+    double* ring = nullptr;                 // pinned: 2 slots x n x S_COUNT
+    hipEvent_t rev[2] = {};
+    size_t bytes = 0;
+};
+
+namespace {
+
+using B = ples_mg_batch;
+
+void batch_drop_graph(B* b) {
+    for (int g = 0; g < 2; ++g) {
+        if (b->gexec[g]) (void)hipGraphExecDestroy(b->gexec[g]);
+        b->gexec[g] = nullptr;
+        b->gin[g].clear();
+        b->gout[g].clear();
+    }
+}
+
+// K steps of every problem on its own stream, forked from and joined back
+// into the batch stream, then the gather of all S into Sall
+int batch_chunk_body(B* b) {
+    const int n = (int)b->hs.size();
+    CK(hipEventRecord(b->fork, b->ms));
+    for (int i = 0; i < n; ++i) {
+        H* h = b->hs[i];
+        CK(hipStreamWaitEvent(h->s, b->fork, 0));
+        for (int k = 0; k < b->K; ++k) {
+            CKS(step_body(h, b->q[i]));
+            b->q[i] ^= 1;
+        }
+        CK(hipEventRecord(b->join[i], h->s));
+        CK(hipStreamWaitEvent(b->ms, b->join[i], 0));
+    }
+    const int tot = n * S_COUNT;
+    hipLaunchKernelGGL(k_gather_S, dim3((tot + 255) / 256), dim3(256), 0, b->ms, (const double* const*)b->dS, n,
+                       b->Sall);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+std::vector<std::vector<int>> batch_state(const B* b) {
+    std::vector<std::vector<int>> v;
+    for (size_t i = 0; i < b->hs.size(); ++i) {
+        v.push_back(step_state(b->hs[i]));
+        v.back().push_back(b->q[i]);
+    }
+    return v;
+}
+
+void set_batch_state(B* b, const std::vector<std::vector<int>>& v) {
+    for (size_t i = 0; i < b->hs.size(); ++i) {
+        set_step_state(b->hs[i], v[i]);
+        b->q[i] = v[i].back();
+    }
+}
+
+// one launch of K steps for the whole batch: replay of the captured graph
+// whose start state (every level's ping-pong, every (Z, W) selector) is the
+// current one -- two graphs cover odd K and odd smoothing counts -- else
+// capture it; eager launches when graphs are off
+int batch_enqueue_chunk(B* b, bool use_graph) {
+    if (!use_graph) return batch_chunk_body(b);
+    const auto now = batch_state(b);
+    int g = -1;
+    for (int t = 0; t < 2; ++t)
+        if (b->gexec[t] && b->gin[t] == now) g = t;
+    if (g < 0) {
+        g = !b->gexec[0] ? 0 : (!b->gexec[1] ? 1 : -1);
+        if (g < 0) {
+            batch_drop_graph(b);
+            g = 0;
+        }
+        CK(hipStreamBeginCapture(b->ms, hipStreamCaptureModeGlobal));
+        const int st = batch_chunk_body(b);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->ms, &gr);
+        if (st != PLES_MG_OK) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return st;
+        }
+        CK(e);
+        const hipError_t ei = hipGraphInstantiate(&b->gexec[g], gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        CK(ei);
+        b->gin[g] = now;
+        b->gout[g] = batch_state(b);
+        set_batch_state(b, now);
+    }
+    CK(hipGraphLaunch(b->gexec[g], b->ms));
+    set_batch_state(b, b->gout[g]);
+    return PLES_MG_OK;
+}
+
+int batch_set_done(B* b, int i, double v) {
+    H* h = b->hs[i];
+    h->Sh[S_DONE] = v;
+    CK(hipMemcpyAsync(h->S + S_DONE, h->Sh + S_DONE, sizeof(double), hipMemcpyHostToDevice, b->ms));
+    return PLES_MG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ples_mg_batch_create(int n, const ples_mg_grid* grids, const ples_mg_opts* opts, ples_mg_batch** out) {
+    if (!out || n < 1 || !grids) return PLES_MG_ERR_ARG;
+    *out = nullptr;
+    B* b = new B();
+    int st = PLES_MG_OK;
+    for (int i = 0; i < n && st == PLES_MG_OK; ++i) {
+        ples_mg_handle* h = nullptr;
+        st = ples_mg_create(&grids[i], opts, &h);
+        if (st == PLES_MG_OK) {
+            b->hs.push_back(h);
+            b->bytes += h->bytes;
+        }
+    }
+    auto fail = [&](int code) {
+        ples_mg_batch_destroy(b);
+        return code;
+    };
+    if (st != PLES_MG_OK) return fail(st);
+    b->dev = b->hs[0]->dev;
+    if (hipSetDevice(b->dev) != hipSuccess) return fail(PLES_MG_ERR_HIP);
+    if (hipStreamCreateWithFlags(&b->ms, hipStreamNonBlocking) != hipSuccess) return fail(PLES_MG_ERR_HIP);
+    if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) return fail(PLES_MG_ERR_HIP);
+    b->join.resize(n);
+    for (int i = 0; i < n; ++i)
+        if (hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) != hipSuccess) return fail(PLES_MG_ERR_HIP);
+    for (int q = 0; q < 2; ++q)
+        if (hipEventCreateWithFlags(&b->rev[q], hipEventDisableTiming) != hipSuccess) return fail(PLES_MG_ERR_HIP);
+    std::vector<const double*> hp(n);
+    for (int i = 0; i < n; ++i) hp[i] = b->hs[i]->S;
+    if (hipMalloc((void**)&b->dS, n * sizeof(double*)) != hipSuccess) return fail(PLES_MG_ERR_OOM);
+    if (hipMalloc((void**)&b->Sall, (size_t)n * S_COUNT * sizeof(double)) != hipSuccess) return fail(PLES_MG_ERR_OOM);
+    if (hipMemcpy(b->dS, hp.data(), n * sizeof(double*), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(PLES_MG_ERR_HIP);
+    if (hipHostMalloc((void**)&b->ring, 2 * (size_t)n * S_COUNT * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return fail(PLES_MG_ERR_OOM);
+    b->bytes += n * sizeof(double*) + (size_t)n * S_COUNT * sizeof(double);
+    *out = b;
+    return PLES_MG_OK;
+}
+
+int ples_mg_batch_destroy(ples_mg_batch* b) {
+    if (!b) return PLES_MG_OK;
+    if (!b->hs.empty()) (void)hipSetDevice(b->hs[0]->dev);
+    if (b->ms) (void)hipStreamSynchronize(b->ms);
+    batch_drop_graph(b);
+    for (H* h : b->hs) ples_mg_destroy(h);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (hipEvent_t e : b->join)
+        if (e) (void)hipEventDestroy(e);
+    for (int q = 0; q < 2; ++q)
+        if (b->rev[q]) (void)hipEventDestroy(b->rev[q]);
+    if (b->dS) (void)hipFree(b->dS);
+    if (b->Sall) (void)hipFree(b->Sall);
+    if (b->ring) (void)hipHostFree(b->ring);
+    if (b->ms) (void)hipStreamDestroy(b->ms);
+    delete b;
+    return PLES_MG_OK;
+}
+
+int ples_mg_batch_size(const ples_mg_batch* b) { return b ? (int)b->hs.size() : 0; }
+
+ples_mg_handle* ples_mg_batch_handle(ples_mg_batch* b, int i) {
+    if (!b || i < 0 || i >= (int)b->hs.size()) return nullptr;
+    return b->hs[i];
+}
+
+size_t ples_mg_batch_device_bytes(const ples_mg_batch* b) {
+    if (!b) return 0;
+    size_t t = 0;
+    for (const H* h : b->hs) t += h->bytes;
+    return t + b->hs.size() * (sizeof(double*) + S_COUNT * sizeof(double));
+}
+
+int ples_mg_batch_solve(ples_mg_batch* b, double eps, int maxcycles, int steps_per_launch, ples_mg_result* res) {
+    if (!b || !(eps > 0.0) || maxcycles < 0 || steps_per_launch < 0 || steps_per_launch > 16) return PLES_MG_ERR_ARG;
+    const int n = (int)b->hs.size();
+    CK(hipSetDevice(b->dev));
+    const int K = steps_per_launch > 0 ? steps_per_launch : 2;
+    if (K != b->K) {
+        batch_drop_graph(b);
+        b->K = K;
+    }
+    b->q.assign(n, 0);
+    bool use_graph = true;
+    for (H* h : b->hs) use_graph = use_graph && h->o.graphs && h->timing == 0;
+    hipEvent_t t0, t1;
+    CK(hipEventCreate(&t0));
+    CK(hipEventCreate(&t1));
+    CK(hipEventRecord(t0, b->ms));
+    // initial residuals, every problem on its own stream
+    std::vector<double> bnorm(n), thr(n), last(n), rho(n, 0.5);
+    std::vector<int> done(n, 0), cycles(n, 0), status(n, PLES_MG_NOT_CONVERGED);
+    std::vector<double> rr(n), rmax(n);
+    for (int i = 0; i < n; ++i) {
+        H* h = b->hs[i];
+        CK(hipStreamWaitEvent(h->s, t0, 0));
+        CKS(outer_residual(h, h->x, h->rvec));
+        CK(hipMemcpyAsync(h->Sh, h->S, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, h->s));
+    }
+    for (int i = 0; i < n; ++i) {
+        H* h = b->hs[i];
+        CK(hipStreamSynchronize(h->s));
+        const bool absmode = h->o.stop_mode == PLES_MG_STOP_ABS_MAX;
+        bnorm[i] = std::sqrt(h->Sh[S_BB]) > 0 ? std::sqrt(h->Sh[S_BB]) : 1.0;
+        thr[i] = absmode ? eps : (eps * bnorm[i]) * (eps * bnorm[i]);
+        rr[i] = h->Sh[S_RR];
+        rmax[i] = h->Sh[S_RMAX];
+        if (!std::isfinite(rr[i])) {
+            status[i] = PLES_MG_ERR_NONFINITE;
+            done[i] = 1;
+        }
+        last[i] = absmode ? rmax[i] : std::sqrt(rr[i]) / bnorm[i];
+        CKS(arm_stop(h, thr[i]));
+        if (!done[i] && last[i] < eps) {
+            status[i] = PLES_MG_OK;
+            done[i] = 1;
+        }
+        if (done[i]) {
+            h->Sh[S_DONE] = 1.0;
+            CK(hipMemcpyAsync(h->S + S_DONE, h->Sh + S_DONE, sizeof(double), hipMemcpyHostToDevice, h->s));
+        }
+        CK(hipEventRecord(b->join[i], h->s));
+        CK(hipStreamWaitEvent(b->ms, b->join[i], 0));
+    }
+    auto measure = [&](int i, double r2, double rm) {
+        return b->hs[i]->o.stop_mode == PLES_MG_STOP_ABS_MAX ? rm : std::sqrt(r2) / bnorm[i];
+    };
+    auto all_done = [&]() {
+        for (int i = 0; i < n; ++i)
+            if (!done[i]) return false;
+        return true;
+    };
+    int enq = 0, seen = 0;     // launches (of K steps) enqueued / read back
+    int base = 0;              // steps applied before the current arming
+    while (!all_done()) {
+        while (enq - seen < 2 && base + (enq + 1) * K <= std::max(maxcycles, K)) {
+            if (enq > seen) {
+                // stop running ahead once every open problem is predicted to
+                // converge inside the launch in flight
+                bool any_left = false;
+                for (int i = 0; i < n; ++i)
+                    if (!done[i] && last[i] * std::pow(rho[i], (double)((enq - seen) * K)) >= 4.0 * eps) any_left = true;
+                if (!any_left) break;
+            }
+            CKS(batch_enqueue_chunk(b, use_graph));
+            double* slot = b->ring + (size_t)(enq % 2) * n * S_COUNT;
+            CK(hipMemcpyAsync(slot, b->Sall, (size_t)n * S_COUNT * sizeof(double), hipMemcpyDeviceToHost, b->ms));
+            CK(hipEventRecord(b->rev[enq % 2], b->ms));
+            ++enq;
+        }
+        if (seen == enq) break;  // cycle cap
+        CK(hipEventSynchronize(b->rev[seen % 2]));
+        const double* slot = b->ring + (size_t)(seen % 2) * n * S_COUNT;
+        ++seen;
+        for (int i = 0; i < n; ++i) {
+            if (done[i]) continue;
+            const double* Si = slot + (size_t)i * S_COUNT;
+            const double m = measure(i, Si[S_RR], Si[S_RMAX]);
+            if (!std::isfinite(Si[S_RR])) {
+                status[i] = PLES_MG_ERR_NONFINITE;
+                done[i] = 1;
+                CKS(batch_set_done(b, i, 1.0));
+                continue;
+            }
+            if (Si[S_DONE] != 0.0) {
+                done[i] = 2;  // converged on the device; confirmed below
+                cycles[i] = base + (int)Si[S_CYC];
+                continue;
+            }
+            if (last[i] > 0 && m > 0) rho[i] = std::min(0.95, std::max(1e-3, std::pow(m / last[i], 1.0 / K)));
+            last[i] = m;
+            if (base + (int)Si[S_CYC] >= maxcycles) {
+                cycles[i] = base + (int)Si[S_CYC];
+                done[i] = 1;  // cap reached: freeze
+                CKS(batch_set_done(b, i, 1.0));
+            }
+        }
+        if (!all_done()) continue;
+        // every problem finished: confirm the device stop test with true residuals
+        CK(hipStreamSynchronize(b->ms));
+        for (int i = 0; i < n; ++i) {
+            if (done[i] != 2) continue;
+            H* h = b->hs[i];
+            CKS(outer_residual(h, h->x, h->rvec));
+            CK(hipMemcpyAsync(h->Sh, h->S, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, h->s));
+        }
+        bool restart = false;
+        for (int i = 0; i < n; ++i) {
+            if (done[i] != 2) continue;
+            H* h = b->hs[i];
+            CK(hipStreamSynchronize(h->s));
+            rr[i] = h->Sh[S_RR];
+            rmax[i] = h->Sh[S_RMAX];
+            if (measure(i, rr[i], rmax[i]) < eps) {
+                status[i] = PLES_MG_OK;
+                done[i] = 1;
+            } else if (cycles[i] < maxcycles) {
+                // the recurrence drifted from the true residual: restart it
+                CKS(arm_stop(h, thr[i]));
+                done[i] = 0;
+                last[i] = measure(i, rr[i], rmax[i]);
+                restart = true;
+            } else {
+                done[i] = 1;
+            }
+        }
+        if (restart) {
+            // re-armed problems count their steps from zero again
+            int mn = maxcycles;
+            for (int i = 0; i < n; ++i)
+                if (!done[i]) mn = std::min(mn, cycles[i]);
+            base = mn;
+            enq = seen = 0;
+            for (int i = 0; i < n; ++i) {
+                CK(hipEventRecord(b->join[i], b->hs[i]->s));
+                CK(hipStreamWaitEvent(b->ms, b->join[i], 0));
+            }
+        }
+    }
+    CK(hipStreamSynchronize(b->ms));
+    // final state: ghosts of every iterate, last residual norms of unconfirmed ones
+    for (int i = 0; i < n; ++i) {
+        H* h = b->hs[i];
+        if (status[i] != PLES_MG_OK && seen > 0) {
+            const double* Si = b->ring + (size_t)((seen - 1) % 2) * n * S_COUNT + (size_t)i * S_COUNT;
+            rr[i] = Si[S_RR];
+            rmax[i] = Si[S_RMAX];
+            if (cycles[i] == 0) cycles[i] = base + (int)Si[S_CYC];
+        }
+        CKS(halo64(h, h->x, true));
+        CK(hipEventRecord(b->join[i], h->s));
+        CK(hipStreamWaitEvent(b->ms, b->join[i], 0));
+    }
+    CK(hipEventRecord(t1, b->ms));
+    CK(hipEventSynchronize(t1));
+    float ms = 0.f;
+    CK(hipEventElapsedTime(&ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    int worst = PLES_MG_OK;
+    for (int i = 0; i < n; ++i) {
+        if (res) {
+            res[i].status = status[i];
+            res[i].cycles = cycles[i];
+            res[i].rel_res = std::sqrt(rr[i]) / bnorm[i];
+            res[i].abs_res = rmax[i];
+            res[i].rhs_mean_removed = b->hs[i]->last_removed;
+            res[i].seconds = ms * 1e-3;
+        }
+        if (status[i] != PLES_MG_OK && worst == PLES_MG_OK) worst = status[i];
+    }
+    return worst;
 }
 
 }  // extern "C"

@@ -54,7 +54,7 @@ class _Opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("nu_pre", C.c_int), ("nu_post", C.c_int),
                 ("nu_coarse", C.c_int), ("omega", C.c_double), ("krylov", C.c_int),
                 ("stop_mode", C.c_int), ("project_rhs", C.c_int), ("halo_passes", C.c_int),
-                ("device", C.c_int), ("graphs", C.c_int)]
+                ("device", C.c_int), ("graphs", C.c_int), ("inner_cycles", C.c_int)]
 
 
 class _Result(C.Structure):
@@ -114,6 +114,14 @@ def lib():
         L.ples_mg_projection_step.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp,
                                               _dp, _dp, _dp, C.POINTER(_dp), _dp, C.c_double, C.c_double,
                                               C.c_int, C.c_int, C.POINTER(_Result), _dp]
+        L.ples_mg_batch_create.argtypes = [C.c_int, C.POINTER(_Grid), C.POINTER(_Opts), C.POINTER(C.c_void_p)]
+        L.ples_mg_batch_destroy.argtypes = [C.c_void_p]
+        L.ples_mg_batch_size.argtypes = [C.c_void_p]
+        L.ples_mg_batch_handle.argtypes = [C.c_void_p, C.c_int]
+        L.ples_mg_batch_handle.restype = C.c_void_p
+        L.ples_mg_batch_device_bytes.argtypes = [C.c_void_p]
+        L.ples_mg_batch_device_bytes.restype = C.c_size_t
+        L.ples_mg_batch_solve.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.POINTER(_Result)]
         L.ples_mg_case_rhs.argtypes = [C.c_void_p, C.c_uint64, C.c_double]
         L.ples_mg_zero_fi.argtypes = [C.c_void_p]
         L.ples_mg_perturb_fi.argtypes = [C.c_void_p, C.c_uint64, C.c_double]
@@ -184,7 +192,20 @@ def estimate_bytes(x, y, z, ip=0, jp=1, kp=0, nlevel=4, precision="fp64", **kw) 
 class Solver:
     """Device-resident multigrid for one grid (one handle, one HIP stream)."""
 
+    @classmethod
+    def _borrow(cls, handle, x, y, z, ip, jp, kp, nlevel):
+        """A view of a handle owned elsewhere (a batch): never destroyed here."""
+        s = cls.__new__(cls)
+        s._x, s._y, s._z = x, y, z
+        s.jz, s.jy, s.jx = (d - 1 for d in x.shape)
+        s.ip, s.jp, s.kp, s.nlevel = ip, jp, kp, nlevel
+        s._metrics = None
+        s._h = C.c_void_p(handle)
+        s._owned = False
+        return s
+
     def __init__(self, x, y, z, ip=0, jp=1, kp=0, nlevel=4, metrics=None, precision="fp64", **opts):
+        self._owned = True
         self._x, self._y, self._z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
         self.jz, self.jy, self.jx = (s - 1 for s in self._x.shape)
         self.ip, self.jp, self.kp, self.nlevel = ip, jp, kp, nlevel
@@ -196,7 +217,7 @@ class Solver:
         self._h = h
 
     def close(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and getattr(self, "_owned", True):
             lib().ples_mg_destroy(self._h)
             self._h = None
 
@@ -395,3 +416,54 @@ class Solver:
         _check(lib().ples_mg_kernel_stats(self._h, fam, C.byref(n), C.byref(ms), C.byref(by)),
                "ples_mg_kernel_stats")
         return n.value, ms.value, by.value
+
+
+class Batch:
+    """n independent problems solved together (ples_mg_batch_*).  problems:
+    list of dicts with x, y, z (node arrays), nlevel and optional ip, kp."""
+
+    def __init__(self, problems, precision="mixed", **opts):
+        self._probs = []
+        grids = (_Grid * len(problems))()
+        for q, pr in enumerate(problems):
+            x, y, z = (np.ascontiguousarray(pr[c], dtype=np.float64) for c in ("x", "y", "z"))
+            ip, kp = pr.get("ip", 0), pr.get("kp", 0)
+            self._probs.append((x, y, z, ip, 1, kp, pr["nlevel"]))
+            grids[q] = _grid(x, y, z, ip, 1, kp, pr["nlevel"])
+        o = _opts(precision, **opts)
+        b = C.c_void_p()
+        _check(lib().ples_mg_batch_create(len(problems), grids, C.byref(o), C.byref(b)), "ples_mg_batch_create")
+        self._b = b
+        self.solvers = [Solver._borrow(lib().ples_mg_batch_handle(b, q), *self._probs[q])
+                        for q in range(len(problems))]
+
+    def __len__(self):
+        return len(self._probs)
+
+    def device_bytes(self) -> int:
+        return int(lib().ples_mg_batch_device_bytes(self._b))
+
+    def solve(self, eps=1e-8, maxcycles=200, steps_per_launch=0):
+        res = (_Result * len(self))()
+        st = lib().ples_mg_batch_solve(self._b, eps, maxcycles, steps_per_launch, res)
+        if st not in (OK, NOT_CONVERGED):
+            raise PlesMGError(st, "ples_mg_batch_solve")
+        return [Result(r.status, r.cycles, r.rel_res, r.abs_res, r.rhs_mean_removed, r.seconds) for r in res]
+
+    def close(self):
+        if getattr(self, "_b", None):
+            self.solvers = []
+            lib().ples_mg_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
