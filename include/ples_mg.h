@@ -81,6 +81,11 @@ typedef struct ples_mg_opts {
                         always launched one by one while kernel timing is on */
     int inner_cycles; /* V-cycles per refinement step, the first from zero, each
                         further one continuing from the last (0 = auto = 1) */
+    int autotune;    /* 1 = at create, time the candidate launch shapes of every
+                        level (smoother segments per column, residual segments x
+                        lanes) and keep the fastest; results are cached per level
+                        shape for the process, so equal shapes get equal kernels.
+                        0 = fixed rule (default) */
 } ples_mg_opts;
 
 typedef struct ples_mg_result {
@@ -236,6 +241,55 @@ size_t ples_mg_batch_device_bytes(const ples_mg_batch* b);
  * Returns PLES_MG_OK when all converged, else the first failing status. */
 int ples_mg_batch_solve(ples_mg_batch* b, double eps, int maxcycles, int steps_per_launch,
                         ples_mg_result* res);
+
+/* ---- serving loop ----
+ * A solve server for many cases (independent simulations, each a grid and a
+ * seeded divergence rhs).  Each request is one time step of a case: it
+ * perturbs the case's previous pressure, kept resident on the device as the
+ * warm start (relative std perturb_rel, seeded), or starts from zero when the
+ * case is cold, and solves to its own tolerance.  Requests are admitted while
+ * fewer than max_active run and the case is, or can be made, resident within
+ * hbm_budget device bytes and warm_cases resident cases (idle cases are
+ * evicted least recently used first, losing their warm start).  Requests that
+ * arrive while others run join at the next ples_mg_server_step; every running
+ * request advances on its own stream (graph-replayed refinement steps), so
+ * their kernels overlap on the device, and none waits for another to finish.
+ * No reference counterpart: PLES solves one field per time step in-line. */
+typedef struct ples_mg_server ples_mg_server;
+typedef struct ples_mg_server_opts {
+    size_t hbm_budget; /* device bytes for resident cases (0 = 80% of free memory) */
+    int max_active;    /* requests running at once (default 8) */
+    int warm_cases;    /* resident cases kept (default 16) */
+    int device;
+} ples_mg_server_opts;
+void ples_mg_server_default_opts(ples_mg_server_opts* so);
+int ples_mg_server_create(const ples_mg_server_opts* so, const ples_mg_opts* opts,
+                          ples_mg_server** out);
+int ples_mg_server_destroy(ples_mg_server* sv);
+/* Register a case (nodes are copied; no device memory until admitted).
+ * PLES_MG_ERR_BUSY if the case alone exceeds the budget. */
+int ples_mg_server_add_case(ples_mg_server* sv, uint64_t case_id, const ples_mg_grid* grid,
+                            uint64_t rhs_seed, double dt);
+int ples_mg_server_submit(ples_mg_server* sv, uint64_t case_id, uint64_t perturb_seed,
+                          double perturb_rel, double eps, int maxcycles, int64_t* ticket);
+/* Admit waiting requests and advance every running one without blocking;
+ * *n_done = requests completed in this call. */
+int ples_mg_server_step(ples_mg_server* sv, int* n_done);
+/* Block until some running request's next device result is available. */
+int ples_mg_server_wait(ples_mg_server* sv);
+/* PLES_MG_OK and the result (seconds = submit-to-completion wall time,
+ * *warm = 1 if it started from the case's previous solution) once done,
+ * PLES_MG_ERR_BUSY while pending. */
+int ples_mg_server_poll(ples_mg_server* sv, int64_t ticket, ples_mg_result* res, int* warm);
+int ples_mg_server_pending(const ples_mg_server* sv);
+/* The case's resident pressure (layout of ples_mg_get_fi); ERR_BUSY if cold or running. */
+int ples_mg_server_get_fi(ples_mg_server* sv, uint64_t case_id, double* fi);
+int ples_mg_server_stats(const ples_mg_server* sv, size_t* used_bytes, int* resident,
+                         long* completed, long* evictions, long* cold_starts);
+
+/* Launch shape of level `level` (1-based): smoother segments per column P,
+ * V-cycle residual segments rP x lanes rbx (outputs may be NULL). */
+int ples_mg_tiles(const ples_mg_handle* h, int level, int* P, int* rP, int* rbx);
 
 /* HIP stream the handle launches on (hipStream_t as void*). */
 void* ples_mg_stream(ples_mg_handle* h);

@@ -81,3 +81,35 @@ def c4_problems(n=64, seed0=2000, sizes=C4_SIZES):
         x, y, z = make_nodes(nx, ny, nz, 1.5, amp)
         probs.append(dict(x=x, y=y, z=z, nlevel=nlevel, seed=seed0 + b, shape=(nx, ny, nz), amp=amp))
     return probs
+
+
+# Serving workload: cases with sizes log-uniform between 32^3 and
+# 256x128x256 cells, aspect 2:1:2, dims multiples of 2^(L-1) (at least two
+# wall-normal cells on the coarsest level), terrain amplitude 0.1 H,
+# tanh 1.5 stretching.  Requests arrive as a Poisson process; each picks a
+# case and a tolerance of 1e-6 or 1e-10 with equal probability.
+def c5_cases(n_cases=24, seed=3000):
+    rng = np.random.default_rng(seed)
+    out = []
+    lo, hi = np.log(32.0 ** 3), np.log(256.0 * 128 * 256)
+    for c in range(n_cases):
+        cells = float(np.exp(rng.uniform(lo, hi)))
+        a = (cells / 4.0) ** (1.0 / 3.0)
+        nlevel = int(max(2, min(6, np.floor(np.log2(a)) - 1)))
+        m = 2 ** (nlevel - 1)
+        a = int(max(2 * m, round(a / m) * m))
+        nx, ny, nz = 2 * a, a, 2 * a
+        x, y, z = make_nodes(nx, ny, nz, 1.5, 0.1 * H)
+        out.append(dict(case_id=c, x=x, y=y, z=z, nlevel=nlevel, shape=(nx, ny, nz), rhs_seed=3500 + c))
+    return out
+
+
+def c5_requests(n_req, n_cases, rate=200.0, seed=3000):
+    """(arrival time s, case, tolerance, perturbation seed) per request."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+    reqs = []
+    for r in range(n_req):
+        q = np.random.default_rng(3001 + r)
+        reqs.append((float(t[r]), int(q.integers(n_cases)), 1e-6 if q.random() < 0.5 else 1e-10, 3001 + r))
+    return reqs

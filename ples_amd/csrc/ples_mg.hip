@@ -3,7 +3,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
+#include <mutex>
 #include <chrono>
+#include <deque>
+#include <unordered_map>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -27,7 +32,8 @@ struct Level {
     float* cpr = nullptr;
     float* lv_v = nullptr;  // segmented line-solve responses
     float* lv_w = nullptr;
-    int P = 1;              // eta segments per column
+    int P = 1;              // eta segments per column (smoother, outer residual)
+    int rP = 0, rbx = 0;    // V-cycle residual launch: segments x lanes (0 = as the smoother)
     void* p[2] = {nullptr, nullptr};
     void* b = nullptr;
     void* r = nullptr;
@@ -360,12 +366,18 @@ int smooth(H* h, Level& L, const TR* rhs, int nu, bool zero_start, bool fine) {
 
 template <class T, class TR>
 int vresid(H* h, Level& L, const TR* rhs, bool fine) {
-    const ColLaunch cl = seg_launch(L.G, L.P);
+    ColLaunch cl = seg_launch(L.G, L.P);
+    int P = L.P;
+    if (L.rP > 0) {  // tuned launch shape (the residual of a cell does not depend on it)
+        P = L.rP;
+        cl.block = dim3(L.rbx, P, 1);
+        cl.grid = dim3((L.G.nx + L.rbx - 1) / L.rbx, L.G.nz, 1);
+    }
     const double N = (double)L.ncell();
     KT kt(h, fine ? PLES_MG_K_RESID_FINE : PLES_MG_K_RESID, N * (9 * sizeof(T) + 2 * sizeof(T) + sizeof(TR)));
     hipLaunchKernelGGL((k_resid<0, T, T, TR, T>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L),
                        (const T*)L.p[L.cur], rhs, (T*)L.r, (T)(1.0 / L.pot), (const double*)nullptr,
-                       (const double*)nullptr, (double*)nullptr, 0, L.P);
+                       (const double*)nullptr, (double*)nullptr, 0, P);
     CK(hipGetLastError());
     return PLES_MG_OK;
 }
@@ -776,7 +788,7 @@ int compute_metrics(H* h, int n, double* const g[9], double* const v[9]) {
 }
 
 int level_factors(H* h, Level& L) {
-    if ((L.P > 1) != (L.lv_v != nullptr && L.lv_w != nullptr)) return PLES_MG_ERR_ARG;
+    if (L.P > 1 && (L.lv_v == nullptr || L.lv_w == nullptr)) return PLES_MG_ERR_ARG;
     if (L.G.ny % L.P != 0 || L.P > 16) return PLES_MG_ERR_ARG;
     const ColLaunch cl = seg_launch(L.G, L.P);
     if (L.f32)
@@ -786,6 +798,105 @@ int level_factors(H* h, Level& L) {
         hipLaunchKernelGGL((k_line_factors<double>), cl.grid, cl.block, 0, h->s, L.G, met<double>(L),
                            1.0 / L.pot, L.P, L.invm, L.cpr, L.lv_v, L.lv_w);
     CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+// Launch shapes of the level kernels, chosen by timing the candidates on the
+// level's own arrays: the smoother's eta segments per column P (1..16,
+// dividing ny, >= 4 rows each; the lanes per block follow from P) and the
+// V-cycle residual's segments x lanes.  Results are kept per level shape and
+// precision for the life of the process, so every handle of a shape runs the
+// same kernels (the smoother's rounding depends on P; the residual's does not).
+struct TuneVal {
+    int P, rP, rbx;
+};
+std::mutex g_tune_mu;
+std::map<std::array<int, 7>, TuneVal> g_tune;
+
+std::vector<int> seg_candidates(int ny) {
+    std::vector<int> v;
+    for (int P = 1; P <= 16; P *= 2)
+        if (ny % P == 0 && ny / P >= 4) v.push_back(P);
+    return v;
+}
+
+template <class F>
+int time_launches(H* h, int reps, double& ms, F&& f) {
+    hipEvent_t a, b;
+    CK(hipEventCreate(&a));
+    CK(hipEventCreate(&b));
+    int st = f();  // warm-up
+    if (st == PLES_MG_OK) {
+        (void)hipEventRecord(a, h->s);
+        for (int r = 0; r < reps && st == PLES_MG_OK; ++r) st = f();
+        (void)hipEventRecord(b, h->s);
+        (void)hipEventSynchronize(b);
+        float e = 0.f;
+        (void)hipEventElapsedTime(&e, a, b);
+        ms = e / reps;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return st;
+}
+
+int lv_smooth(H* h, int n, const double* r0, int nu, bool zero);
+int lv_resid(H* h, int n, const double* r0);
+bool coarse_one_block(const Level& L);
+
+int tune_level(H* h, int n) {
+    Level& L = h->lv[n];
+    if (n == h->L - 1 && coarse_one_block(L)) return PLES_MG_OK;  // whole-level LDS solver
+    const std::array<int, 7> key{L.G.nx, L.G.ny, L.G.nz, L.G.ip, L.G.kp, L.f32 ? 1 : 0, n == 0 ? 1 : 0};
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    const size_t cells = (size_t)L.G.n;
+    auto ensure_vw = [&]() -> int {
+        if (!L.lv_v) CKS(dalloc(h, (void**)&L.lv_v, cells * 4));
+        if (!L.lv_w) CKS(dalloc(h, (void**)&L.lv_w, cells * 4));
+        return PLES_MG_OK;
+    };
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+        if (it->second.P > 1) CKS(ensure_vw());
+        L.P = it->second.P;
+        L.rP = it->second.rP;
+        L.rbx = it->second.rbx;
+        return level_factors(h, L);
+    }
+    const double* r0 = h->rvec;  // level-0 rhs (any content: timing only)
+    TuneVal best{L.P, 0, 0};
+    double tbest = 1e30;
+    for (int P : seg_candidates(L.G.ny)) {
+        if (P > 1) CKS(ensure_vw());
+        L.P = P;
+        CKS(level_factors(h, L));
+        double ms = 0;
+        CKS(time_launches(h, 3, ms, [&] { return lv_smooth(h, n, r0, 2, false); }));
+        if (ms < tbest) {
+            tbest = ms;
+            best.P = P;
+        }
+    }
+    L.P = best.P;
+    CKS(level_factors(h, L));
+    tbest = 1e30;
+    for (int P : seg_candidates(L.G.ny))
+        for (int bx = 32; bx <= 256; bx *= 2) {
+            if (bx * P > 512 || bx > 2 * L.G.nx) continue;
+            L.rP = P;
+            L.rbx = bx;
+            double ms = 0;
+            CKS(time_launches(h, 3, ms, [&] { return lv_resid(h, n, r0); }));
+            if (ms < tbest) {
+                tbest = ms;
+                best.rP = P;
+                best.rbx = bx;
+            }
+        }
+    L.rP = best.rP;
+    L.rbx = best.rbx;
+    g_tune[key] = best;
     return PLES_MG_OK;
 }
 
@@ -850,6 +961,7 @@ void ples_mg_default_opts(ples_mg_opts* o) {
     o->device = 0;
     o->graphs = 1;
     o->inner_cycles = 0;  // auto
+    o->autotune = 0;
 }
 
 const char* ples_mg_status_string(int s) {
@@ -989,8 +1101,12 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
                            (size_t)g->jx * g->jz, (size_t)g->jx * g->jy, (size_t)g->jx * g->jy};
     for (int q = 0; q < 6; ++q) CKS(dalloc(h, (void**)&h->cs[q], csz[q] * 8));
     CKS(dalloc(h, (void**)&h->S, S_COUNT * sizeof(double)));
-    h->part_n = std::max(std::max(col_launch(h->lv[0].G).nblocks, seg_launch(h->lv[0].G, h->lv[0].P).nblocks), 1);
+    // partial sums of the widest launch any segment count can give (32 lanes)
+    h->part_n = std::max(col_launch(h->lv[0].G).nblocks, ((h->lv[0].G.nx + 31) / 32) * h->lv[0].G.nz);
+    h->part_n = std::max(h->part_n, 1);
     CKS(dalloc(h, (void**)&h->part, (size_t)h->part_n * 4 * sizeof(double)));
+    if (o.autotune)
+        for (int n = 0; n < h->L; ++n) CKS(tune_level(h, n));
     CK(hipStreamSynchronize(h->s));
     return PLES_MG_OK;
 }
@@ -2028,6 +2144,429 @@ int ples_mg_batch_solve(ples_mg_batch* b, double eps, int maxcycles, int steps_p
         if (status[i] != PLES_MG_OK && worst == PLES_MG_OK) worst = status[i];
     }
     return worst;
+}
+
+
+int ples_mg_tiles(const ples_mg_handle* h, int level, int* P, int* rP, int* rbx) {
+    if (!h || level < 1 || level > h->L) return PLES_MG_ERR_ARG;
+    const Level& L = h->lv[level - 1];
+    const ColLaunch cl = seg_launch(L.G, L.P);
+    if (P) *P = L.P;
+    if (rP) *rP = L.rP > 0 ? L.rP : L.P;
+    if (rbx) *rbx = L.rP > 0 ? L.rbx : (int)cl.block.x;
+    return PLES_MG_OK;
+}
+
+
+// ---------------------------------------------------------------- serving loop
+}  // extern "C"
+
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct SCase {
+    uint64_t id = 0;
+    int jx = 0, jy = 0, jz = 0, ip = 0, kp = 0, nlevel = 1;
+    std::vector<double> x, y, z;  // host copy of the nodes
+    uint64_t rhs_seed = 0;
+    double dt = 1.0;
+    H* h = nullptr;               // resident handle (nullptr = cold)
+    size_t est = 0;               // device bytes when resident
+    bool busy = false;
+    bool solved = false;          // the resident iterate holds a previous solution
+    uint64_t last_use = 0;
+};
+
+enum { RQ_QUEUED = 0, RQ_INIT, RQ_RUN, RQ_CONFIRM, RQ_DONE };
+
+struct SReq {
+    int64_t ticket = 0;
+    uint64_t case_id = 0;
+    uint64_t seed = 0;
+    double rel = 0.0, eps = 1e-10;
+    int maxcyc = 200;
+    int state = RQ_QUEUED;
+    bool warm = false;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    double bnorm = 1, thr = 0, last = 0, rho = 0.5;
+    int enq = 0, seen = 0, q = 0, cycles = 0, base = 0;
+    hipEvent_t ev = nullptr;       // init / confirmation read-back
+    ples_mg_result res{};
+};
+
+}  // namespace
+
+struct ples_mg_server {
+    ples_mg_server_opts so{};
+    ples_mg_opts o{};
+    std::unordered_map<uint64_t, SCase> cases;
+    std::deque<int64_t> queue;                    // waiting tickets, arrival order
+    std::unordered_map<int64_t, SReq> reqs;
+    std::vector<int64_t> active;
+    int64_t next_ticket = 1;
+    uint64_t clock = 0;
+    size_t used = 0;
+    long completed = 0, evictions = 0, cold_starts = 0;
+};
+
+namespace {
+
+using SV = ples_mg_server;
+
+ples_mg_grid case_grid(const SCase& c) {
+    ples_mg_grid g{};
+    g.jx = c.jx; g.jy = c.jy; g.jz = c.jz;
+    g.ip = c.ip; g.jp = 1; g.kp = c.kp;
+    g.nlevel = c.nlevel;
+    g.x = c.x.data(); g.y = c.y.data(); g.z = c.z.data();
+    return g;
+}
+
+void evict(SV* sv, SCase& c) {
+    if (!c.h) return;
+    sv->used -= c.est;
+    ples_mg_destroy(c.h);
+    c.h = nullptr;
+    c.solved = false;
+    ++sv->evictions;
+}
+
+int resident_count(const SV* sv) {
+    int n = 0;
+    for (const auto& kv : sv->cases) n += kv.second.h != nullptr;
+    return n;
+}
+
+// make room for `need` bytes (and a resident slot) by evicting idle cases,
+// least recently used first; false if that cannot be done now
+bool make_room(SV* sv, size_t need, bool need_slot) {
+    for (;;) {
+        const bool mem_ok = sv->used + need <= sv->so.hbm_budget;
+        const bool slot_ok = !need_slot || resident_count(sv) < sv->so.warm_cases;
+        if (mem_ok && slot_ok) return true;
+        SCase* lru = nullptr;
+        for (auto& kv : sv->cases) {
+            SCase& c = kv.second;
+            if (c.h && !c.busy && (!lru || c.last_use < lru->last_use)) lru = &c;
+        }
+        if (!lru) return false;
+        evict(sv, *lru);
+    }
+}
+
+// admission: the case must be idle, resident or creatable within the budget
+int try_admit(SV* sv, SReq& r) {
+    SCase& c = sv->cases.at(r.case_id);
+    if (c.busy) return 0;
+    if ((int)sv->active.size() >= sv->so.max_active) return 0;
+    if (!c.h) {
+        if (!make_room(sv, c.est, true)) return 0;
+        ples_mg_grid g = case_grid(c);
+        ples_mg_handle* h = nullptr;
+        ples_mg_opts o = sv->o;
+        o.device = sv->so.device;
+        const int st = ples_mg_create(&g, &o, &h);
+        if (st == PLES_MG_ERR_OOM) return 0;
+        if (st != PLES_MG_OK) return st;
+        c.h = h;
+        c.est = std::max(c.est, h->bytes);
+        sv->used += c.est;
+        CKS(ples_mg_case_rhs(h, c.rhs_seed, c.dt));
+        ++sv->cold_starts;
+    }
+    H* h = c.h;
+    c.busy = true;
+    c.last_use = ++sv->clock;
+    r.warm = c.solved;
+    r.t_admit = now_s();
+    // warm: perturb the previous solution; cold: start from zero
+    if (r.warm) {
+        const Level& L0 = h->lv[0];
+        const ColLaunch cl = col_launch(L0.G);
+        hipLaunchKernelGGL(k_sum, cl.grid, cl.block, 0, h->s, L0.G, (const double*)h->x, h->part, cl.nblocks);
+        hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1), 0, h->s, (const double*)h->part, cl.nblocks, 2, h->S);
+        hipLaunchKernelGGL(k_perturb, cl.grid, cl.block, 0, h->s, L0.G, h->x, (const double*)h->S, r.seed, r.rel,
+                           (double)L0.ncell());
+    } else {
+        CK(hipMemsetAsync(h->x, 0, (size_t)h->lv[0].G.n * 8, h->s));
+    }
+    CK(hipGetLastError());
+    CKS(outer_residual(h, h->x, h->rvec));
+    CK(hipMemcpyAsync(h->Sh, h->S, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, h->s));
+    if (!r.ev) CK(hipEventCreateWithFlags(&r.ev, hipEventDisableTiming));
+    CK(hipEventRecord(r.ev, h->s));
+    r.state = RQ_INIT;
+    sv->active.push_back(r.ticket);
+    return 1;
+}
+
+void finish(SV* sv, SReq& r, int status, double rr, double rmax) {
+    SCase& c = sv->cases.at(r.case_id);
+    H* h = c.h;
+    (void)halo64(h, h->x, true);
+    (void)hipStreamSynchronize(h->s);
+    r.t_done = now_s();
+    r.state = RQ_DONE;
+    r.res.status = status;
+    r.res.cycles = r.cycles;
+    r.res.rel_res = std::sqrt(rr) / r.bnorm;
+    r.res.abs_res = rmax;
+    r.res.rhs_mean_removed = h->last_removed;
+    r.res.seconds = r.t_done - r.t_submit;
+    c.busy = false;
+    c.solved = c.solved || status == PLES_MG_OK;
+    c.last_use = ++sv->clock;
+    if (r.ev) (void)hipEventDestroy(r.ev);
+    r.ev = nullptr;
+    ++sv->completed;
+}
+
+// advance one running request without blocking: enqueue refinement steps
+// (graph replays on its handle's stream) up to two in flight, consume the
+// read-backs that have landed, confirm convergence with the true residual
+int advance(SV* sv, SReq& r) {
+    SCase& c = sv->cases.at(r.case_id);
+    H* h = c.h;
+    const bool absmode = h->o.stop_mode == PLES_MG_STOP_ABS_MAX;
+    auto measure = [&](double rr, double rm) { return absmode ? rm : std::sqrt(rr) / r.bnorm; };
+    if (r.state == RQ_INIT || r.state == RQ_CONFIRM) {
+        const hipError_t q = hipEventQuery(r.ev);
+        if (q == hipErrorNotReady) return 0;
+        CK(q);
+        const double rr = h->Sh[S_RR], rmax = h->Sh[S_RMAX];
+        if (r.state == RQ_INIT) {
+            r.bnorm = std::sqrt(h->Sh[S_BB]) > 0 ? std::sqrt(h->Sh[S_BB]) : 1.0;
+            r.thr = absmode ? r.eps : (r.eps * r.bnorm) * (r.eps * r.bnorm);
+            r.last = measure(rr, rmax);
+        }
+        if (!std::isfinite(rr)) {
+            finish(sv, r, PLES_MG_ERR_NONFINITE, rr, rmax);
+            return 1;
+        }
+        if (measure(rr, rmax) < r.eps) {
+            finish(sv, r, PLES_MG_OK, rr, rmax);
+            return 1;
+        }
+        if (r.cycles >= r.maxcyc) {
+            finish(sv, r, PLES_MG_NOT_CONVERGED, rr, rmax);
+            return 1;
+        }
+        CKS(arm_stop(h, r.thr));
+        r.base = r.cycles;
+        r.enq = r.seen = 0;
+        r.q = 0;
+        r.last = measure(rr, rmax);
+        r.state = RQ_RUN;
+    }
+    int progressed = 0;
+    // consume landed read-backs
+    while (r.seen < r.enq) {
+        const hipError_t q = hipEventQuery(h->sev[r.seen % 4]);
+        if (q == hipErrorNotReady) break;
+        CK(q);
+        const double* slot = h->Sh + 2 * S_COUNT + (r.seen % 4) * S_COUNT;
+        ++r.seen;
+        progressed = 1;
+        const double rr = slot[S_RR], rmax = slot[S_RMAX];
+        if (!std::isfinite(rr)) {
+            CK(hipStreamSynchronize(h->s));
+            finish(sv, r, PLES_MG_ERR_NONFINITE, rr, rmax);
+            return 1;
+        }
+        r.cycles = r.base + (int)slot[S_CYC];
+        if (slot[S_DONE] != 0.0 || r.cycles >= r.maxcyc) {
+            // confirm with the true residual after the steps in flight
+            CKS(outer_residual(h, h->x, h->rvec));
+            CK(hipMemcpyAsync(h->Sh, h->S, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, h->s));
+            CK(hipEventRecord(r.ev, h->s));
+            r.state = RQ_CONFIRM;
+            return 1;
+        }
+        const double m = measure(rr, rmax);
+        if (r.last > 0 && m > 0) r.rho = std::min(0.95, std::max(1e-3, m / r.last));
+        r.last = m;
+    }
+    // keep up to two steps in flight unless the one in flight should converge
+    while (r.enq - r.seen < 2 && r.base + r.enq < r.maxcyc) {
+        if (r.enq > r.seen && r.last * std::pow(r.rho, (double)(r.enq - r.seen)) < 4.0 * r.eps) break;
+        CKS(enqueue_step(h, r.q));
+        r.q ^= 1;
+        double* slot = h->Sh + 2 * S_COUNT + (r.enq % 4) * S_COUNT;
+        CK(hipMemcpyAsync(slot, h->S, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, h->s));
+        CK(hipEventRecord(h->sev[r.enq % 4], h->s));
+        ++r.enq;
+        progressed = 1;
+    }
+    return progressed;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ples_mg_server_default_opts(ples_mg_server_opts* so) {
+    if (!so) return;
+    so->hbm_budget = 0;
+    so->max_active = 8;
+    so->warm_cases = 16;
+    so->device = 0;
+}
+
+int ples_mg_server_create(const ples_mg_server_opts* so, const ples_mg_opts* opts, ples_mg_server** out) {
+    if (!out) return PLES_MG_ERR_ARG;
+    *out = nullptr;
+    SV* sv = new SV();
+    ples_mg_server_default_opts(&sv->so);
+    if (so) sv->so = *so;
+    ples_mg_default_opts(&sv->o);
+    if (opts) sv->o = *opts;
+    if (sv->so.max_active < 1 || sv->so.warm_cases < 1) {
+        delete sv;
+        return PLES_MG_ERR_ARG;
+    }
+    if (hipSetDevice(sv->so.device) != hipSuccess) {
+        delete sv;
+        return PLES_MG_ERR_HIP;
+    }
+    if (sv->so.hbm_budget == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+            delete sv;
+            return PLES_MG_ERR_HIP;
+        }
+        sv->so.hbm_budget = (size_t)(0.8 * (double)fr);
+    }
+    *out = sv;
+    return PLES_MG_OK;
+}
+
+int ples_mg_server_destroy(ples_mg_server* sv) {
+    if (!sv) return PLES_MG_OK;
+    for (auto& kv : sv->cases)
+        if (kv.second.h) ples_mg_destroy(kv.second.h);
+    for (auto& kv : sv->reqs)
+        if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
+    delete sv;
+    return PLES_MG_OK;
+}
+
+int ples_mg_server_add_case(ples_mg_server* sv, uint64_t case_id, const ples_mg_grid* g, uint64_t rhs_seed,
+                            double dt) {
+    if (!sv || !g || !(dt > 0.0) || sv->cases.count(case_id)) return PLES_MG_ERR_ARG;
+    CKS(check_grid(g));
+    SCase c;
+    c.id = case_id;
+    c.jx = g->jx; c.jy = g->jy; c.jz = g->jz;
+    c.ip = g->ip; c.kp = g->kp;
+    c.nlevel = g->nlevel;
+    const size_t nn = (size_t)(g->jx + 1) * (g->jy + 1) * (g->jz + 1);
+    c.x.assign(g->x, g->x + nn);
+    c.y.assign(g->y, g->y + nn);
+    c.z.assign(g->z, g->z + nn);
+    c.rhs_seed = rhs_seed;
+    c.dt = dt;
+    c.est = estimate(g, &sv->o) + ((size_t)64 << 20);  // + allocator slack and temporaries
+    if (c.est > sv->so.hbm_budget) return PLES_MG_ERR_BUSY;  // can never be admitted
+    sv->cases.emplace(case_id, std::move(c));
+    return PLES_MG_OK;
+}
+
+int ples_mg_server_submit(ples_mg_server* sv, uint64_t case_id, uint64_t perturb_seed, double perturb_rel,
+                          double eps, int maxcycles, int64_t* ticket) {
+    if (!sv || !ticket || !(eps > 0.0) || maxcycles < 1 || !sv->cases.count(case_id)) return PLES_MG_ERR_ARG;
+    SReq r;
+    r.ticket = sv->next_ticket++;
+    r.case_id = case_id;
+    r.seed = perturb_seed;
+    r.rel = perturb_rel;
+    r.eps = eps;
+    r.maxcyc = maxcycles;
+    r.t_submit = now_s();
+    sv->reqs.emplace(r.ticket, r);
+    sv->queue.push_back(r.ticket);
+    *ticket = r.ticket;
+    return PLES_MG_OK;
+}
+
+int ples_mg_server_step(ples_mg_server* sv, int* n_done) {
+    if (!sv) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(sv->so.device));
+    int done = 0;
+    // admit in arrival order; a request whose case is busy waits without
+    // blocking the ones behind it
+    for (auto it = sv->queue.begin(); it != sv->queue.end();) {
+        if ((int)sv->active.size() >= sv->so.max_active) break;
+        SReq& r = sv->reqs.at(*it);
+        const int a = try_admit(sv, r);
+        if (a < 0 || a > 1) return a;  // error status
+        if (a == 1) it = sv->queue.erase(it);
+        else ++it;
+    }
+    for (size_t k = 0; k < sv->active.size();) {
+        SReq& r = sv->reqs.at(sv->active[k]);
+        const int st = advance(sv, r);
+        if (st < 0 || st > 1) return st;
+        if (r.state == RQ_DONE) {
+            ++done;
+            sv->active.erase(sv->active.begin() + k);
+        } else {
+            ++k;
+        }
+    }
+    if (n_done) *n_done = done;
+    return PLES_MG_OK;
+}
+
+int ples_mg_server_wait(ples_mg_server* sv) {
+    if (!sv) return PLES_MG_ERR_ARG;
+    // block on the oldest outstanding read-back of any running request
+    for (int64_t t : sv->active) {
+        SReq& r = sv->reqs.at(t);
+        H* h = sv->cases.at(r.case_id).h;
+        if ((r.state == RQ_INIT || r.state == RQ_CONFIRM) && r.ev) {
+            CK(hipEventSynchronize(r.ev));
+            return PLES_MG_OK;
+        }
+        if (r.state == RQ_RUN && r.seen < r.enq) {
+            CK(hipEventSynchronize(h->sev[r.seen % 4]));
+            return PLES_MG_OK;
+        }
+    }
+    return PLES_MG_OK;
+}
+
+int ples_mg_server_poll(ples_mg_server* sv, int64_t ticket, ples_mg_result* res, int* warm) {
+    if (!sv || !sv->reqs.count(ticket)) return PLES_MG_ERR_ARG;
+    const SReq& r = sv->reqs.at(ticket);
+    if (r.state != RQ_DONE) return PLES_MG_ERR_BUSY;
+    if (res) *res = r.res;
+    if (warm) *warm = r.warm ? 1 : 0;
+    return PLES_MG_OK;
+}
+
+int ples_mg_server_pending(const ples_mg_server* sv) {
+    return sv ? (int)(sv->queue.size() + sv->active.size()) : 0;
+}
+
+int ples_mg_server_get_fi(ples_mg_server* sv, uint64_t case_id, double* fi) {
+    if (!sv || !fi || !sv->cases.count(case_id)) return PLES_MG_ERR_ARG;
+    SCase& c = sv->cases.at(case_id);
+    if (!c.h || c.busy) return PLES_MG_ERR_BUSY;
+    return ples_mg_get_fi(c.h, fi);
+}
+
+int ples_mg_server_stats(const ples_mg_server* sv, size_t* used_bytes, int* resident, long* completed,
+                         long* evictions, long* cold_starts) {
+    if (!sv) return PLES_MG_ERR_ARG;
+    if (used_bytes) *used_bytes = sv->used;
+    if (resident) *resident = resident_count(sv);
+    if (completed) *completed = sv->completed;
+    if (evictions) *evictions = sv->evictions;
+    if (cold_starts) *cold_starts = sv->cold_starts;
+    return PLES_MG_OK;
 }
 
 }  // extern "C"

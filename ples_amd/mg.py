@@ -30,6 +30,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libples_mg.so")
 
 OK = 0
 NOT_CONVERGED = 1
+BUSY = 8
 FP64 = 0
 MIXED = 1
 STOP_REL_L2 = 0
@@ -54,12 +55,18 @@ class _Opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("nu_pre", C.c_int), ("nu_post", C.c_int),
                 ("nu_coarse", C.c_int), ("omega", C.c_double), ("krylov", C.c_int),
                 ("stop_mode", C.c_int), ("project_rhs", C.c_int), ("halo_passes", C.c_int),
-                ("device", C.c_int), ("graphs", C.c_int), ("inner_cycles", C.c_int)]
+                ("device", C.c_int), ("graphs", C.c_int), ("inner_cycles", C.c_int),
+                ("autotune", C.c_int)]
 
 
 class _Result(C.Structure):
     _fields_ = [("status", C.c_int), ("cycles", C.c_int), ("rel_res", C.c_double),
                 ("abs_res", C.c_double), ("rhs_mean_removed", C.c_double), ("seconds", C.c_double)]
+
+
+class _ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("warm_cases", C.c_int),
+                ("device", C.c_int)]
 
 
 @dataclass
@@ -122,6 +129,20 @@ def lib():
         L.ples_mg_batch_device_bytes.argtypes = [C.c_void_p]
         L.ples_mg_batch_device_bytes.restype = C.c_size_t
         L.ples_mg_batch_solve.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.POINTER(_Result)]
+        L.ples_mg_tiles.argtypes = [C.c_void_p, C.c_int] + [C.POINTER(C.c_int)] * 3
+        L.ples_mg_server_default_opts.argtypes = [C.POINTER(_ServerOpts)]
+        L.ples_mg_server_create.argtypes = [C.POINTER(_ServerOpts), C.POINTER(_Opts), C.POINTER(C.c_void_p)]
+        L.ples_mg_server_destroy.argtypes = [C.c_void_p]
+        L.ples_mg_server_add_case.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(_Grid), C.c_uint64, C.c_double]
+        L.ples_mg_server_submit.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_int,
+                                            C.POINTER(C.c_int64)]
+        L.ples_mg_server_step.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.ples_mg_server_wait.argtypes = [C.c_void_p]
+        L.ples_mg_server_poll.argtypes = [C.c_void_p, C.c_int64, C.POINTER(_Result), C.POINTER(C.c_int)]
+        L.ples_mg_server_pending.argtypes = [C.c_void_p]
+        L.ples_mg_server_get_fi.argtypes = [C.c_void_p, C.c_uint64, _dp]
+        L.ples_mg_server_stats.argtypes = [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_long)]
         L.ples_mg_case_rhs.argtypes = [C.c_void_p, C.c_uint64, C.c_double]
         L.ples_mg_zero_fi.argtypes = [C.c_void_p]
         L.ples_mg_perturb_fi.argtypes = [C.c_void_p, C.c_uint64, C.c_double]
@@ -397,6 +418,12 @@ class Solver:
     def device_bytes(self) -> int:
         return int(lib().ples_mg_device_bytes(self._h))
 
+    def tiles(self, level: int):
+        """(smoother segments P, residual segments rP, residual lanes rbx) of a level."""
+        v = [C.c_int(), C.c_int(), C.c_int()]
+        _check(lib().ples_mg_tiles(self._h, level, *[C.byref(a) for a in v]), "ples_mg_tiles")
+        return tuple(a.value for a in v)
+
     def set_timing(self, families=None):
         """Time the given kernel families (names or ids); None/False = off,
         True = all."""
@@ -455,6 +482,89 @@ class Batch:
             self.solvers = []
             lib().ples_mg_batch_destroy(self._b)
             self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Server:
+    """Serving loop (ples_mg_server_*): cases with resident warm starts,
+    HBM-budget admission, continuous batching of running requests."""
+
+    def __init__(self, hbm_budget=0, max_active=8, warm_cases=16, precision="mixed", **opts):
+        so = _ServerOpts()
+        lib().ples_mg_server_default_opts(C.byref(so))
+        so.hbm_budget, so.max_active, so.warm_cases = int(hbm_budget), max_active, warm_cases
+        o = _opts(precision, **opts)
+        sv = C.c_void_p()
+        _check(lib().ples_mg_server_create(C.byref(so), C.byref(o), C.byref(sv)), "ples_mg_server_create")
+        self._sv = sv
+        self._cases = {}
+
+    def add_case(self, case_id, x, y, z, nlevel, rhs_seed, dt=1.0, ip=0, kp=0):
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+        self._cases[case_id] = (x.shape, (x, y, z))
+        g = _grid(x, y, z, ip, 1, kp, nlevel)
+        _check(lib().ples_mg_server_add_case(self._sv, case_id, C.byref(g), rhs_seed, dt), "ples_mg_server_add_case")
+
+    def submit(self, case_id, perturb_seed, perturb_rel=1e-3, eps=1e-10, maxcycles=200) -> int:
+        t = C.c_int64()
+        _check(lib().ples_mg_server_submit(self._sv, case_id, perturb_seed, perturb_rel, eps, maxcycles,
+                                           C.byref(t)), "ples_mg_server_submit")
+        return t.value
+
+    def step(self) -> int:
+        n = C.c_int()
+        _check(lib().ples_mg_server_step(self._sv, C.byref(n)), "ples_mg_server_step")
+        return n.value
+
+    def wait(self):
+        _check(lib().ples_mg_server_wait(self._sv), "ples_mg_server_wait")
+
+    def poll(self, ticket):
+        """(Result, warm) once done, else None."""
+        r = _Result()
+        w = C.c_int()
+        st = lib().ples_mg_server_poll(self._sv, ticket, C.byref(r), C.byref(w))
+        if st == BUSY:
+            return None
+        _check(st, "ples_mg_server_poll")
+        return Result(r.status, r.cycles, r.rel_res, r.abs_res, r.rhs_mean_removed, r.seconds), bool(w.value)
+
+    def pending(self) -> int:
+        return int(lib().ples_mg_server_pending(self._sv))
+
+    def drain(self):
+        while self.pending():
+            if self.step() == 0:
+                self.wait()
+
+    def get_fi(self, case_id):
+        (nzp, nyp, nxp), _ = self._cases[case_id]
+        a = np.zeros((nzp + 1, nyp + 1, nxp + 1))
+        _check(lib().ples_mg_server_get_fi(self._sv, case_id, _p(a)), "ples_mg_server_get_fi")
+        return a
+
+    def stats(self) -> dict:
+        u, r = C.c_size_t(), C.c_int()
+        c, e, k = C.c_long(), C.c_long(), C.c_long()
+        _check(lib().ples_mg_server_stats(self._sv, C.byref(u), C.byref(r), C.byref(c), C.byref(e), C.byref(k)),
+               "ples_mg_server_stats")
+        return dict(used_bytes=u.value, resident=r.value, completed=c.value, evictions=e.value, cold_starts=k.value)
+
+    def close(self):
+        if getattr(self, "_sv", None):
+            lib().ples_mg_server_destroy(self._sv)
+            self._sv = None
 
     def __del__(self):
         try:
