@@ -28,6 +28,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -64,6 +66,100 @@ def time_solves(s, steps, warmup, tol, prepare, timing_fams=None):
     hip.hipDeviceSynchronize()
     dt = time.perf_counter() - t0
     return dt, res, cycles
+
+
+def bench_c4(steps):
+    """64 uneven problems in one batch (graph launches of the whole batch)
+    against the same 64 solved one after another on their own handles."""
+    import ctypes
+
+    from ples_amd import cases as PC
+    from ples_amd import mg
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    probs = PC.c4_problems()
+    tol = 1e-8
+    out = {"problems": len(probs), "cells": int(sum(int(np.prod(p["shape"])) for p in probs)), "tolerance": tol}
+    with mg.Batch(probs, precision="mixed") as b:
+        for p, s in zip(probs, b.solvers):
+            s.case_rhs(p["seed"])
+
+        def run():
+            for s in b.solvers:
+                s.zero_fi()
+            return b.solve(tol, 200)
+
+        run()  # warm-up (graph capture)
+        hip.hipDeviceSynchronize()
+        reps = max(1, steps // 3)
+        t = time.perf_counter()
+        for _ in range(reps):
+            res = run()
+        hip.hipDeviceSynchronize()
+        dt = (time.perf_counter() - t) / reps
+        assert all(r.ok for r in res), res
+        out.update(batch_solves_per_s=len(probs) / dt, batch_ms=1e3 * dt,
+                   cycles_max=max(r.cycles for r in res), cycles_mean=float(np.mean([r.cycles for r in res])))
+        # the same problems one at a time (each handle's own graph-replayed loop)
+        hip.hipDeviceSynchronize()
+        t = time.perf_counter()
+        for s in b.solvers:
+            s.zero_fi()
+            r = s.solve_resident(tol, 200)
+            assert r.ok
+        hip.hipDeviceSynchronize()
+        dts = time.perf_counter() - t
+        out.update(sequential_solves_per_s=len(probs) / dts, sequential_ms=1e3 * dts,
+                   speedup_batch_vs_sequential=dts / dt)
+    return out
+
+
+def bench_c5(n_req=400, n_cases=16, rate=200.0, max_active=16):
+    """Serving stream: Poisson arrivals in real time into the serving loop."""
+    from ples_amd import cases as PC
+    from ples_amd import mg
+
+    cases = PC.c5_cases(n_cases)
+    reqs = PC.c5_requests(n_req, n_cases, rate)
+    lat, warm, cyc = [], [], []
+    with mg.Server(max_active=max_active, warm_cases=16, precision="mixed") as sv:
+        for c in cases:
+            sv.add_case(c["case_id"], c["x"], c["y"], c["z"], c["nlevel"], c["rhs_seed"])
+        # warm-up: one request per case (kernel loading, graph capture), untimed
+        for c in cases[:16]:
+            sv.submit(c["case_id"], 1, 1e-3, 1e-6, 200)
+        sv.drain()
+        t0 = time.perf_counter()
+        nxt, open_t = 0, {}
+        while nxt < len(reqs) or open_t:
+            now = time.perf_counter() - t0
+            while nxt < len(reqs) and reqs[nxt][0] <= now:
+                at, c, eps, seed = reqs[nxt]
+                open_t[sv.submit(c, seed, 1e-3, eps, 200)] = at
+                nxt += 1
+            progressed = sv.step()
+            for tk in list(open_t):
+                r = sv.poll(tk)
+                if r is not None:
+                    res, w = r
+                    assert res.ok, res
+                    lat.append(time.perf_counter() - t0 - open_t.pop(tk))
+                    warm.append(w)
+                    cyc.append(res.cycles)
+            if not progressed and open_t:
+                if sv.pending():
+                    sv.wait()
+            elif not open_t and nxt < len(reqs):
+                time.sleep(max(0.0, min(1e-3, reqs[nxt][0] - (time.perf_counter() - t0))))
+        wall = time.perf_counter() - t0
+        st = sv.stats()
+    lat = np.array(lat)
+    return {"requests": n_req, "rate_per_s": rate, "cases": n_cases, "warm_cases": 16, "max_active": max_active,
+            "resident_bytes": st["used_bytes"],
+            "solves_per_s": n_req / wall, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+            "p99_ms": 1e3 * float(np.percentile(lat, 99)), "mean_ms": 1e3 * float(lat.mean()),
+            "warm_fraction": float(np.mean(warm)), "cycles_mean": float(np.mean(cyc)),
+            "evictions": st["evictions"], "cold_starts": st["cold_starts"], "wall_s": wall}
 
 
 def cpu_baseline(cfg):
@@ -105,6 +201,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
     ap.add_argument("--no-c3", action="store_true", help="skip the 512x256x512 warm-start line")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-c4", action="store_true", help="skip the 64-problem batch")
+    ap.add_argument("--no-c5", action="store_true", help="skip the serving stream")
     ap.add_argument("--no-live-timing", action="store_true",
                     help="no per-kernel HIP events in the timed region (roofline fields null)")
     args = ap.parse_args()
@@ -163,6 +261,15 @@ def main():
                 print(f"[bench] c3 {prec}: {c3[prec]}", file=sys.stderr, flush=True)
         c3["speedup_mixed_vs_fp64"] = c3["mixed"]["solves_per_s"] / c3["fp64"]["solves_per_s"]
 
+    c4 = None
+    if not args.no_c4:
+        c4 = bench_c4(args.steps)
+        print(f"[bench] c4: {c4}", file=sys.stderr, flush=True)
+    c5 = None
+    if not args.no_c5:
+        c5 = bench_c5()
+        print(f"[bench] c5: {c5}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
@@ -204,6 +311,8 @@ def main():
         },
         "cpu_baseline": cpu,
         "c3": c3,
+        "c4_batch": c4,
+        "c5_serving": c5,
     }
     print(json.dumps(line), flush=True)
 
