@@ -86,6 +86,10 @@ typedef struct ples_mg_opts {
                         lanes) and keep the fastest; results are cached per level
                         shape for the process, so equal shapes get equal kernels.
                         0 = fixed rule (default) */
+    int fp32_krylov; /* PLES_MG_MIXED only: 1 = FP64 iterative refinement around
+                        an FP32 GCR(2) preconditioned by the FP32 V-cycle (default);
+                        0 = the FP32 V-cycle directly inside the FP64 GCR(2).
+                        Batched and served solves use the latter. */
 } ples_mg_opts;
 
 typedef struct ples_mg_result {

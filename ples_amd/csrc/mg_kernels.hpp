@@ -626,11 +626,11 @@ __global__ void __launch_bounds__(512) k_yline_staged(Geo G, Met<TM> M, LineF F,
 //   MODE 1: r = b - A p, partial sums of r.r, max|r|, b.b
 //   MODE 2: w = A p, partial sums of w.w, w.wp, r.w, r.wp
 // ------------------------------------------------------------------------
-template <int MODE, class T, class TM, class TR, class TO>
+template <int MODE, class T, class TM, class TR, class TO, class TV = double>
 __global__ void __launch_bounds__(512) k_resid(Geo G, Met<TM> M, const T* __restrict__ p,
                                                const TR* __restrict__ b, TO* __restrict__ out,
-                                               T inv_pot, const double* __restrict__ rvec,
-                                               const double* __restrict__ wp, double* partials,
+                                               T inv_pot, const TV* __restrict__ rvec,
+                                               const TV* __restrict__ wp, double* partials,
                                                int nblocks, int P) {
     Seg q;
     const bool act = seg_of(G, P, q);
@@ -656,12 +656,13 @@ __global__ void __launch_bounds__(512) k_resid(Geo G, Met<TM> M, const T* __rest
                 acc[1] = fmax(acc[1], fabs(r));
                 acc[2] += bb * bb;
             } else {
-                const double wv = (double)ap;
-                out[c] = (TO)wv;
-                const double rv = rvec[c];
+                const TO wo = (TO)ap;
+                out[c] = wo;
+                const double wv = (double)wo;  // the dots see the stored w
+                const double rv = (double)rvec[c];
                 acc[0] += wv * wv;
                 if (wp) {
-                    const double wpv = wp[c];
+                    const double wpv = (double)wp[c];
                     acc[1] += wv * wpv;
                     acc[3] += rv * wpv;
                 }
@@ -756,10 +757,11 @@ __global__ void __launch_bounds__(256) k_fin_apply(const double* __restrict__ pa
 
 // w <- w - beta wp ; z <- z - beta zp ; x += alpha z ; r -= alpha w ;
 // partial sums of r.r and max|r| (interior cells, segmented columns)
-__global__ void __launch_bounds__(512) k_gcr_update(Geo G, double* __restrict__ x, double* __restrict__ r,
-                                                    double* __restrict__ z, double* __restrict__ w,
-                                                    const double* __restrict__ zp,
-                                                    const double* __restrict__ wp, const double* S,
+template <class T>
+__global__ void __launch_bounds__(512) k_gcr_update(Geo G, T* __restrict__ x, T* __restrict__ r,
+                                                    T* __restrict__ z, T* __restrict__ w,
+                                                    const T* __restrict__ zp,
+                                                    const T* __restrict__ wp, const double* S,
                                                     double* partials, int nblocks, int P) {
     if (S[S_DONE] != 0.0) return;  // converged: uniform exit, partials unused
     Seg q;
@@ -770,18 +772,18 @@ __global__ void __launch_bounds__(512) k_gcr_update(Geo G, double* __restrict__ 
         const long cb = G.idx(q.i, q.j0 - 1, q.k);
         for (int j = q.j0; j <= q.j1; ++j) {
             const long c = cb + G.sy * (j - q.j0 + 1);
-            double zv = z[c], wv = w[c];
+            T zv = z[c], wv = w[c];
             if (beta != 0.0) {
-                zv -= beta * zp[c];
-                wv -= beta * wp[c];
+                zv = (T)(zv - beta * zp[c]);
+                wv = (T)(wv - beta * wp[c]);
                 z[c] = zv;
                 w[c] = wv;
             }
-            x[c] += alpha * zv;
-            const double rv = r[c] - alpha * wv;
+            x[c] = (T)(x[c] + alpha * zv);
+            const T rv = (T)(r[c] - alpha * wv);
             r[c] = rv;
-            acc[0] += rv * rv;
-            acc[1] = fmax(acc[1], fabs(rv));
+            acc[0] += (double)rv * (double)rv;
+            acc[1] = fmax(acc[1], fabs((double)rv));
         }
     }
     double v[2] = {acc[0], acc[1]};
@@ -1482,6 +1484,28 @@ __global__ void __launch_bounds__(256) k_perturb(Geo G, double* __restrict__ a, 
     for (int j = 1; j <= G.ny; ++j) {
         const long c = G.idx(i, j, k);
         a[c] += rel * sd * gauss(seed, 7u, (int64_t)c);
+    }
+}
+
+// r32 = s * r64 over the interior (the FP32 inner problem's rhs)
+__global__ void __launch_bounds__(256) k_scale_to32(Geo G, const double* __restrict__ r, double s,
+                                                    float* __restrict__ r32) {
+    int i, k;
+    if (!column_of(G, i, k)) return;
+    for (int j = 1; j <= G.ny; ++j) {
+        const long c = G.idx(i, j, k);
+        r32[c] = (float)(s * r[c]);
+    }
+}
+
+// x += s * e32 over the interior (FP64 iterative-refinement update)
+__global__ void __launch_bounds__(256) k_add_back(Geo G, double* __restrict__ x, double s,
+                                                  const float* __restrict__ e32) {
+    int i, k;
+    if (!column_of(G, i, k)) return;
+    for (int j = 1; j <= G.ny; ++j) {
+        const long c = G.idx(i, j, k);
+        x[c] += s * (double)e32[c];
     }
 }
 

@@ -78,6 +78,16 @@ struct ples_mg_handle {
     std::vector<int> gin[2], gout[2];
     double last_removed = 0.0;
     int inner = 1;                   // V-cycles per refinement step (resolved from opts)
+    // FP32 inner FGCR of the mixed solver (level-0 FP32 vectors, own scalars)
+    bool ir32 = false;
+    float* e32 = nullptr;            // inner correction
+    float* rr32 = nullptr;           // inner residual (the V-cycle's level-0 rhs)
+    float* Z32[2] = {};
+    float* W32[2] = {};
+    double* S2 = nullptr;
+    const float* rhs32 = nullptr;    // set while a V-cycle runs on rr32
+    hipGraphExec_t gexec32[2] = {};
+    std::vector<int> gin32[2], gout32[2];
     // projection-step state (allocated on first use): level-1 face vectors
     // and Jacobian, cell velocities, face fluxes, increments, gradients
     bool pj_ready = false;
@@ -405,6 +415,7 @@ int prolong_to(H* h, Level& Cc, Level& F) {
 
 int lv_smooth(H* h, int n, const double* r0, int nu, bool zero) {
     Level& L = h->lv[n];
+    if (n == 0 && h->rhs32) return smooth<float, float>(h, L, h->rhs32, nu, zero, true);
     if (n == 0) {
         return L.f32 ? smooth<float, double>(h, L, r0, nu, zero, true)
                      : smooth<double, double>(h, L, r0, nu, zero, true);
@@ -415,6 +426,7 @@ int lv_smooth(H* h, int n, const double* r0, int nu, bool zero) {
 
 int lv_resid(H* h, int n, const double* r0) {
     Level& L = h->lv[n];
+    if (n == 0 && h->rhs32) return vresid<float, float>(h, L, h->rhs32, true);
     if (n == 0) return L.f32 ? vresid<float, double>(h, L, r0, true) : vresid<double, double>(h, L, r0, true);
     return L.f32 ? vresid<float, float>(h, L, (const float*)L.b, false)
                  : vresid<double, double>(h, L, (const double*)L.b, false);
@@ -555,7 +567,7 @@ int step_body(H* h, int q) {
                        h->o.krylov);
     {
         KT kt(h, PLES_MG_K_OUTER_UPDATE, N0 * (48 + (kry ? 32 + 16 : 0)));
-        hipLaunchKernelGGL(k_gcr_update, cl.grid, cl.block, 0, h->s, L0.G, h->x, h->rvec, Zq, Wq,
+        hipLaunchKernelGGL(k_gcr_update<double>, cl.grid, cl.block, 0, h->s, L0.G, h->x, h->rvec, Zq, Wq,
                            (const double*)Zp, (const double*)Wp, (const double*)h->S, h->part, cl.nblocks, L0.P);
     }
     hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S);
@@ -616,76 +628,170 @@ int enqueue_step(H* h, int q) {
     return PLES_MG_OK;
 }
 
-// stop test parameters on the device; clears S_DONE and the GCR history
-int arm_stop(H* h, double thr) {
-    h->Sh[S_DONE] = 0.0;
-    h->Sh[S_THR] = thr;
-    h->Sh[S_MODE] = (h->o.stop_mode == PLES_MG_STOP_ABS_MAX) ? 1.0 : 0.0;
-    h->Sh[S_CYC] = 0.0;
-    CK(hipMemcpyAsync(h->S + S_DONE, h->Sh + S_DONE, 4 * sizeof(double), hipMemcpyHostToDevice, h->s));
-    CKS(set_S(h, S_HAVEP, 0.0));
+// stop test parameters of the scalar set Sd on the device; clears S_DONE, the
+// step count and the GCR history.  mode 1: max|r| < thr, 0: r.r < thr.
+int arm_stop(H* h, double* Sd, double thr, int mode) {
+    double* st = h->Sh + (Sd == h->S ? 6 : 7) * S_COUNT;  // pinned staging per scalar set
+    st[0] = 0.0;                    // S_DONE
+    st[1] = thr;                    // S_THR
+    st[2] = mode ? 1.0 : 0.0;       // S_MODE
+    st[3] = 0.0;                    // S_CYC
+    st[4] = 0.0;                    // S_HAVEP
+    CK(hipMemcpyAsync(Sd + S_DONE, st, 4 * sizeof(double), hipMemcpyHostToDevice, h->s));
+    CK(hipMemcpyAsync(Sd + S_HAVEP, st + 4, sizeof(double), hipMemcpyHostToDevice, h->s));
     return PLES_MG_OK;
 }
 
-// The refinement loop.  Steps are enqueued ahead of the residual read-back
-// (a ring of pinned snapshots of S, one event each) while the observed
-// contraction predicts that the step in flight will not yet converge, so the
-// device never idles on the host between steps; near convergence the host
-// waits for each step.  The stop test itself runs on the device (S_DONE),
-// which makes any step enqueued past convergence a no-op for x and r.
-int solve_core(H* h, double eps, int maxcyc, ples_mg_result* res) {
+// One step of the FP32 inner FGCR(2) of the mixed solver on A32 e = r32:
+// z = V(rr32) (FP32 V-cycle), its exact homogeneous ghosts, w = A32 z with
+// the dots, the GCR update of e and rr32.  Scalars in S2, FP64 sums.
+int step32_body(H* h, int q) {
     Level& L0 = h->lv[0];
-    CK(hipEventRecord(h->t0, h->s));
-    CKS(outer_residual(h, h->x, h->rvec));
-    CKS(read_S(h));
-    const double bb = h->Sh[S_BB];
-    const double bnorm = std::sqrt(bb) > 0 ? std::sqrt(bb) : 1.0;
-    const bool absmode = h->o.stop_mode == PLES_MG_STOP_ABS_MAX;
-    auto measure = [&](double rr, double rmax) { return absmode ? rmax : std::sqrt(rr) / bnorm; };
-    const double thr = absmode ? eps : (eps * bnorm) * (eps * bnorm);
-    double rr = h->Sh[S_RR], rmax = h->Sh[S_RMAX];
-    if (!std::isfinite(rr)) return PLES_MG_ERR_NONFINITE;
-    (void)L0;
-    int cycles = 0;
-    int status = PLES_MG_NOT_CONVERGED;
-    bool conv = measure(rr, rmax) < eps;
-    if (conv) status = PLES_MG_OK;
-    CKS(arm_stop(h, thr));
-    int q = 0;
-    int enq = 0, seen = 0;        // steps enqueued / read back since arming
-    double last = measure(rr, rmax), rho = 0.5;
-    const int depth = 2;          // at most this many steps in flight
-    while (!conv) {
-        while (cycles + enq < maxcyc && enq - seen < depth) {
+    const ColLaunch cl = seg_launch(L0.G, L0.P);
+    h->rhs32 = h->rr32;
+    int st = PLES_MG_OK;
+    for (int c = 0; c < h->inner && st == PLES_MG_OK; ++c) st = vcycle(h, nullptr, c == 0);
+    h->rhs32 = nullptr;
+    CKS(st);
+    float* Zq = h->Z32[q];
+    float* Wq = h->W32[q];
+    float* Zp = h->Z32[1 - q];
+    float* Wp = h->W32[1 - q];
+    {
+        const long n = L0.G.n;
+        const int nb = (int)std::min<long>((n + 255) / 256, 8192);
+        hipLaunchKernelGGL((k_convert<float, float>), dim3(nb), dim3(256), 0, h->s, n, (const float*)L0.p[L0.cur],
+                           Zq);
+    }
+    CKS(halo<float>(h, L0, Zq, 0.0, false, std::max(1, h->o.halo_passes)));
+    const double N0 = (double)L0.ncell();
+    const bool kry = h->o.krylov != 0;
+    {
+        KT kt(h, PLES_MG_K_OUTER_APPLY, N0 * (36 + 12 + (kry ? 4 : 0)));
+        hipLaunchKernelGGL((k_resid<2, float, float, float, float, float>), cl.grid, cl.block, 0, h->s, L0.G,
+                           met<float>(L0), (const float*)Zq, (const float*)nullptr, Wq, 1.0f,
+                           (const float*)h->rr32, (const float*)(kry ? Wp : nullptr), h->part, cl.nblocks, L0.P);
+    }
+    hipLaunchKernelGGL(k_fin_apply, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S2,
+                       h->o.krylov);
+    {
+        KT kt(h, PLES_MG_K_OUTER_UPDATE, N0 * (24 + (kry ? 16 + 8 : 0)));
+        hipLaunchKernelGGL(k_gcr_update<float>, cl.grid, cl.block, 0, h->s, L0.G, h->e32, h->rr32, Zq, Wq,
+                           (const float*)Zp, (const float*)Wp, (const double*)h->S2, h->part, cl.nblocks, L0.P);
+    }
+    hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S2);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+void drop_graphs32(H* h) {
+    for (int q = 0; q < 2; ++q) {
+        if (h->gexec32[q]) (void)hipGraphExecDestroy(h->gexec32[q]);
+        h->gexec32[q] = nullptr;
+        h->gin32[q].clear();
+        h->gout32[q].clear();
+    }
+}
+
+int enqueue_step32(H* h, int q) {
+    if (!h->o.graphs || h->timing != 0) return step32_body(h, q);
+    if (h->gexec32[q] && step_state(h) != h->gin32[q]) drop_graphs32(h);
+    if (!h->gexec32[q]) {
+        const std::vector<int> before = step_state(h);
+        h->gin32[q] = before;
+        CK(hipStreamBeginCapture(h->s, hipStreamCaptureModeThreadLocal));
+        const int st = step32_body(h, q);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(h->s, &g);
+        if (st != PLES_MG_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        CK(e);
+        const hipError_t ei = hipGraphInstantiate(&h->gexec32[q], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        CK(ei);
+        h->gout32[q] = step_state(h);
+        set_step_state(h, before);
+    }
+    CK(hipGraphLaunch(h->gexec32[q], h->s));
+    set_step_state(h, h->gout32[q]);
+    return PLES_MG_OK;
+}
+
+// Run refinement steps (kind 0: FP64 GCR step, 1: FP32 inner step, scalars
+// in Sd) until the device stop test holds or maxsteps steps ran.  Steps are
+// enqueued ahead of the residual read-back (a ring of pinned snapshots of Sd,
+// one event each) while the observed contraction predicts that the step in
+// flight will not converge yet, so the device never idles on the host
+// between steps; near convergence the host waits for each step.  A step
+// enqueued past convergence changes nothing (S_DONE).  meas(rr, rmax) is the
+// host's measure of the stop test, eps its threshold.
+struct StepRun {
+    int applied = 0;
+    bool done = false;
+    bool nonfinite = false;
+    double rr = 0.0, rmax = 0.0;
+};
+
+template <class Meas>
+int run_steps(H* h, int kind, double* Sd, double start, double eps, int maxsteps, Meas meas, StepRun& out) {
+    int q = 0, enq = 0, seen = 0;
+    double last = start, rho = 0.5;
+    const int depth = 2;
+    out = StepRun{};
+    while (true) {
+        while (enq < maxsteps && enq - seen < depth) {
             if (enq > seen && last * std::pow(rho, (double)(enq - seen)) < 4.0 * eps) break;
-            CKS(enqueue_step(h, q));
+            CKS(kind == 0 ? enqueue_step(h, q) : enqueue_step32(h, q));
             q = 1 - q;
             double* slot = h->Sh + 2 * S_COUNT + (enq % 4) * S_COUNT;
-            CK(hipMemcpyAsync(slot, h->S, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, h->s));
+            CK(hipMemcpyAsync(slot, Sd, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, h->s));
             CK(hipEventRecord(h->sev[enq % 4], h->s));
             ++enq;
         }
-        if (seen == enq) break;  // cycle cap reached
+        if (seen == enq) break;
         CK(hipEventSynchronize(h->sev[seen % 4]));
         const double* slot = h->Sh + 2 * S_COUNT + (seen % 4) * S_COUNT;
         ++seen;
-        const double r2 = slot[S_RR], rm = slot[S_RMAX];
-        if (!std::isfinite(r2)) {
-            status = PLES_MG_ERR_NONFINITE;
-            cycles += seen;
+        out.rr = slot[S_RR];
+        out.rmax = slot[S_RMAX];
+        if (!std::isfinite(out.rr)) {
+            out.nonfinite = true;
+            out.applied = seen;
             break;
         }
-        if (slot[S_DONE] == 0.0) {
-            const double m = measure(r2, rm);
-            if (last > 0.0 && m > 0.0) rho = std::min(0.95, std::max(1e-3, m / last));
-            last = m;
-            continue;
+        if (slot[S_DONE] != 0.0) {
+            out.done = true;
+            out.applied = (int)slot[S_CYC];
+            break;
         }
-        // converged at step `seen`; later steps in flight change nothing
-        cycles += seen;
-        CK(hipStreamSynchronize(h->s));
-        rr = r2;
-        rmax = rm;
+        const double m = meas(out.rr, out.rmax);
+        if (last > 0.0 && m > 0.0) rho = std::min(0.95, std::max(1e-3, m / last));
+        last = m;
+        out.applied = seen;
+    }
+    CK(hipStreamSynchronize(h->s));  // steps in flight past convergence are no-ops
+    return PLES_MG_OK;
+}
+
+// FP64 multigrid-preconditioned GCR(2) (both precisions' V-cycles).
+int solve_gcr64(H* h, double eps, int maxcyc, int& cycles, double& rr, double& rmax, double bnorm) {
+    const bool absmode = h->o.stop_mode == PLES_MG_STOP_ABS_MAX;
+    auto measure = [&](double r2, double rm) { return absmode ? rm : std::sqrt(r2) / bnorm; };
+    const double thr = absmode ? eps : (eps * bnorm) * (eps * bnorm);
+    int status = PLES_MG_NOT_CONVERGED;
+    bool conv = measure(rr, rmax) < eps;
+    if (conv) return PLES_MG_OK;
+    CKS(arm_stop(h, h->S, thr, absmode));
+    while (!conv && cycles < maxcyc) {
+        StepRun run;
+        CKS(run_steps(h, 0, h->S, measure(rr, rmax), eps, maxcyc - cycles, measure, run));
+        cycles += run.applied;
+        rr = run.rr;
+        rmax = run.rmax;
+        if (run.nonfinite) return PLES_MG_ERR_NONFINITE;
+        if (!run.done) break;
         // confirm with the true residual; restart the recurrence if it drifted
         CKS(outer_residual(h, h->x, h->rvec));
         CKS(read_S(h));
@@ -695,20 +801,70 @@ int solve_core(H* h, double eps, int maxcyc, ples_mg_result* res) {
             conv = true;
             status = PLES_MG_OK;
         } else {
-            CKS(arm_stop(h, thr));
-            enq = seen = 0;
-            last = measure(rr, rmax);
+            CKS(arm_stop(h, h->S, thr, absmode));
         }
     }
-    if (!conv && status != PLES_MG_ERR_NONFINITE) {
-        CK(hipStreamSynchronize(h->s));
-        if (seen > 0) {
-            const double* slot = h->Sh + 2 * S_COUNT + ((seen - 1) % 4) * S_COUNT;
-            rr = slot[S_RR];
-            rmax = slot[S_RMAX];
+    return status;
+}
+
+// Mixed precision: FP64 iterative refinement around an FP32 FGCR(2)
+// preconditioned by FP32 V-cycles.  Each refinement takes the FP64 residual
+// r = b - A x (FP64 metrics), solves A32 e = r/|r| in FP32 to a relative
+// tolerance the outer test needs (at most 1e-5: FP32 precision), and adds
+// x += |r| e in FP64.  The coarsest level stays FP64 inside the V-cycle.
+int solve_ir32(H* h, double eps, int maxcyc, int& cycles, double& rr, double& rmax, double bnorm) {
+    const bool absmode = h->o.stop_mode == PLES_MG_STOP_ABS_MAX;
+    auto measure = [&](double r2, double rm) { return absmode ? rm : std::sqrt(r2) / bnorm; };
+    const Level& L0 = h->lv[0];
+    const ColLaunch cc = col_launch(L0.G);
+    int stall = 0;
+    while (cycles < maxcyc) {
+        const double m = measure(rr, rmax);
+        if (m < eps) return PLES_MG_OK;
+        const double rn = std::sqrt(rr);
+        if (!(rn > 0.0)) return PLES_MG_OK;
+        hipLaunchKernelGGL(k_scale_to32, cc.grid, cc.block, 0, h->s, L0.G, (const double*)h->rvec, 1.0 / rn,
+                           h->rr32);
+        CK(hipMemsetAsync(h->e32, 0, (size_t)L0.G.n * sizeof(float), h->s));
+        CK(hipGetLastError());
+        // relative reduction the outer test still needs (L2 terms), FP32-bounded
+        const double need = absmode ? eps / std::max(rmax, 1e-300) : eps * bnorm / rn;
+        const double tol = std::min(0.1, std::max(1e-5, 0.5 * need));
+        CKS(arm_stop(h, h->S2, tol * tol, 0));
+        auto rel = [&](double r2, double) { return std::sqrt(r2); };
+        StepRun run;
+        CKS(run_steps(h, 1, h->S2, 1.0, tol, std::min(40, maxcyc - cycles), rel, run));
+        cycles += run.applied;
+        if (run.nonfinite) return PLES_MG_ERR_NONFINITE;
+        hipLaunchKernelGGL(k_add_back, cc.grid, cc.block, 0, h->s, L0.G, h->x, rn, (const float*)h->e32);
+        CK(hipGetLastError());
+        CKS(outer_residual(h, h->x, h->rvec));
+        CKS(read_S(h));
+        const double m_old = m;
+        rr = h->Sh[S_RR];
+        rmax = h->Sh[S_RMAX];
+        if (!std::isfinite(rr)) return PLES_MG_ERR_NONFINITE;
+        if (measure(rr, rmax) > 0.5 * m_old) {
+            if (++stall >= 3) break;  // FP32 cannot reduce it further
+        } else {
+            stall = 0;
         }
-        cycles += seen;
     }
+    return measure(rr, rmax) < eps ? PLES_MG_OK : PLES_MG_NOT_CONVERGED;
+}
+
+int solve_core(H* h, double eps, int maxcyc, ples_mg_result* res) {
+    CK(hipEventRecord(h->t0, h->s));
+    CKS(outer_residual(h, h->x, h->rvec));
+    CKS(read_S(h));
+    const double bb = h->Sh[S_BB];
+    const double bnorm = std::sqrt(bb) > 0 ? std::sqrt(bb) : 1.0;
+    double rr = h->Sh[S_RR], rmax = h->Sh[S_RMAX];
+    if (!std::isfinite(rr)) return PLES_MG_ERR_NONFINITE;
+    int cycles = 0;
+    int status = h->ir32 ? solve_ir32(h, eps, maxcyc, cycles, rr, rmax, bnorm)
+                         : solve_gcr64(h, eps, maxcyc, cycles, rr, rmax, bnorm);
+    if (status != PLES_MG_OK && status != PLES_MG_NOT_CONVERGED && status != PLES_MG_ERR_NONFINITE) return status;
     // leave the iterate with exact ghosts (the reference returns fi with ghosts)
     CKS(halo64(h, h->x, true));
     CK(hipEventRecord(h->t1, h->s));
@@ -929,6 +1085,7 @@ size_t estimate(const ples_mg_grid* g, const ples_mg_opts* o) {
     }
     const size_t c0 = (size_t)(g->jx + 2) * (g->jy + 2) * (g->jz + 2);
     tot += c0 * 8 * 7;                                             // x, b, r, Z[2], W[2]
+    if (mixed && g->nlevel >= 2) tot += c0 * 4 * 6;                // FP32 inner e, r, Z[2], W[2]
     tot += (size_t)(g->jx + 1) * (g->jy + 1) * (g->jz + 1) * 24;   // nodes
     return tot;
 }
@@ -962,6 +1119,7 @@ void ples_mg_default_opts(ples_mg_opts* o) {
     o->graphs = 1;
     o->inner_cycles = 0;  // auto
     o->autotune = 0;
+    o->fp32_krylov = 1;
 }
 
 const char* ples_mg_status_string(int s) {
@@ -994,6 +1152,7 @@ int ples_mg_destroy(ples_mg_handle* h) {
     }
     for (auto& a : h->allocs) (void)hipFree(a.first);
     drop_graphs(h);
+    drop_graphs32(h);
     for (int q = 0; q < 4; ++q)
         if (h->sev[q]) (void)hipEventDestroy(h->sev[q]);
     if (h->Sh) (void)hipHostFree(h->Sh);
@@ -1021,9 +1180,10 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
     CK(hipEventCreate(&h->t0));
     CK(hipEventCreate(&h->t1));
     // pinned S mirror: [0, S_COUNT) scalars, then 4 divergence maxima, then
-    // the 4-slot read-back ring of the refinement loop
-    CK(hipHostMalloc((void**)&h->Sh, 6 * S_COUNT * sizeof(double), hipHostMallocDefault));
-    std::memset(h->Sh, 0, 6 * S_COUNT * sizeof(double));
+    // the 4-slot read-back ring of the refinement loop, then the arming
+    // staging of the two scalar sets
+    CK(hipHostMalloc((void**)&h->Sh, 8 * S_COUNT * sizeof(double), hipHostMallocDefault));
+    std::memset(h->Sh, 0, 8 * S_COUNT * sizeof(double));
     for (int q = 0; q < 4; ++q) CK(hipEventCreateWithFlags(&h->sev[q], hipEventDisableTiming));
     h->jx = g->jx; h->jy = g->jy; h->jz = g->jz;
     h->ip = g->ip; h->jp = g->jp; h->kp = g->kp;
@@ -1096,6 +1256,16 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
     for (int q = 0; q < 2; ++q) {
         CKS(dalloc(h, (void**)&h->Z[q], c0 * 8));
         CKS(dalloc(h, (void**)&h->W[q], c0 * 8));
+    }
+    h->ir32 = mixed && h->L >= 2 && o.fp32_krylov != 0;
+    if (h->ir32) {
+        CKS(dalloc(h, (void**)&h->e32, c0 * 4));
+        CKS(dalloc(h, (void**)&h->rr32, c0 * 4));
+        for (int q = 0; q < 2; ++q) {
+            CKS(dalloc(h, (void**)&h->Z32[q], c0 * 4));
+            CKS(dalloc(h, (void**)&h->W32[q], c0 * 4));
+        }
+        CKS(dalloc(h, (void**)&h->S2, S_COUNT * sizeof(double)));
     }
     const size_t csz[6] = {(size_t)g->jy * g->jz, (size_t)g->jy * g->jz, (size_t)g->jx * g->jz,
                            (size_t)g->jx * g->jz, (size_t)g->jx * g->jy, (size_t)g->jx * g->jy};
@@ -2004,7 +2174,7 @@ int ples_mg_batch_solve(ples_mg_batch* b, double eps, int maxcycles, int steps_p
             done[i] = 1;
         }
         last[i] = absmode ? rmax[i] : std::sqrt(rr[i]) / bnorm[i];
-        CKS(arm_stop(h, thr[i]));
+        CKS(arm_stop(h, h->S, thr[i], h->o.stop_mode == PLES_MG_STOP_ABS_MAX));
         if (!done[i] && last[i] < eps) {
             status[i] = PLES_MG_OK;
             done[i] = 1;
@@ -2090,7 +2260,7 @@ int ples_mg_batch_solve(ples_mg_batch* b, double eps, int maxcycles, int steps_p
                 done[i] = 1;
             } else if (cycles[i] < maxcycles) {
                 // the recurrence drifted from the true residual: restart it
-                CKS(arm_stop(h, thr[i]));
+                CKS(arm_stop(h, h->S, thr[i], h->o.stop_mode == PLES_MG_STOP_ABS_MAX));
                 done[i] = 0;
                 last[i] = measure(i, rr[i], rmax[i]);
                 restart = true;
@@ -2354,7 +2524,7 @@ int advance(SV* sv, SReq& r) {
             finish(sv, r, PLES_MG_NOT_CONVERGED, rr, rmax);
             return 1;
         }
-        CKS(arm_stop(h, r.thr));
+        CKS(arm_stop(h, h->S, r.thr, absmode));
         r.base = r.cycles;
         r.enq = r.seen = 0;
         r.q = 0;

@@ -19,7 +19,8 @@ def _problems():
 
 
 def _alone(pr, precision, tol):
-    with mg.Solver(pr["x"], pr["y"], pr["z"], 0, 1, 0, pr["nlevel"], precision=precision) as s:
+    # batches run the FP32 V-cycle directly inside the FP64 GCR (fp32_krylov = 0)
+    with mg.Solver(pr["x"], pr["y"], pr["z"], 0, 1, 0, pr["nlevel"], precision=precision, fp32_krylov=0) as s:
         s.case_rhs(pr["seed"])
         s.zero_fi()
         r = s.solve_resident(tol, 200)
