@@ -3,10 +3,12 @@
 Workload (default): the terrain-following 256x128x256 grid (tanh 1.5
 stretching, terrain amplitude 0.1 H), 6 multigrid levels, seeded Gaussian
 divergence right-hand side, solved from zero to ||b - A p||/||b|| < 1e-10.
-One step = one complete solve.  `value` is the mixed-precision solver (FP32
-V-cycles inside the FP64 GCR refinement loop); the FP64 multigrid is timed on
-the same steps and reported beside it.  The 512x256x512 warm-start solve is
-reported in the same line under "c3".
+One step = one complete solve.  `value` is the mixed-precision solver (FP64
+iterative refinement with FP64 residuals around an FP32 GCR preconditioned by
+FP32 V-cycles); the FP64 multigrid is timed on the same steps and reported
+beside it.  Both use 3/3 sweeps and autotuned launch shapes.  The
+512x256x512 warm-start solve is reported under "c3", the 64-problem batch
+under "c4_batch" and the Poisson-arrival serving stream under "c5_serving".
 
 The roofline entry is the fine-level eta-line smoother of the timed mixed
 solves: algorithmic bytes per launch (the library's per-launch byte count)
@@ -37,9 +39,12 @@ HBM_PEAK_GBS = 8000.0  # MI355X, AMD published
 
 
 def solver_for(cfg, precision, **kw):
+    """Both precisions with the library defaults (3/3 sweeps) and the launch
+    shapes chosen by the autotuner."""
     from ples_amd import mg
 
     x, y, z = cfg.nodes()
+    kw.setdefault("autotune", 1)
     return mg.Solver(x, y, z, 0, 1, 0, cfg.nlevel, precision=precision, **kw)
 
 
@@ -66,6 +71,19 @@ def time_solves(s, steps, warmup, tol, prepare, timing_fams=None):
     hip.hipDeviceSynchronize()
     dt = time.perf_counter() - t0
     return dt, res, cycles
+
+
+def traffic_per_launch():
+    """HBM bytes per launch of the fine-level mixed smoother at the default
+    workload, from the committed rocprofv3 PMC summary (FETCH_SIZE scaled by
+    the copy-kernel factor, plus WRITE_SIZE), or None without one."""
+    path = os.path.join(ROOT, "profiles", "pmc_smooth_fine_c2_256x128x256.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def bench_c4(steps):
@@ -214,6 +232,11 @@ def main():
     from ples_amd import cases as PC
 
     cfg = PC.CONFIGS[args.config]
+    from ples_amd import mg
+
+    tuned = mg.tuning_load()  # committed launch-shape table of this GPU (else tuned live)
+    copy_gbs = mg.copy_bandwidth(1 << 30, 20)
+    print(f"[bench] measured copy bandwidth {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
 
     def from_zero(s):
         s.zero_fi()
@@ -291,6 +314,7 @@ def main():
         "dtype": "fp32+fp64",
         "data": "synthetic (seeded Gaussian divergence rhs generated on device)",
         "config": {"workload": cfg.text, "cells": cfg.cells, "levels": cfg.nlevel, "tolerance": cfg.tol,
+                   "launch_shapes": "tuning table" if tuned else "tuned at create",
                    "precision": "FP32 V-cycles, FP64 GCR refinement and outer residual"},
         "fp64_solves_per_s": args.steps / fp64["seconds"],
         "speedup_mixed_vs_fp64": fp64["seconds"] / mixed["seconds"],
@@ -303,7 +327,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
-            "traffic": None,
+            "traffic": traffic_per_launch(),
+            "copy_measured_GBs": copy_gbs,
+            "frac_of_copy": achieved / copy_gbs if achieved else None,
             "launches": n,
             "avg_us": 1e3 * avg_ms,
             "bytes_per_launch": by / max(n, 1),

@@ -66,8 +66,8 @@ typedef struct ples_mg_grid {
 
 typedef struct ples_mg_opts {
     int precision;   /* PLES_MG_FP64 or PLES_MG_MIXED */
-    int nu_pre;      /* smoothing sweeps per level on the way down (default 2) */
-    int nu_post;     /* ... on the way up (default 2) */
+    int nu_pre;      /* smoothing sweeps per level on the way down (default 3) */
+    int nu_post;     /* ... on the way up (default 3) */
     int nu_coarse;   /* sweeps on the coarsest level (default 16) */
     double omega;    /* damping of the eta-line Jacobi smoother (default 0.8) */
     int krylov;      /* 1 = GCR acceleration of the refinement loop (default), 0 = plain */
@@ -290,6 +290,16 @@ int ples_mg_server_pending(const ples_mg_server* sv);
 int ples_mg_server_get_fi(ples_mg_server* sv, uint64_t case_id, double* fi);
 int ples_mg_server_stats(const ples_mg_server* sv, size_t* used_bytes, int* resident,
                          long* completed, long* evictions, long* cold_starts);
+
+/* Autotuner table (process-wide): load entries from a text file (one level
+ * shape per line: nx ny nz ip kp f32 fine P rP rbx), so handles created with
+ * opts.autotune take those shapes without timing; save the current table. */
+int ples_mg_tuning_load(const char* path);
+int ples_mg_tuning_save(const char* path);
+
+/* Measured copy bandwidth of device `device` (GB/s, read + write bytes) for a
+ * 16-byte-per-lane copy of `bytes` bytes repeated reps times. */
+int ples_mg_copy_bandwidth(int device, size_t bytes, int reps, double* gbs);
 
 /* Launch shape of level `level` (1-based): smoother segments per column P,
  * V-cycle residual segments rP x lanes rbx (outputs may be NULL). */

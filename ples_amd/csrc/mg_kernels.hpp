@@ -1509,6 +1509,13 @@ __global__ void __launch_bounds__(256) k_add_back(Geo G, double* __restrict__ x,
     }
 }
 
+// plain 16-byte copy (bandwidth probe)
+__global__ void __launch_bounds__(256) k_copy16(long n, const float4* __restrict__ a, float4* __restrict__ b) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) b[t] = a[t];
+}
+
 // elementwise helpers over the full ghosted array
 template <class TI, class TO>
 __global__ void k_convert(long n, const TI* __restrict__ a, TO* __restrict__ b) {

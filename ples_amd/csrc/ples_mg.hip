@@ -1021,15 +1021,20 @@ int tune_level(H* h, int n) {
         return level_factors(h, L);
     }
     const double* r0 = h->rvec;  // level-0 rhs (any content: timing only)
-    TuneVal best{L.P, 0, 0};
+    // the fixed rule's shape is the incumbent; another shape must be at least
+    // 3% faster to replace it, which keeps choices stable between runs
+    const int P0 = L.P;
+    TuneVal best{P0, 0, 0};
     double tbest = 1e30;
-    for (int P : seg_candidates(L.G.ny)) {
+    std::vector<int> cands = seg_candidates(L.G.ny);
+    std::stable_partition(cands.begin(), cands.end(), [&](int P) { return P == P0; });
+    for (int P : cands) {
         if (P > 1) CKS(ensure_vw());
         L.P = P;
         CKS(level_factors(h, L));
         double ms = 0;
         CKS(time_launches(h, 3, ms, [&] { return lv_smooth(h, n, r0, 2, false); }));
-        if (ms < tbest) {
+        if (ms < 0.97 * tbest || tbest == 1e30) {
             tbest = ms;
             best.P = P;
         }
@@ -1037,19 +1042,22 @@ int tune_level(H* h, int n) {
     L.P = best.P;
     CKS(level_factors(h, L));
     tbest = 1e30;
+    const ColLaunch c0 = seg_launch(L.G, L.P);
+    std::vector<std::pair<int, int>> rc{{L.P, (int)c0.block.x}};
     for (int P : seg_candidates(L.G.ny))
-        for (int bx = 32; bx <= 256; bx *= 2) {
-            if (bx * P > 512 || bx > 2 * L.G.nx) continue;
-            L.rP = P;
-            L.rbx = bx;
-            double ms = 0;
-            CKS(time_launches(h, 3, ms, [&] { return lv_resid(h, n, r0); }));
-            if (ms < tbest) {
-                tbest = ms;
-                best.rP = P;
-                best.rbx = bx;
-            }
+        for (int bx = 32; bx <= 256; bx *= 2)
+            if (bx * P <= 512 && bx <= 2 * L.G.nx && !(P == L.P && bx == (int)c0.block.x)) rc.push_back({P, bx});
+    for (auto [P, bx] : rc) {
+        L.rP = P;
+        L.rbx = bx;
+        double ms = 0;
+        CKS(time_launches(h, 3, ms, [&] { return lv_resid(h, n, r0); }));
+        if (ms < 0.97 * tbest || tbest == 1e30) {
+            tbest = ms;
+            best.rP = P;
+            best.rbx = bx;
         }
+    }
     L.rP = best.rP;
     L.rbx = best.rbx;
     g_tune[key] = best;
@@ -1107,8 +1115,8 @@ extern "C" {
 void ples_mg_default_opts(ples_mg_opts* o) {
     if (!o) return;
     o->precision = PLES_MG_FP64;
-    o->nu_pre = 2;
-    o->nu_post = 2;
+    o->nu_pre = 3;
+    o->nu_post = 3;
     o->nu_coarse = 16;
     o->omega = 0.8;
     o->krylov = 1;
@@ -2736,6 +2744,89 @@ int ples_mg_server_stats(const ples_mg_server* sv, size_t* used_bytes, int* resi
     if (completed) *completed = sv->completed;
     if (evictions) *evictions = sv->evictions;
     if (cold_starts) *cold_starts = sv->cold_starts;
+    return PLES_MG_OK;
+}
+
+
+// Copy bandwidth of this device: a 16-byte-per-lane copy of `bytes` bytes,
+// reps launches timed with HIP events.  The measured streaming ceiling that
+// the stencil kernels' achieved bandwidth is read against.
+int ples_mg_copy_bandwidth(int device, size_t bytes, int reps, double* gbs) {
+    if (!gbs || bytes < 4096 || reps < 1) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(device));
+    const size_t n4 = bytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    CK(hipMalloc((void**)&a, n4 * 16));
+    if (hipMalloc((void**)&b, n4 * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return PLES_MG_ERR_OOM;
+    }
+    hipStream_t s;
+    hipEvent_t e0, e1;
+    int st = PLES_MG_OK;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) st = PLES_MG_ERR_HIP;
+    if (st == PLES_MG_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        st = PLES_MG_ERR_HIP;
+    if (st == PLES_MG_OK) {
+        (void)hipMemsetAsync(a, 0, n4 * 16, s);
+        const int nb = 32768;
+        hipLaunchKernelGGL(k_copy16, dim3(nb), dim3(256), 0, s, (long)n4, (const float4*)a, b);
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy16, dim3(nb), dim3(256), 0, s, (long)n4, (const float4*)a, b);
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) st = PLES_MG_ERR_HIP;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        *gbs = 2.0 * (double)(n4 * 16) * reps / (ms * 1e-3) / 1e9;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+    }
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return st;
+}
+
+
+// ---- tuning table: one line per level shape
+//   nx ny nz ip kp f32 fine P rP rbx
+int ples_mg_tuning_load(const char* path) {
+    if (!path) return PLES_MG_ERR_ARG;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return PLES_MG_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    char line[256];
+    int n = 0;
+    while (std::fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        std::array<int, 7> k{};
+        TuneVal v{};
+        if (std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d", &k[0], &k[1], &k[2], &k[3], &k[4], &k[5], &k[6],
+                        &v.P, &v.rP, &v.rbx) != 10)
+            continue;
+        const bool okP = v.P >= 1 && v.P <= 16 && (v.P & (v.P - 1)) == 0 && k[1] % v.P == 0;
+        const bool okR = v.rP >= 1 && v.rP <= 16 && (v.rP & (v.rP - 1)) == 0 && k[1] % v.rP == 0 && v.rbx >= 1 && (v.rbx & (v.rbx - 1)) == 0 &&
+                         v.rbx <= 256 && v.rbx * v.rP <= 512;
+        if (!okP || !okR) continue;
+        g_tune[k] = v;
+        ++n;
+    }
+    std::fclose(f);
+    return n > 0 ? PLES_MG_OK : PLES_MG_ERR_ARG;
+}
+
+int ples_mg_tuning_save(const char* path) {
+    if (!path) return PLES_MG_ERR_ARG;
+    FILE* f = std::fopen(path, "w");
+    if (!f) return PLES_MG_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    std::fprintf(f, "# nx ny nz ip kp f32 fine  P rP rbx  (launch shapes chosen by the autotuner)\n");
+    for (const auto& kv : g_tune) {
+        const auto& k = kv.first;
+        std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d\n", k[0], k[1], k[2], k[3], k[4], k[5], k[6], kv.second.P,
+                     kv.second.rP, kv.second.rbx);
+    }
+    std::fclose(f);
     return PLES_MG_OK;
 }
 

@@ -130,6 +130,9 @@ def lib():
         L.ples_mg_batch_device_bytes.restype = C.c_size_t
         L.ples_mg_batch_solve.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.POINTER(_Result)]
         L.ples_mg_tiles.argtypes = [C.c_void_p, C.c_int] + [C.POINTER(C.c_int)] * 3
+        L.ples_mg_copy_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
+        L.ples_mg_tuning_load.argtypes = [C.c_char_p]
+        L.ples_mg_tuning_save.argtypes = [C.c_char_p]
         L.ples_mg_server_default_opts.argtypes = [C.POINTER(_ServerOpts)]
         L.ples_mg_server_create.argtypes = [C.POINTER(_ServerOpts), C.POINTER(_Opts), C.POINTER(C.c_void_p)]
         L.ples_mg_server_destroy.argtypes = [C.c_void_p]
@@ -203,6 +206,27 @@ def _grid(x, y, z, ip, jp, kp, nlevel, metrics=None):
         for q in range(9):
             g.g[q] = _p(metrics[q])
     return g
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=20, device=0) -> float:
+    """Measured device copy bandwidth in GB/s (read + write)."""
+    g = C.c_double()
+    _check(lib().ples_mg_copy_bandwidth(device, nbytes, reps, C.byref(g)), "ples_mg_copy_bandwidth")
+    return g.value
+
+
+TUNING_TABLE = os.path.join(_HERE, "tuning", "mi355x.txt")
+
+
+def tuning_load(path=TUNING_TABLE) -> bool:
+    """Load a launch-shape table for autotune=1 handles; False if absent."""
+    if not os.path.exists(path):
+        return False
+    return lib().ples_mg_tuning_load(path.encode()) == OK
+
+
+def tuning_save(path):
+    _check(lib().ples_mg_tuning_save(path.encode()), "ples_mg_tuning_save")
 
 
 def estimate_bytes(x, y, z, ip=0, jp=1, kp=0, nlevel=4, precision="fp64", **kw) -> int:

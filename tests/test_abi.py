@@ -32,4 +32,19 @@ def test_status_strings_and_defaults():
     assert mg.status_string(0) == "ok"
     assert "dimensions" in mg.status_string(4)
     o = mg.default_opts()
-    assert o["precision"] == mg.FP64 and o["krylov"] == 1 and o["nu_pre"] == 2
+    assert o["precision"] == mg.FP64 and o["krylov"] == 1 and o["nu_pre"] == 3 and o["nu_post"] == 3
+    assert o["graphs"] == 1 and o["fp32_krylov"] == 1 and o["autotune"] == 0
+
+
+def test_tuning_table_parses():
+    """The committed launch-shape table loads (no GPU needed) and every
+    entry is a valid shape: P, rP powers of two dividing ny, rbx*rP <= 512."""
+    assert mg.tuning_load()
+    n = 0
+    for line in open(mg.TUNING_TABLE):
+        if line.startswith("#"):
+            continue
+        nx, ny, nz, ip, kp, f32, fine, P, rP, rbx = map(int, line.split())
+        assert ny % P == 0 and ny % rP == 0 and rbx * rP <= 512
+        n += 1
+    assert n > 20

@@ -193,3 +193,27 @@ def test_warm_start_resident_and_determinism():
         p3 = s.get_fi()
     assert r3.cycles == r1.cycles
     assert np.array_equal(p1, p3)  # bitwise deterministic
+
+
+def test_tuned_launch_shapes_same_solution():
+    """Launch shapes from the tuning table change the smoother's segments and
+    the residual's blocks, not the solution: tuned and fixed-rule solves
+    agree to the parity bar, and the tuned handle reports the table's shapes."""
+    assert mg.tuning_load()
+    g = cases.grid(64, 32, 64, gamma=1.5, amp=0.2)
+    out = []
+    for tune in (0, 1):
+        with _solver(g, 4, "mixed", autotune=tune) as s:
+            s.case_rhs(77)
+            s.zero_fi()
+            r = s.solve_resident(1e-11, 200)
+            assert r.ok
+            out.append(s.get_fi())
+            if tune:
+                table = {}
+                for line in open(mg.TUNING_TABLE):
+                    if not line.startswith("#"):
+                        v = list(map(int, line.split()))
+                        table[tuple(v[:7])] = tuple(v[7:])
+                assert s.tiles(1) == table[(64, 32, 64, 0, 0, 1, 1)]
+    assert cases.rel_l2_meanfree(out[1], out[0]) < 1e-9
