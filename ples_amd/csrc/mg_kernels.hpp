@@ -444,32 +444,69 @@ __global__ void __launch_bounds__(256) k_coarse_lds(Geo G, Met<T> M, LineF F, T*
 // known at compile time): forward pass fused with the 19-point residual,
 // local backward substitution in registers, reduced 2x2 block system of the
 // column in LDS, final pass adds the neighbour responses and writes once.
+// FP32 variants whose coefficients fit 32 KB of LDS factor the segment's
+// tridiagonal block on the fly from the metrics the stencil loads anyway
+// (diagonal = s^2 * sum of the non-wall normal face metrics, off-diagonals
+// -s^2 g22), keeping c'_j in LDS for the backward pass; the others read the
+// precomputed factors (invm, cpr).
+template <int P, int MR, class T>
+struct YlineOtf {
+    static constexpr int BX = SegBX<P>::value;
+    static constexpr bool value = sizeof(T) == 4 && MR * P * BX * 4 <= 32768;
+};
+
 template <int P, int MR, class T, class TM, class TR>
-__global__ void __launch_bounds__(512) k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin,
-                                               T* __restrict__ pout, const TR* __restrict__ b, T omega,
-                                               T inv_pot, int zero_in) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin, T* __restrict__ pout, const TR* __restrict__ b,
+        T omega, T inv_pot, int zero_in) {
     constexpr int BX = SegBX<P>::value;
+    constexpr bool OTF = YlineOtf<P, MR, T>::value;
     __shared__ T red[8][P][BX];
+    __shared__ T cps[OTF ? MR : 1][P][BX];
     Seg q;
     const bool act = seg_of(G, P, q);
     const int tx = threadIdx.x, s = q.s;
     const long sy = G.sy, sz = G.sz;
     const T s2 = inv_pot * inv_pot;
     const long cb = act ? G.idx(q.i, q.j0 - 1, q.k) : 0;  // row j0-1
+    // wall faces are not part of the line operator's diagonal
+    const T mxm = (G.ip == 1 && q.i == 1) ? (T)0 : (T)1, mxp = (G.ip == 1 && q.i == G.nx) ? (T)0 : (T)1;
+    const T mzm = (G.kp == 1 && q.k == 1) ? (T)0 : (T)1, mzp = (G.kp == 1 && q.k == G.nz) ? (T)0 : (T)1;
+    const T myf = (q.j0 == 1) ? (T)0 : (T)1, myl = (q.j1 == G.ny) ? (T)0 : (T)1;
     T e[MR];
     T yF = 0, yL = 0, vF = 0, vL = 0, wF = 0, wL = 0;
     if (act) {
-        T dprev = 0, pfirst = 0, plast = 0;
+        T dprev = 0, pfirst = 0, plast = 0, cprev = 0;
+        // factor of row jj: 1/m and c'; g22m = g22(j-1), g22p = g22(j)
+        auto factor = [&](int jj, long c, T g22m, T g22p, T& inv) {
+            const T den = s2 * (mxp * (T)M.g[0][c] + mxm * (T)M.g[0][c - 1] + mzp * (T)M.g[8][c] +
+                                mzm * (T)M.g[8][c - sz] + ((jj == MR - 1) ? myl : (T)1) * g22p +
+                                ((jj == 0) ? myf : (T)1) * g22m);
+            const T a = (jj > 0) ? -s2 * g22m : (T)0;
+            const T cc = (jj < MR - 1) ? -s2 * g22p : (T)0;
+            inv = (T)1 / (den - a * cprev);
+            const T cp = cc * inv;
+            cps[jj][s][tx] = cp;
+            cprev = cp;
+            return a;
+        };
         if (zero_in) {
             T g22m = (T)M.g[4][cb];
 #pragma unroll
             for (int jj = 0; jj < MR; ++jj) {
                 const long c = cb + sy * (jj + 1);
-                const T a = (jj > 0) ? -s2 * g22m : (T)0;
-                const T d = (-(T)b[c] - a * dprev) * (T)F.invm[c];
+                const T g22p = (T)M.g[4][c];
+                T a, inv;
+                if constexpr (OTF) {
+                    a = factor(jj, c, g22m, g22p, inv);
+                } else {
+                    a = (jj > 0) ? -s2 * g22m : (T)0;
+                    inv = (T)F.invm[c];
+                }
+                const T d = (-(T)b[c] - a * dprev) * inv;
                 e[jj] = omega * d;
                 dprev = d;
-                g22m = (T)M.g[4][c];
+                g22m = g22p;
             }
         } else {
             Win<T> w;
@@ -482,9 +519,17 @@ __global__ void __launch_bounds__(512) k_yline(Geo G, Met<TM> M, LineF F, const 
                 win_load(w, pin, c, sy, sz);
                 const G2<T> g0 = load_g2<T>(M, c);
                 const T r = stencil(w, M, c, sz, gm, g0, inv_pot) - (T)b[c];
-                const T a = (jj > 0) ? -s2 * gm.g22 : (T)0;
-                const T d = (r - a * dprev) * (T)F.invm[c];
-                e[jj] = w.p0 + omega * d + (T)F.cpr[c] * w.pp;
+                T a, inv, cp;
+                if constexpr (OTF) {
+                    a = factor(jj, c, gm.g22, g0.g22, inv);
+                    cp = cprev;
+                } else {
+                    a = (jj > 0) ? -s2 * gm.g22 : (T)0;
+                    inv = (T)F.invm[c];
+                    cp = (T)F.cpr[c];
+                }
+                const T d = (r - a * dprev) * inv;
+                e[jj] = w.p0 + omega * d + cp * w.pp;
                 dprev = d;
                 plast = w.p0;
                 gm = g0;
@@ -494,7 +539,10 @@ __global__ void __launch_bounds__(512) k_yline(Geo G, Met<TM> M, LineF F, const 
         T qn = 0;
 #pragma unroll
         for (int jj = MR - 1; jj >= 0; --jj) {
-            qn = e[jj] - (T)F.cpr[cb + sy * (jj + 1)] * qn;  // re-read: cache hit, saves registers
+            T cp;
+            if constexpr (OTF) cp = cps[jj][s][tx];
+            else cp = (T)F.cpr[cb + sy * (jj + 1)];  // re-read: cache hit, saves registers
+            qn = e[jj] - cp * qn;
             e[jj] = qn;
         }
         yF = e[0] - pfirst;

@@ -353,10 +353,15 @@ int smooth(H* h, Level& L, const TR* rhs, int nu, bool zero_start, bool fine) {
     for (int s = 0; s < nu; ++s) {
         const int zin = (zero_start && s == 0) ? 1 : 0;
         // forward: rhs, metrics, iterate, invm, cpr; last pass: (v, w), write.  The staged variant
-        // adds the e write/read, the cpr re-read and the q write/read.
-        const double stage = reg_rows(L.G.ny / L.P) ? 0.0 : (4 * sp + 4);
-        const double bytes = zin ? N * (sr + sm + 8 + sp + seg + stage)
-                                 : N * (9 * sm + sp + sr + 8 + sp + seg + stage);
+        // adds the e write/read, the cpr re-read and the q write/read.  FP32 register variants
+        // factor on the fly (no invm, cpr; the zero-start sweep reads g11, g33 instead).
+        const int MR = L.G.ny / L.P;
+        const int bxs = std::max(32, std::min(128, 256 / L.P));
+        const bool otf = sizeof(T) == 4 && reg_rows(MR) && MR * L.P * bxs * 4 <= 32768;
+        const double stage = reg_rows(MR) ? 0.0 : (4 * sp + 4);
+        const double fac = otf ? 0.0 : 8.0;
+        const double bytes = zin ? N * (sr + sm + (otf ? 2 * sm : 8.0) + sp + seg + stage)
+                                 : N * (9 * sm + sp + sr + fac + sp + seg + stage);
         {
             KT kt(h, fine ? PLES_MG_K_SMOOTH_FINE : PLES_MG_K_SMOOTH, bytes);
             switch (L.P) {
