@@ -45,6 +45,8 @@ module ples_mg_f
 
    public :: ples_mg_default_opts, ples_mg_create, ples_mg_destroy, ples_mg_solve
    public :: ples_mg_residual, ples_mg_case_rhs, ples_mg_get_rhs
+   public :: ples_mg_diver, ples_mg_gradie, ples_mg_update, ples_mg_contra, ples_mg_vel_up
+   public :: ples_mg_check_divergence, ples_mg_projection_step
    public :: ples_multi_init, ples_multi, ples_multi_finalize, ples_multi_handle
 
    interface
@@ -101,6 +103,86 @@ module ples_mg_f
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: h
          real(c_double), intent(out) :: rhs(*)
+      end function
+
+      ! diver (fsrc/diver.f03:2-46): rhs from uc(0:jx,1:jy,1:jz), vc(1:jx,0:jy,1:jz), wc(1:jx,1:jy,0:jz)
+      integer(c_int) function ples_mg_diver(h, uc, vc, wc, dt, rhs) bind(C, name="ples_mg_diver")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: uc(*), vc(*), wc(*)
+         real(c_double), value :: dt
+         real(c_double), intent(out) :: rhs(*)
+      end function
+
+      ! gradie (fsrc/gradie.f03:2-120); outputs may be c_null_ptr
+      integer(c_int) function ples_mg_gradie(h, fi, deepl, deepr, dt, gra1, gra2, gra3, cgra1, cgra2, cgra3) &
+         bind(C, name="ples_mg_gradie")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: fi(*)
+         integer(c_int), value :: deepl, deepr
+         real(c_double), value :: dt
+         type(c_ptr), value :: gra1, gra2, gra3, cgra1, cgra2, cgra3
+      end function
+
+      ! update (fsrc/update.f03:2-289): u,v,w shaped like fi; delu.. (0:jx+1,0:jy+1,0:jz+1)
+      integer(c_int) function ples_mg_update(h, u, v, w, deepl, deepr, delu, delv, delw) bind(C, name="ples_mg_update")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(inout) :: u(*), v(*), w(*)
+         integer(c_int), value :: deepl, deepr
+         real(c_double), intent(in) :: delu(*), delv(*), delw(*)
+      end function
+
+      ! contra (fsrc/contra.f03:2-241): cs(1:6) = c_loc(cs1) .. c_loc(cs6) (or c_null_ptr)
+      integer(c_int) function ples_mg_contra(h, u, v, w, deepl, deepr, uc, vc, wc, cs, potenziale) &
+         bind(C, name="ples_mg_contra")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: u(*), v(*), w(*)
+         integer(c_int), value :: deepl, deepr
+         real(c_double), intent(inout) :: uc(*), vc(*), wc(*)
+         type(c_ptr), intent(in) :: cs(6)
+         integer(c_int), value :: potenziale
+      end function
+
+      ! vel_up (fsrc/vel_up.f03:2-313): gra(1:3), cgra(1:3) = c_loc of the gradient arrays
+      integer(c_int) function ples_mg_vel_up(h, u, v, w, deepl, deepr, uc, vc, wc, gra, cgra, dt, potenziale) &
+         bind(C, name="ples_mg_vel_up")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(inout) :: u(*), v(*), w(*)
+         integer(c_int), value :: deepl, deepr
+         real(c_double), intent(inout) :: uc(*), vc(*), wc(*)
+         type(c_ptr), intent(in) :: gra(3), cgra(3)
+         real(c_double), value :: dt
+         integer(c_int), value :: potenziale
+      end function
+
+      ! check_divergence (fsrc/check_divergence.f03:131-190)
+      integer(c_int) function ples_mg_check_divergence(h, uc, vc, wc, out) bind(C, name="ples_mg_check_divergence")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: uc(*), vc(*), wc(*)
+         real(c_double), intent(out) :: out(4)
+      end function
+
+      ! the projection step update -> contra -> diver -> multi -> gradie -> vel_up -> check_divergence
+      integer(c_int) function ples_mg_projection_step(h, u, v, w, deepl, deepr, delu, delv, delw, uc, vc, wc, cs, &
+                                                      fi, dt, eps, ficycle, potenziale, res, div) &
+         bind(C, name="ples_mg_projection_step")
+         import :: c_int, c_ptr, c_double, ples_mg_result
+         type(c_ptr), value :: h
+         real(c_double), intent(inout) :: u(*), v(*), w(*)
+         integer(c_int), value :: deepl, deepr
+         real(c_double), intent(in) :: delu(*), delv(*), delw(*)
+         real(c_double), intent(inout) :: uc(*), vc(*), wc(*)
+         type(c_ptr), intent(in) :: cs(6)
+         real(c_double), intent(inout) :: fi(*)
+         real(c_double), value :: dt, eps
+         integer(c_int), value :: ficycle, potenziale
+         type(ples_mg_result), intent(out) :: res
+         real(c_double), intent(out) :: div(4)
       end function
    end interface
 
