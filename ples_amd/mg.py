@@ -296,13 +296,15 @@ class Solver:
         return [None if c is None else np.ascontiguousarray(c, np.float64) for c in cs]
 
     # ---- drop-in
-    def multi(self, fi, rhs, cs=None, dt=1.0, eps=1e-8, ficycle=1000) -> Result:
-        """Solve in place (fi = warm start -> solution with ghosts)."""
-        assert fi.shape == self.fi_shape and rhs.shape == self.rhs_shape
+    def multi(self, fi, rhs, cs=None, dt=1.0, eps=1e-8, ficycle=1000, deepl=1, deepr=1) -> Result:
+        """Solve in place (fi = warm start -> solution with ghosts).  fi holds
+        k planes 1-deepl .. jz+deepr as the reference allocates it."""
+        assert fi.shape == (self.jz + deepl + deepr, self.jy + 2, self.jx + 2) and rhs.shape == self.rhs_shape
+        assert fi.flags["C_CONTIGUOUS"] and fi.dtype == np.float64
         cs = self._cs(cs)
         r = _Result()
         rhs = np.ascontiguousarray(rhs, np.float64)
-        st = lib().ples_mg_solve(self._h, _p(fi), 1, 1, _p(rhs), *[_p(c) for c in cs], dt, eps,
+        st = lib().ples_mg_solve(self._h, _p(fi), deepl, deepr, _p(rhs), *[_p(c) for c in cs], dt, eps,
                                  ficycle, C.byref(r))
         if st not in (OK, NOT_CONVERGED):
             raise PlesMGError(st, "ples_mg_solve")
