@@ -465,6 +465,7 @@ k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin, T* __restrict__ po
     __shared__ T cps[OTF ? MR : 1][P][BX];
     Seg q;
     const bool act = seg_of(G, P, q);
+    constexpr int mr = MR;
     const int tx = threadIdx.x, s = q.s;
     const long sy = G.sy, sz = G.sz;
     const T s2 = inv_pot * inv_pot;
@@ -475,15 +476,16 @@ k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin, T* __restrict__ po
     const T myf = (q.j0 == 1) ? (T)0 : (T)1, myl = (q.j1 == G.ny) ? (T)0 : (T)1;
     T e[MR];
     T yF = 0, yL = 0, vF = 0, vL = 0, wF = 0, wL = 0;
+    T elast = 0;  // final value of the segment's last row
     if (act) {
         T dprev = 0, pfirst = 0, plast = 0, cprev = 0;
         // factor of row jj: 1/m and c'; g22m = g22(j-1), g22p = g22(j)
         auto factor = [&](int jj, long c, T g22m, T g22p, T& inv) {
             const T den = s2 * (mxp * (T)M.g[0][c] + mxm * (T)M.g[0][c - 1] + mzp * (T)M.g[8][c] +
-                                mzm * (T)M.g[8][c - sz] + ((jj == MR - 1) ? myl : (T)1) * g22p +
+                                mzm * (T)M.g[8][c - sz] + ((jj == mr - 1) ? myl : (T)1) * g22p +
                                 ((jj == 0) ? myf : (T)1) * g22m);
             const T a = (jj > 0) ? -s2 * g22m : (T)0;
-            const T cc = (jj < MR - 1) ? -s2 * g22p : (T)0;
+            const T cc = (jj < mr - 1) ? -s2 * g22p : (T)0;
             inv = (T)1 / (den - a * cprev);
             const T cp = cc * inv;
             cps[jj][s][tx] = cp;
@@ -544,11 +546,12 @@ k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin, T* __restrict__ po
             else cp = (T)F.cpr[cb + sy * (jj + 1)];  // re-read: cache hit, saves registers
             qn = e[jj] - cp * qn;
             e[jj] = qn;
+            if (jj == mr - 1) elast = qn;
         }
         yF = e[0] - pfirst;
-        yL = e[MR - 1] - plast;
+        yL = elast - plast;
         if (P > 1) {
-            const long cF = cb + sy, cL = cb + sy * MR;
+            const long cF = cb + sy, cL = cb + sy * mr;
             vF = (T)F.v[cF];
             vL = (T)F.v[cL];
             wF = (T)F.w[cF];
@@ -568,6 +571,7 @@ k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin, T* __restrict__ po
     }
     if (!act) return;
     const int i = q.i, k = q.k;
+    T pfin = 0;
 #pragma unroll
     for (int jj = 0; jj < MR; ++jj) {
         const long c = cb + sy * (jj + 1);
@@ -576,8 +580,9 @@ k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin, T* __restrict__ po
         e[jj] = pn;
         pout[c] = pn;
         write_images(G, pout, i, q.j0 + jj, k, pn);
+        if (jj == mr - 1) pfin = pn;
     }
-    yline_ghosts(G, M, pin, pout, i, k, q.j0 == 1, q.j1 == G.ny, e[0], e[MR - 1], zero_in);
+    yline_ghosts(G, M, pin, pout, i, k, q.j0 == 1, q.j1 == G.ny, e[0], pfin, zero_in);
 }
 
 // Segmented smoother for segment lengths without a register variant: the
@@ -936,8 +941,8 @@ __device__ inline int wrapk(const Geo& G, int k) {
 // one launch per face set (0 eta, 1 xi, 2 zeta) and colour, so that no
 // ghost read in a launch is written by the same launch
 template <class T, class TM>
-__global__ void k_halo_face(Geo G, Met<TM> M, T* __restrict__ p, int color, BCs bc, int set, int images) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void halo_face_point(const Geo& G, const Met<TM>& M, T* __restrict__ p, long t, int color,
+                                       const BCs& bc, int set, int images) {
     if (set == 1) t += 2L * G.nx * G.nz;
     if (set == 2) t += 2L * G.nx * G.nz + (G.ip == 1 ? 2L * G.ny * G.nz : 0L);
     const double an = bc.an_over_dt;
@@ -1079,10 +1084,9 @@ __device__ inline bool ghost_source(const Geo& G, int i, int j, int k, int& si, 
 }
 
 template <class T>
-__global__ void k_halo_edges(Geo G, T* __restrict__ p) {
+__device__ inline void halo_edge_point(const Geo& G, T* __restrict__ p, long t) {
     // every ghost position: enumerate the six ghost planes of the box, each
     // position counted once by giving planes i-ghost > j-ghost > k-ghost
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long NX = G.nx + 2, NY = G.ny + 2, NZ = G.nz + 2;
     int i, j, k;
     const long ni = 2 * NY * NZ;  // i in {0, nx+1}
@@ -1112,6 +1116,17 @@ __global__ void k_halo_edges(Geo G, T* __restrict__ p) {
     if (!ghost_source(G, i, j, k, si, sj, sk)) return;
     p[G.idx(i, j, k)] = p[G.idx(si, sj, sk)];
 }
+
+template <class T, class TM>
+__global__ void k_halo_face(Geo G, Met<TM> M, T* __restrict__ p, int color, BCs bc, int set, int images) {
+    halo_face_point(G, M, p, (long)blockIdx.x * blockDim.x + threadIdx.x, color, bc, set, images);
+}
+
+template <class T>
+__global__ void k_halo_edges(Geo G, T* __restrict__ p) {
+    halo_edge_point(G, p, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 
 // ------------------------------------------------------------------------
 // Smoother factors of the segmented eta-line solve (see LineF), computed in

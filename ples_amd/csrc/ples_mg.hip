@@ -78,6 +78,7 @@ struct ples_mg_handle {
     std::vector<int> gin[2], gout[2];
     double last_removed = 0.0;
     int inner = 1;                   // V-cycles per refinement step (resolved from opts)
+    int lds_max = 65536;             // LDS bytes per workgroup for the coarsest-level solver
     // FP32 inner FGCR of the mixed solver (level-0 FP32 vectors, own scalars)
     bool ir32 = false;
     float* e32 = nullptr;            // inner correction
@@ -312,38 +313,36 @@ int halo64(H* h, double* p, bool with_bc, bool fast = false) {
 }
 
 // ---------------------------------------------------------------- V-cycle pieces
+// register variant of a segment length: 4, 8, 16 or 32 rows; other
+// lengths are staged through memory (measured faster than the register
+// variant with guarded row loops, whose larger register arrays lower the
+// occupancy)
+int yline_mr(int m) { return (m == 4 || m == 8 || m == 16 || m == 32) ? m : 0; }
+
+template <int P, int MR, class T, class TR>
+void launch_yline_mr(H* h, Level& L, const TR* rhs, int zin) {
+    const ColLaunch cl = seg_launch(L.G, P);
+    hipLaunchKernelGGL((k_yline<P, MR, T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), line_f(L),
+                       (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->o.omega, (T)(1.0 / L.pot), zin);
+}
+
 template <int P, class T, class TR>
 void launch_yline(H* h, Level& L, const TR* rhs, int zin) {
-    const ColLaunch cl = seg_launch(L.G, P);
-    const LineF F = line_f(L);
-    const T* pin = (const T*)L.p[L.cur];
-    T* pout = (T*)L.p[1 - L.cur];
-    const T om = (T)h->o.omega, ip = (T)(1.0 / L.pot);
-    switch (L.G.ny / P) {
-        case 4:
-            hipLaunchKernelGGL((k_yline<P, 4, T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), F, pin, pout,
-                               rhs, om, ip, zin);
-            break;
-        case 8:
-            hipLaunchKernelGGL((k_yline<P, 8, T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), F, pin, pout,
-                               rhs, om, ip, zin);
-            break;
-        case 16:
-            hipLaunchKernelGGL((k_yline<P, 16, T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), F, pin,
-                               pout, rhs, om, ip, zin);
-            break;
-        case 32:
-            hipLaunchKernelGGL((k_yline<P, 32, T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), F, pin,
-                               pout, rhs, om, ip, zin);
-            break;
-        default:
-            hipLaunchKernelGGL((k_yline_staged<P, T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), F, pin,
-                               pout, rhs, om, ip, zin);
+    switch (yline_mr(L.G.ny / P)) {
+        case 4: launch_yline_mr<P, 4, T, TR>(h, L, rhs, zin); break;
+        case 8: launch_yline_mr<P, 8, T, TR>(h, L, rhs, zin); break;
+        case 16: launch_yline_mr<P, 16, T, TR>(h, L, rhs, zin); break;
+        case 32: launch_yline_mr<P, 32, T, TR>(h, L, rhs, zin); break;
+        default: {
+            const ColLaunch cl = seg_launch(L.G, P);
+            hipLaunchKernelGGL((k_yline_staged<P, T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), line_f(L),
+                               (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->o.omega, (T)(1.0 / L.pot), zin);
+        }
     }
 }
 
 // segment lengths with a register-resident smoother variant
-bool reg_rows(int m) { return m == 4 || m == 8 || m == 16 || m == 32; }
+bool reg_rows(int m) { return yline_mr(m) != 0; }
 
 template <class T, class TR>
 int smooth(H* h, Level& L, const TR* rhs, int nu, bool zero_start, bool fine) {
@@ -355,10 +354,10 @@ int smooth(H* h, Level& L, const TR* rhs, int nu, bool zero_start, bool fine) {
         // forward: rhs, metrics, iterate, invm, cpr; last pass: (v, w), write.  The staged variant
         // adds the e write/read, the cpr re-read and the q write/read.  FP32 register variants
         // factor on the fly (no invm, cpr; the zero-start sweep reads g11, g33 instead).
-        const int MR = L.G.ny / L.P;
+        const int MR = yline_mr(L.G.ny / L.P);
         const int bxs = std::max(32, std::min(128, 256 / L.P));
-        const bool otf = sizeof(T) == 4 && reg_rows(MR) && MR * L.P * bxs * 4 <= 32768;
-        const double stage = reg_rows(MR) ? 0.0 : (4 * sp + 4);
+        const bool otf = sizeof(T) == 4 && MR > 0 && MR * L.P * bxs * 4 <= 32768;
+        const double stage = MR > 0 ? 0.0 : (4 * sp + 4);
         const double fac = otf ? 0.0 : 8.0;
         const double bytes = zin ? N * (sr + sm + (otf ? 2 * sm : 8.0) + sp + seg + stage)
                                  : N * (9 * sm + sp + sr + fac + sp + seg + stage);
@@ -465,7 +464,7 @@ int coarse_solve(H* h, Level& L, const double* rhs, int nu) {
     const int ncol = L.G.nx * L.G.nz;
     KT kt(h, PLES_MG_K_SMOOTH, (double)L.ncell() * (9 * 8 + 8 + 8 + 8 + 8));
     const size_t lds = coarse_lds_bytes(L);
-    if (lds <= 64 * 1024) {
+    if (lds <= (size_t)h->lds_max) {
         // result lands in p[1 - cur]
         hipLaunchKernelGGL((k_coarse_lds<double>), dim3(1), dim3(256), lds, h->s, L.G, met<double>(L), line_f(L),
                            (double*)L.p[1 - L.cur], rhs, (double)h->o.omega, 1.0 / L.pot, nu);
@@ -1190,6 +1189,18 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
     h->dev = o.device;
     CK(hipSetDevice(h->dev));
     CK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    // workgroup LDS the coarsest-level solver may use (up to 160 KB per
+    // workgroup on this GPU; 64 KB without the opt-in)
+    {
+        int v = 0;
+        h->lds_max = 64 * 1024;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, h->dev) == hipSuccess &&
+            v > h->lds_max &&
+            hipFuncSetAttribute(reinterpret_cast<const void*>(&k_coarse_lds<double>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, v) == hipSuccess)
+            h->lds_max = v;
+        (void)hipGetLastError();
+    }
     CK(hipEventCreate(&h->t0));
     CK(hipEventCreate(&h->t1));
     // pinned S mirror: [0, S_COUNT) scalars, then 4 divergence maxima, then
