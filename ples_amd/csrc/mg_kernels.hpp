@@ -295,7 +295,10 @@ __device__ inline void reduced_solve(T (&red)[8][P][BX], int tx, int s, T& uLm, 
 // lanes of a segmented-smoother block along xi for P segments
 template <int P>
 struct SegBX {
-    static constexpr int value = (256 / P < 32) ? 32 : ((256 / P > 128) ? 128 : 256 / P);
+    // 256/P rounded down to a power of two, clamped to [32, 128]
+    static constexpr int raw = 256 / P;
+    static constexpr int pow2 = raw >= 128 ? 128 : (raw >= 64 ? 64 : 32);
+    static constexpr int value = pow2;
 };
 
 // eta-wall ghosts of column (i,k) after its new values pn1 = p(i,1,k),

@@ -182,10 +182,17 @@ ColLaunch col_launch(const Geo& G) {
     return c;
 }
 
+// lanes per block of a segmented launch: 256/P rounded down to a power of
+// two, clamped to [32, 128] (SegBX on the device)
+int seg_bx(int P) {
+    const int raw = 256 / P;
+    return raw >= 128 ? 128 : (raw >= 64 ? 64 : 32);
+}
+
 // segmented-column launch: block (bx, P) over (i, segment), one zeta plane per block row
 ColLaunch seg_launch(const Geo& G, int P) {
     ColLaunch c;
-    const int bxmax = std::max(32, std::min(128, 256 / P));  // = SegBX<P>
+    const int bxmax = seg_bx(P);  // = SegBX<P>
     const int bx = std::min(bxmax, pow2ceil(G.nx));
     c.block = dim3(bx, P, 1);
     c.grid = dim3((G.nx + bx - 1) / bx, G.nz, 1);
@@ -198,6 +205,15 @@ ColLaunch seg_launch(const Geo& G, int P) {
 int choose_segments(int ny) {
     int P = 1;
     while (P < 16 && ny % (2 * P) == 0 && ny / (2 * P) >= 16) P *= 2;
+    // a segment length without a register variant: try three-way splits,
+    // which turn the 3*2^k row counts (48, 96, ...) into 4..32
+    const int m = ny / P;
+    if (m != 4 && m != 8 && m != 16 && m != 32)
+        for (int Q : {3, 6, 12})
+            if (ny % Q == 0) {
+                const int mq = ny / Q;
+                if (mq == 4 || mq == 8 || mq == 16 || mq == 32) return Q;
+            }
     return P;
 }
 
@@ -355,7 +371,7 @@ int smooth(H* h, Level& L, const TR* rhs, int nu, bool zero_start, bool fine) {
         // adds the e write/read, the cpr re-read and the q write/read.  FP32 register variants
         // factor on the fly (no invm, cpr; the zero-start sweep reads g11, g33 instead).
         const int MR = yline_mr(L.G.ny / L.P);
-        const int bxs = std::max(32, std::min(128, 256 / L.P));
+        const int bxs = seg_bx(L.P);
         const bool otf = sizeof(T) == 4 && MR > 0 && MR * L.P * bxs * 4 <= 32768;
         const double stage = MR > 0 ? 0.0 : (4 * sp + 4);
         const double fac = otf ? 0.0 : 8.0;
@@ -365,8 +381,11 @@ int smooth(H* h, Level& L, const TR* rhs, int nu, bool zero_start, bool fine) {
             KT kt(h, fine ? PLES_MG_K_SMOOTH_FINE : PLES_MG_K_SMOOTH, bytes);
             switch (L.P) {
                 case 16: launch_yline<16, T, TR>(h, L, rhs, zin); break;
+                case 12: launch_yline<12, T, TR>(h, L, rhs, zin); break;
                 case 8: launch_yline<8, T, TR>(h, L, rhs, zin); break;
+                case 6: launch_yline<6, T, TR>(h, L, rhs, zin); break;
                 case 4: launch_yline<4, T, TR>(h, L, rhs, zin); break;
+                case 3: launch_yline<3, T, TR>(h, L, rhs, zin); break;
                 case 2: launch_yline<2, T, TR>(h, L, rhs, zin); break;
                 default: launch_yline<1, T, TR>(h, L, rhs, zin); break;
             }
@@ -974,9 +993,13 @@ struct TuneVal {
 std::mutex g_tune_mu;
 std::map<std::array<int, 7>, TuneVal> g_tune;
 
+bool valid_segments(int P) {
+    return P == 1 || P == 2 || P == 3 || P == 4 || P == 6 || P == 8 || P == 12 || P == 16;
+}
+
 std::vector<int> seg_candidates(int ny) {
     std::vector<int> v;
-    for (int P = 1; P <= 16; P *= 2)
+    for (int P : {1, 2, 3, 4, 6, 8, 12, 16})
         if (ny % P == 0 && ny / P >= 4) v.push_back(P);
     return v;
 }
@@ -2820,8 +2843,8 @@ int ples_mg_tuning_load(const char* path) {
         if (std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d", &k[0], &k[1], &k[2], &k[3], &k[4], &k[5], &k[6],
                         &v.P, &v.rP, &v.rbx) != 10)
             continue;
-        const bool okP = v.P >= 1 && v.P <= 16 && (v.P & (v.P - 1)) == 0 && k[1] % v.P == 0;
-        const bool okR = v.rP >= 1 && v.rP <= 16 && (v.rP & (v.rP - 1)) == 0 && k[1] % v.rP == 0 && v.rbx >= 1 && (v.rbx & (v.rbx - 1)) == 0 &&
+        const bool okP = valid_segments(v.P) && k[1] % v.P == 0;
+        const bool okR = valid_segments(v.rP) && k[1] % v.rP == 0 && v.rbx >= 1 && (v.rbx & (v.rbx - 1)) == 0 &&
                          v.rbx <= 256 && v.rbx * v.rP <= 512;
         if (!okP || !okR) continue;
         g_tune[k] = v;

@@ -217,3 +217,39 @@ def test_tuned_launch_shapes_same_solution():
                         table[tuple(v[:7])] = tuple(v[7:])
                 assert s.tiles(1) == table[(64, 32, 64, 0, 0, 1, 1)]
     assert cases.rel_l2_meanfree(out[1], out[0]) < 1e-9
+
+
+_SEG_REF = {}
+
+
+def _seg_reference():
+    """oracle solution of the 32x48x32 case (computed once per session)"""
+    if not _SEG_REF:
+        g = cases.grid(32, 48, 32, gamma=1.5, amp=0.2)
+        rhs = O.case_rhs(g, 321)
+        fiv = g.zeros()
+        _, rmax, _ = O.multi(g, 2, fiv, rhs, eps=1e-13, ficycle=3000)
+        _SEG_REF.update(g=g, rhs=rhs, fiv=fiv, rmax=rmax)
+    return _SEG_REF["g"], _SEG_REF["rhs"], _SEG_REF["fiv"], _SEG_REF["rmax"]
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 4, 6, 12])
+def test_segment_counts_same_solution(P, tmp_path):
+    """Every segment count of the line smoother (powers of two and the
+    three-way splits) solves the same problem to the same solution: the
+    fine level's P is forced through a tuning-table entry (a shape no other
+    test uses) and compared with the oracle."""
+    g, rhs, fiv, rmax = _seg_reference()
+    table = tmp_path / "t.txt"
+    rows = []
+    for f32 in (0, 1):
+        rows.append(f"32 48 32 0 0 {f32} 1 {P} {P} 32")
+    table.write_text("\n".join(rows) + "\n")
+    assert mg.tuning_load(str(table))
+    with _solver(g, 3, "fp64", autotune=1, stop_mode="abs_max") as s:
+        assert s.tiles(1)[0] == P
+        fi = np.zeros(s.fi_shape)
+        r = s.multi(fi, cases.interior(rhs), eps=1e-13, ficycle=300)
+        assert r.ok, r
+    assert rmax < 1e-13
+    assert cases.rel_l2_meanfree(fi, fiv) < 1e-9
