@@ -291,6 +291,21 @@ int ples_mg_server_get_fi(ples_mg_server* sv, uint64_t case_id, double* fi);
 int ples_mg_server_stats(const ples_mg_server* sv, size_t* used_bytes, int* resident,
                          long* completed, long* evictions, long* cold_starts);
 
+/* ---- restart file of the flow field ----
+ * The layout PLES reads at restart (old_res_form, fsrc/restart.f03:33-98):
+ * header nscal / ti / bbx dbbx, then one record u v w fi rhov(1..nscal) per
+ * cell over (0:jx+1,0:jy+1,0:jz+1), then uc (0:jx,1:jy,1:jz), vc
+ * (1:jx,0:jy,1:jz), wc (1:jx,1:jy,0:jz) one value per record, data in
+ * Fortran format 10e18.10.  Arrays in the shapes above (rhov(isc,i,j,k),
+ * isc fastest); read outputs may be NULL.  Host-only. */
+int ples_mg_restart_write(const char* path, int jx, int jy, int jz, int nscal, double ti, double bbx,
+                          double dbbx, const double* u, const double* v, const double* w,
+                          const double* fi, const double* rhov, const double* uc, const double* vc,
+                          const double* wc);
+int ples_mg_restart_read(const char* path, int jx, int jy, int jz, int nscal, double* ti, double* bbx,
+                         double* dbbx, double* u, double* v, double* w, double* fi, double* rhov,
+                         double* uc, double* vc, double* wc);
+
 /* Autotuner table (process-wide): load entries from a text file (one level
  * shape per line: nx ny nz ip kp f32 fine P rP rbx), so handles created with
  * opts.autotune take those shapes without timing; save the current table. */

@@ -42,9 +42,9 @@ def build_fortran(verbose: bool = False) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
-    src = os.path.join(CSRC, "ples_mg.hip")
+    srcs = [os.path.join(CSRC, "ples_mg.hip"), os.path.join(CSRC, "restart.cpp")]
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp, src]
+    cmd = [HIPCC] + FLAGS + ["-o", tmp] + srcs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -132,6 +132,8 @@ def lib():
         L.ples_mg_tiles.argtypes = [C.c_void_p, C.c_int] + [C.POINTER(C.c_int)] * 3
         L.ples_mg_copy_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
         L.ples_mg_tuning_load.argtypes = [C.c_char_p]
+        L.ples_mg_restart_write.argtypes = [C.c_char_p] + [C.c_int] * 4 + [C.c_double] * 3 + [_dp] * 8
+        L.ples_mg_restart_read.argtypes = [C.c_char_p] + [C.c_int] * 4 + [_dp] * 3 + [_dp] * 8
         L.ples_mg_tuning_save.argtypes = [C.c_char_p]
         L.ples_mg_server_default_opts.argtypes = [C.POINTER(_ServerOpts)]
         L.ples_mg_server_create.argtypes = [C.POINTER(_ServerOpts), C.POINTER(_Opts), C.POINTER(C.c_void_p)]
@@ -213,6 +215,33 @@ def copy_bandwidth(nbytes=1 << 30, reps=20, device=0) -> float:
     g = C.c_double()
     _check(lib().ples_mg_copy_bandwidth(device, nbytes, reps, C.byref(g)), "ples_mg_copy_bandwidth")
     return g.value
+
+
+def restart_write(path, ti, bbx, dbbx, u, v, w, fi, rhov, uc, vc, wc):
+    """Write the reference's restart file (old_res_form layout).  Arrays in C
+    order: u, v, w, fi (jz+2, jy+2, jx+2); rhov (jz+2, jy+2, jx+2, nscal);
+    uc (jz, jy, jx+1), vc (jz, jy+1, jx), wc (jz+1, jy, jx)."""
+    jz, jy, jx = (d - 2 for d in u.shape)
+    nscal = rhov.shape[-1] if rhov is not None else 0
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (u, v, w, fi)]
+    rh = np.ascontiguousarray(rhov, np.float64) if nscal else None
+    fl = [np.ascontiguousarray(a, np.float64) for a in (uc, vc, wc)]
+    _check(lib().ples_mg_restart_write(path.encode(), jx, jy, jz, nscal, ti, bbx, dbbx, *[_p(a) for a in arrs],
+                                       _p(rh) if rh is not None else None, *[_p(a) for a in fl]),
+           "ples_mg_restart_write")
+
+
+def restart_read(path, jx, jy, jz, nscal):
+    """Read a restart file; returns (ti, bbx, dbbx, u, v, w, fi, rhov, uc, vc, wc)."""
+    cells = (jz + 2, jy + 2, jx + 2)
+    u, v, w, fi = (np.zeros(cells) for _ in range(4))
+    rhov = np.zeros(cells + (nscal,)) if nscal else None
+    uc, vc, wc = np.zeros((jz, jy, jx + 1)), np.zeros((jz, jy + 1, jx)), np.zeros((jz + 1, jy, jx))
+    t, b1, b2 = C.c_double(), C.c_double(), C.c_double()
+    _check(lib().ples_mg_restart_read(path.encode(), jx, jy, jz, nscal, C.byref(t), C.byref(b1), C.byref(b2),
+                                      *[_p(a) for a in (u, v, w, fi)], _p(rhov) if rhov is not None else None,
+                                      *[_p(a) for a in (uc, vc, wc)]), "ples_mg_restart_read")
+    return t.value, b1.value, b2.value, u, v, w, fi, rhov, uc, vc, wc
 
 
 TUNING_TABLE = os.path.join(_HERE, "tuning", "mi355x.txt")
