@@ -288,6 +288,12 @@ int ples_mg_server_poll(ples_mg_server* sv, int64_t ticket, ples_mg_result* res,
 int ples_mg_server_pending(const ples_mg_server* sv);
 /* The case's resident pressure (layout of ples_mg_get_fi); ERR_BUSY if cold or running. */
 int ples_mg_server_get_fi(ples_mg_server* sv, uint64_t case_id, double* fi);
+/* Start the case's next request from this pressure (layout of ples_mg_set_fi),
+ * as a warm start: a field PLES wrote to its restart file (fsrc/restart.f03:33-98
+ * reads it back into fi) resumes a served simulation where it stopped.  Copied
+ * to the device now if the case is resident, else kept on the host until the
+ * case is admitted; ERR_BUSY while the case runs. */
+int ples_mg_server_set_fi(ples_mg_server* sv, uint64_t case_id, const double* fi);
 int ples_mg_server_stats(const ples_mg_server* sv, size_t* used_bytes, int* resident,
                          long* completed, long* evictions, long* cold_starts);
 

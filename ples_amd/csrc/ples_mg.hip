@@ -2395,6 +2395,7 @@ struct SCase {
     bool busy = false;
     bool solved = false;          // the resident iterate holds a previous solution
     uint64_t last_use = 0;
+    std::vector<double> seed_fi;  // pressure the next admission starts from (host, ples_mg_set_fi layout)
 };
 
 enum { RQ_QUEUED = 0, RQ_INIT, RQ_RUN, RQ_CONFIRM, RQ_DONE };
@@ -2493,6 +2494,11 @@ int try_admit(SV* sv, SReq& r) {
         sv->used += c.est;
         CKS(ples_mg_case_rhs(h, c.rhs_seed, c.dt));
         ++sv->cold_starts;
+    }
+    if (!c.seed_fi.empty()) {  // pressure given by the caller (e.g. read from a restart file)
+        CKS(ples_mg_set_fi(c.h, c.seed_fi.data()));
+        std::vector<double>().swap(c.seed_fi);
+        c.solved = true;
     }
     H* h = c.h;
     c.busy = true;
@@ -2773,6 +2779,20 @@ int ples_mg_server_get_fi(ples_mg_server* sv, uint64_t case_id, double* fi) {
     SCase& c = sv->cases.at(case_id);
     if (!c.h || c.busy) return PLES_MG_ERR_BUSY;
     return ples_mg_get_fi(c.h, fi);
+}
+
+int ples_mg_server_set_fi(ples_mg_server* sv, uint64_t case_id, const double* fi) {
+    if (!sv || !fi || !sv->cases.count(case_id)) return PLES_MG_ERR_ARG;
+    SCase& c = sv->cases.at(case_id);
+    if (c.busy) return PLES_MG_ERR_BUSY;
+    if (c.h) {
+        CKS(ples_mg_set_fi(c.h, fi));
+        c.solved = true;
+        return PLES_MG_OK;
+    }
+    const size_t n = (size_t)(c.jx + 2) * (c.jy + 2) * (c.jz + 2);
+    c.seed_fi.assign(fi, fi + n);
+    return PLES_MG_OK;
 }
 
 int ples_mg_server_stats(const ples_mg_server* sv, size_t* used_bytes, int* resident, long* completed,

@@ -146,6 +146,7 @@ def lib():
         L.ples_mg_server_poll.argtypes = [C.c_void_p, C.c_int64, C.POINTER(_Result), C.POINTER(C.c_int)]
         L.ples_mg_server_pending.argtypes = [C.c_void_p]
         L.ples_mg_server_get_fi.argtypes = [C.c_void_p, C.c_uint64, _dp]
+        L.ples_mg_server_set_fi.argtypes = [C.c_void_p, C.c_uint64, _dp]
         L.ples_mg_server_stats.argtypes = [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_int),
                                            C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_long)]
         L.ples_mg_case_rhs.argtypes = [C.c_void_p, C.c_uint64, C.c_double]
@@ -608,6 +609,15 @@ class Server:
         a = np.zeros((nzp + 1, nyp + 1, nxp + 1))
         _check(lib().ples_mg_server_get_fi(self._sv, case_id, _p(a)), "ples_mg_server_get_fi")
         return a
+
+    def set_fi(self, case_id, fi):
+        """Start the case's next request from this pressure (shape of get_fi),
+        e.g. the fi of a restart file."""
+        (nzp, nyp, nxp), _ = self._cases[case_id]
+        a = np.ascontiguousarray(fi, np.float64)
+        if a.shape != (nzp + 1, nyp + 1, nxp + 1):
+            raise ValueError(f"fi shape {a.shape} != {(nzp + 1, nyp + 1, nxp + 1)}")
+        _check(lib().ples_mg_server_set_fi(self._sv, case_id, _p(a)), "ples_mg_server_set_fi")
 
     def stats(self) -> dict:
         u, r = C.c_size_t(), C.c_int()

@@ -84,3 +84,50 @@ def test_hbm_budget_admission_evicts_idle_cases():
     assert all(r.ok and r.rel_res < 1e-8 for r, _ in out)
     assert st["evictions"] > 0 and st["cold_starts"] > 4
     assert st["used_bytes"] <= int(2.6 * per_case + (128 << 20))
+
+
+def test_served_case_resumes_from_its_pressure_and_restart_file(tmp_path):
+    """A case resumed on a fresh server from its pressure (ples_mg_server_set_fi)
+    gives bitwise the result of continuing on the server that solved it; resumed
+    from the flow-field restart file (ten significant digits) it starts warm,
+    meets its tolerance and needs fewer cycles than a cold start."""
+    c = _cases()[2]
+    cid = c["case_id"]
+    steps = [(7000, 1e-10), (7001, 1e-10), (7002, 1e-10)]
+
+    def run(sv, seed, eps):
+        t = sv.submit(cid, seed, 1e-3, eps, 200)
+        sv.drain()
+        return sv.poll(t)
+
+    with _server([c], max_active=1) as sv:
+        for seed, eps in steps[:2]:
+            assert run(sv, seed, eps)[0].ok
+        fi = sv.get_fi(cid)
+        cont, w = run(sv, *steps[2])
+        assert w and cont.ok
+        fi_cont = sv.get_fi(cid)
+    with _server([c], max_active=1) as sv:
+        sv.set_fi(cid, fi)  # not resident yet: kept until admission
+        r, w = run(sv, *steps[2])
+        assert w and r.cycles == cont.cycles and r.rel_res == cont.rel_res
+        assert np.array_equal(sv.get_fi(cid), fi_cont)
+        sv.set_fi(cid, fi)  # resident: copied at once
+        r, w = run(sv, *steps[2])
+        assert w and r.cycles == cont.cycles and r.rel_res == cont.rel_res
+
+    zc = np.zeros(fi.shape)
+    jz, jy, jx = (d - 2 for d in fi.shape)
+    path = str(tmp_path / "new_res_form")
+    mg.restart_write(path, 1.5, 0.0, 0.0, zc, zc, zc, fi, None, np.zeros((jz, jy, jx + 1)),
+                     np.zeros((jz, jy + 1, jx)), np.zeros((jz + 1, jy, jx)))
+    fi_file = mg.restart_read(path, jx, jy, jz, 0)[6]
+    assert np.allclose(fi_file, fi, rtol=5.0001e-10, atol=0)
+    with _server([c], max_active=1) as sv:
+        cold, w = run(sv, *steps[2])
+        assert not w and cold.ok
+    with _server([c], max_active=1) as sv:
+        sv.set_fi(cid, fi_file)
+        r, w = run(sv, *steps[2])
+        assert w and r.ok and r.rel_res < 1e-10
+        assert abs(r.cycles - cont.cycles) <= 1 and r.cycles < cold.cycles
