@@ -321,7 +321,7 @@ def main():
         "cycles": {"mixed": mixed["cycles"][-1], "fp64": fp64["cycles"][-1]},
         "rel_res": {"mixed": mixed["res"].rel_res, "fp64": fp64["res"].rel_res},
         "roofline": {
-            "kernel": "k_yline<float> fine level (eta-line smoother)",
+            "kernel": "k_yline<float, 16-bit metrics> fine level (eta-line smoother)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

@@ -90,6 +90,14 @@ typedef struct ples_mg_opts {
                         an FP32 GCR(2) preconditioned by the FP32 V-cycle (default);
                         0 = the FP32 V-cycle directly inside the FP64 GCR(2).
                         Batched and served solves use the latter. */
+    int smoother_fp16; /* PLES_MG_MIXED only: 1 = the smoother of the FP32 levels
+                        reads a 16-bit copy of the level's metrics (scaled by a
+                        power of four, folded into 1/pot); the V-cycle residual,
+                        the FP32 operator and the FP64 outer residual keep their
+                        metrics, so only the preconditioner changes, not the
+                        solution.  Used on the fine level (the autotuner may
+                        choose per level).  0 = FP32 metrics everywhere.
+                        Default 1 */
 } ples_mg_opts;
 
 typedef struct ples_mg_result {
@@ -313,8 +321,11 @@ int ples_mg_restart_read(const char* path, int jx, int jy, int jz, int nscal, do
                          double* uc, double* vc, double* wc);
 
 /* Autotuner table (process-wide): load entries from a text file (one level
- * shape per line: nx ny nz ip kp f32 fine P rP rbx), so handles created with
- * opts.autotune take those shapes without timing; save the current table. */
+ * shape per line: nx ny nz ip kp f32 fine half P rP rbx use16 -- half: the
+ * level has the 16-bit smoother metrics, use16: its smoother reads them;
+ * lines of the older form nx ny nz ip kp f32 fine P rP rbx are read with
+ * half = use16 = 0), so handles created with opts.autotune take those shapes
+ * without timing; save the current table. */
 int ples_mg_tuning_load(const char* path);
 int ples_mg_tuning_save(const char* path);
 

@@ -1590,6 +1590,23 @@ __global__ void k_convert(long n, const TI* __restrict__ a, TO* __restrict__ b) 
     for (; t < n; t += stride) b[t] = (TO)a[t];
 }
 
+// b = 16-bit copy of a * scale (round to nearest)
+__global__ void k_to_half(long n, const float* __restrict__ a, _Float16* __restrict__ b, float scale) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) b[t] = (_Float16)(a[t] * scale);
+}
+
+// per-block maxima of |a| over n values (grid-stride), one partial per block
+__global__ void __launch_bounds__(256) k_absmax(long n, const float* __restrict__ a, double* partials) {
+    double v[1] = {0.0};
+    const bool mx[1] = {true};
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) v[0] = fmax(v[0], fabs((double)a[t]));
+    block_reduce_store<1>(v, partials, gridDim.x, mx);
+}
+
 // gradie (fsrc/gradie.f03:28-117)
 __global__ void __launch_bounds__(256) k_gradie(Geo G, const double* __restrict__ fi, CsvP V,
                                                 const double* __restrict__ giac, Met<double> M,

@@ -28,6 +28,9 @@ struct Level {
     double pot = 1.0;
     double* g64[9] = {};
     float* g32[9] = {};
+    _Float16* g16[9] = {};  // smoother copy of g32 * 4^m (FP32 levels, smoother_fp16)
+    double h16s = 1.0;      // 2^-m: 1/pot of the smoother reading g16
+    bool use16 = false;     // the smoother reads g16
     float* invm = nullptr;
     float* cpr = nullptr;
     float* lv_v = nullptr;  // segmented line-solve responses
@@ -335,11 +338,29 @@ int halo64(H* h, double* p, bool with_bc, bool fast = false) {
 // occupancy)
 int yline_mr(int m) { return (m == 4 || m == 8 || m == 16 || m == 32) ? m : 0; }
 
+// one smoother sweep with metrics m (MR = 0: the staged variant)
+template <int P, int MR, class T, class TM, class TR>
+void yline_go(H* h, Level& L, const Met<TM>& m, double inv_pot, const TR* rhs, int zin) {
+    const ColLaunch cl = seg_launch(L.G, P);
+    if constexpr (MR > 0)
+        hipLaunchKernelGGL((k_yline<P, MR, T, TM, TR>), cl.grid, cl.block, 0, h->s, L.G, m, line_f(L),
+                           (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->o.omega, (T)inv_pot, zin);
+    else
+        hipLaunchKernelGGL((k_yline_staged<P, T, TM, TR>), cl.grid, cl.block, 0, h->s, L.G, m, line_f(L),
+                           (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->o.omega, (T)inv_pot, zin);
+}
+
 template <int P, int MR, class T, class TR>
 void launch_yline_mr(H* h, Level& L, const TR* rhs, int zin) {
-    const ColLaunch cl = seg_launch(L.G, P);
-    hipLaunchKernelGGL((k_yline<P, MR, T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), line_f(L),
-                       (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->o.omega, (T)(1.0 / L.pot), zin);
+    if constexpr (sizeof(T) == 4) {
+        if (L.use16 && L.g16[0]) {  // 16-bit metrics scaled by 4^m: 1/pot scaled by 2^-m
+            Met<_Float16> m;
+            for (int q = 0; q < 9; ++q) m.g[q] = L.g16[q];
+            yline_go<P, MR, T, _Float16, TR>(h, L, m, L.h16s / L.pot, rhs, zin);
+            return;
+        }
+    }
+    yline_go<P, MR, T, T, TR>(h, L, met<T>(L), 1.0 / L.pot, rhs, zin);
 }
 
 template <int P, class T, class TR>
@@ -349,11 +370,7 @@ void launch_yline(H* h, Level& L, const TR* rhs, int zin) {
         case 8: launch_yline_mr<P, 8, T, TR>(h, L, rhs, zin); break;
         case 16: launch_yline_mr<P, 16, T, TR>(h, L, rhs, zin); break;
         case 32: launch_yline_mr<P, 32, T, TR>(h, L, rhs, zin); break;
-        default: {
-            const ColLaunch cl = seg_launch(L.G, P);
-            hipLaunchKernelGGL((k_yline_staged<P, T, T, TR>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L), line_f(L),
-                               (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->o.omega, (T)(1.0 / L.pot), zin);
-        }
+        default: launch_yline_mr<P, 0, T, TR>(h, L, rhs, zin); break;
     }
 }
 
@@ -362,7 +379,7 @@ bool reg_rows(int m) { return yline_mr(m) != 0; }
 
 template <class T, class TR>
 int smooth(H* h, Level& L, const TR* rhs, int nu, bool zero_start, bool fine) {
-    const double sm = sizeof(T), sp = sizeof(T), sr = sizeof(TR);
+    const double sm = (L.use16 && L.g16[0]) ? 2.0 : sizeof(T), sp = sizeof(T), sr = sizeof(TR);
     const double N = (double)L.ncell();
     const double seg = L.P > 1 ? 8.0 : 0.0;  // v, w reads of the last pass
     for (int s = 0; s < nu; ++s) {
@@ -988,10 +1005,12 @@ int level_factors(H* h, Level& L) {
 // precision for the life of the process, so every handle of a shape runs the
 // same kernels (the smoother's rounding depends on P; the residual's does not).
 struct TuneVal {
-    int P, rP, rbx;
+    int P, rP, rbx, use16;
 };
+// key: nx ny nz ip kp, FP32 level, fine level, 16-bit smoother metrics present
+using TuneKey = std::array<int, 8>;
 std::mutex g_tune_mu;
-std::map<std::array<int, 7>, TuneVal> g_tune;
+std::map<TuneKey, TuneVal> g_tune;
 
 bool valid_segments(int P) {
     return P == 1 || P == 2 || P == 3 || P == 4 || P == 6 || P == 8 || P == 12 || P == 16;
@@ -1031,7 +1050,8 @@ bool coarse_one_block(const Level& L);
 int tune_level(H* h, int n) {
     Level& L = h->lv[n];
     if (n == h->L - 1 && coarse_one_block(L)) return PLES_MG_OK;  // whole-level LDS solver
-    const std::array<int, 7> key{L.G.nx, L.G.ny, L.G.nz, L.G.ip, L.G.kp, L.f32 ? 1 : 0, n == 0 ? 1 : 0};
+    const bool half = L.g16[0] != nullptr;
+    const TuneKey key{L.G.nx, L.G.ny, L.G.nz, L.G.ip, L.G.kp, L.f32 ? 1 : 0, n == 0 ? 1 : 0, half ? 1 : 0};
     std::lock_guard<std::mutex> lock(g_tune_mu);
     const size_t cells = (size_t)L.G.n;
     auto ensure_vw = [&]() -> int {
@@ -1045,13 +1065,14 @@ int tune_level(H* h, int n) {
         L.P = it->second.P;
         L.rP = it->second.rP;
         L.rbx = it->second.rbx;
+        L.use16 = half && it->second.use16 != 0;
         return level_factors(h, L);
     }
     const double* r0 = h->rvec;  // level-0 rhs (any content: timing only)
     // the fixed rule's shape is the incumbent; another shape must be at least
     // 3% faster to replace it, which keeps choices stable between runs
     const int P0 = L.P;
-    TuneVal best{P0, 0, 0};
+    TuneVal best{P0, 0, 0, L.use16 ? 1 : 0};
     double tbest = 1e30;
     std::vector<int> cands = seg_candidates(L.G.ny);
     std::stable_partition(cands.begin(), cands.end(), [&](int P) { return P == P0; });
@@ -1068,6 +1089,20 @@ int tune_level(H* h, int n) {
     }
     L.P = best.P;
     CKS(level_factors(h, L));
+    if (half) {  // smoother metrics: the fixed rule's choice is the incumbent
+        tbest = 1e30;
+        const bool u0 = L.use16;
+        for (bool u : {u0, !u0}) {
+            L.use16 = u;
+            double ms = 0;
+            CKS(time_launches(h, 3, ms, [&] { return lv_smooth(h, n, r0, 2, false); }));
+            if (ms < 0.97 * tbest || tbest == 1e30) {
+                tbest = ms;
+                best.use16 = u ? 1 : 0;
+            }
+        }
+        L.use16 = best.use16 != 0;
+    }
     tbest = 1e30;
     const ColLaunch c0 = seg_launch(L.G, L.P);
     std::vector<std::pair<int, int>> rc{{L.P, (int)c0.block.x}};
@@ -1103,6 +1138,38 @@ int check_grid(const ples_mg_grid* g) {
     return PLES_MG_OK;
 }
 
+// 16-bit copy of an FP32 level's metrics for its smoother, scaled by 4^m so
+// that the largest normal metric lands in [2^8, 2^10): far from the FP16
+// overflow, and values down to 2^-24 of it stay normal numbers.  The scale
+// is exact and is taken back out through 1/pot (2^-m).
+int make_half_metrics(H* h, Level& L) {
+    const long cells = L.G.n;
+    const int nb = (int)std::min<long>((cells + 255) / 256, 1024);
+    double* part = nullptr;
+    CKS(dalloc(h, (void**)&part, (size_t)nb * 3 * sizeof(double)));
+    const int diag[3] = {0, 4, 8};
+    for (int d = 0; d < 3; ++d)
+        hipLaunchKernelGGL(k_absmax, dim3(nb), dim3(256), 0, h->s, cells, (const float*)L.g32[diag[d]], part + d * nb);
+    CK(hipGetLastError());
+    std::vector<double> hp((size_t)nb * 3);
+    CK(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, h->s));
+    CK(hipStreamSynchronize(h->s));
+    dfree(h, part);
+    double mx = 0.0;
+    for (double v : hp) mx = std::max(mx, v);
+    if (!(mx > 0.0) || !std::isfinite(mx)) return PLES_MG_ERR_NONFINITE;
+    const int e = (int)std::floor(std::log2(mx));
+    const int m = (int)std::floor((9.0 - (double)e) / 2.0);  // e + 2m in {8, 9}
+    const float scale = std::ldexp(1.0f, 2 * m);
+    L.h16s = std::ldexp(1.0, -m);
+    for (int q = 0; q < 9; ++q) {
+        CKS(dalloc(h, (void**)&L.g16[q], (size_t)cells * 2));
+        hipLaunchKernelGGL(k_to_half, dim3(nb), dim3(256), 0, h->s, cells, (const float*)L.g32[q], L.g16[q], scale);
+    }
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
 size_t estimate(const ples_mg_grid* g, const ples_mg_opts* o) {
     size_t tot = 0;
     const bool mixed = o && o->precision == PLES_MG_MIXED;
@@ -1112,6 +1179,7 @@ size_t estimate(const ples_mg_grid* g, const ples_mg_opts* o) {
         const bool f32 = mixed && n < g->nlevel - 1;
         const size_t sp = f32 ? 4 : 8;
         tot += cells * 9 * sp;                  // V-cycle metrics
+        if (f32 && o->smoother_fp16) tot += cells * 9 * 2;  // smoother's 16-bit copy
         if (n == 0 && f32) tot += cells * 9 * 8;  // FP64 outer metrics
         tot += cells * 16;                      // invm, cpr, v, w
         tot += cells * sp * 2;                  // ping-pong
@@ -1155,6 +1223,7 @@ void ples_mg_default_opts(ples_mg_opts* o) {
     o->inner_cycles = 0;  // auto
     o->autotune = 0;
     o->fp32_krylov = 1;
+    o->smoother_fp16 = 1;
 }
 
 const char* ples_mg_status_string(int s) {
@@ -1274,6 +1343,12 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
                                    (const double*)L.g64[q], L.g32[q]);
             }
             CK(hipGetLastError());
+            if (o.smoother_fp16) {
+                CKS(make_half_metrics(h, L));
+                // fixed rule: the fine level only (measured: faster there, slower on
+                // the smaller, latency-bound levels); the autotuner times both
+                L.use16 = n == 0;
+            }
             if (n > 0) {
                 CK(hipStreamSynchronize(h->s));
                 for (int q = 0; q < 9; ++q) {
@@ -2848,7 +2923,7 @@ int ples_mg_copy_bandwidth(int device, size_t bytes, int reps, double* gbs) {
 
 
 // ---- tuning table: one line per level shape
-//   nx ny nz ip kp f32 fine P rP rbx
+//   nx ny nz ip kp f32 fine half P rP rbx use16
 int ples_mg_tuning_load(const char* path) {
     if (!path) return PLES_MG_ERR_ARG;
     FILE* f = std::fopen(path, "r");
@@ -2858,11 +2933,20 @@ int ples_mg_tuning_load(const char* path) {
     int n = 0;
     while (std::fgets(line, sizeof line, f)) {
         if (line[0] == '#') continue;
-        std::array<int, 7> k{};
+        TuneKey k{};
         TuneVal v{};
-        if (std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d", &k[0], &k[1], &k[2], &k[3], &k[4], &k[5], &k[6],
-                        &v.P, &v.rP, &v.rbx) != 10)
+        // nx ny nz ip kp f32 fine half  P rP rbx use16; lines of ten fields
+        // (no half column, no use16) are read as half = use16 = 0
+        const int got = std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d", &k[0], &k[1], &k[2], &k[3], &k[4],
+                                    &k[5], &k[6], &k[7], &v.P, &v.rP, &v.rbx, &v.use16);
+        if (got == 10) {
+            k[7] = 0;
+            v.use16 = 0;
+            if (std::sscanf(line, "%*d %*d %*d %*d %*d %*d %*d %d %d %d", &v.P, &v.rP, &v.rbx) != 3) continue;
+        } else if (got != 12) {
             continue;
+        }
+        if ((v.use16 != 0 && v.use16 != 1) || (k[7] != 0 && k[7] != 1) || (v.use16 && !k[7])) continue;
         const bool okP = valid_segments(v.P) && k[1] % v.P == 0;
         const bool okR = valid_segments(v.rP) && k[1] % v.rP == 0 && v.rbx >= 1 && (v.rbx & (v.rbx - 1)) == 0 &&
                          v.rbx <= 256 && v.rbx * v.rP <= 512;
@@ -2879,11 +2963,11 @@ int ples_mg_tuning_save(const char* path) {
     FILE* f = std::fopen(path, "w");
     if (!f) return PLES_MG_ERR_ARG;
     std::lock_guard<std::mutex> lock(g_tune_mu);
-    std::fprintf(f, "# nx ny nz ip kp f32 fine  P rP rbx  (launch shapes chosen by the autotuner)\n");
+    std::fprintf(f, "# nx ny nz ip kp f32 fine half  P rP rbx use16  (launch shapes chosen by the autotuner)\n");
     for (const auto& kv : g_tune) {
         const auto& k = kv.first;
-        std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d\n", k[0], k[1], k[2], k[3], k[4], k[5], k[6], kv.second.P,
-                     kv.second.rP, kv.second.rbx);
+        std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d %d\n", k[0], k[1], k[2], k[3], k[4], k[5], k[6], k[7],
+                     kv.second.P, kv.second.rP, kv.second.rbx, kv.second.use16);
     }
     std::fclose(f);
     return PLES_MG_OK;

@@ -33,7 +33,7 @@ def test_status_strings_and_defaults():
     assert "dimensions" in mg.status_string(4)
     o = mg.default_opts()
     assert o["precision"] == mg.FP64 and o["krylov"] == 1 and o["nu_pre"] == 3 and o["nu_post"] == 3
-    assert o["graphs"] == 1 and o["fp32_krylov"] == 1 and o["autotune"] == 0
+    assert o["graphs"] == 1 and o["fp32_krylov"] == 1 and o["autotune"] == 0 and o["smoother_fp16"] == 1
 
 
 def test_tuning_table_parses():
@@ -44,7 +44,11 @@ def test_tuning_table_parses():
     for line in open(mg.TUNING_TABLE):
         if line.startswith("#"):
             continue
-        nx, ny, nz, ip, kp, f32, fine, P, rP, rbx = map(int, line.split())
+        f = list(map(int, line.split()))
+        if len(f) == 10:  # older layout: no 16-bit smoother metrics
+            f = f[:7] + [0] + f[7:] + [0]
+        nx, ny, nz, ip, kp, f32, fine, half, P, rP, rbx, use16 = f
         assert ny % P == 0 and ny % rP == 0 and rbx * rP <= 512
+        assert use16 in (0, 1) and half in (0, 1) and use16 <= half and half <= f32
         n += 1
     assert n > 20
