@@ -1120,6 +1120,67 @@ __device__ inline void halo_edge_point(const Geo& G, T* __restrict__ p, long t) 
     p[G.idx(i, j, k)] = p[G.idx(si, sj, sk)];
 }
 
+// Eta-wall ghosts of a level periodic in xi and zeta (no other walls).  The
+// face relation of fsrc/multigrid.f03:1161-1264 couples a wall's ghosts only
+// along the wall plane: g = f + cA (g(i+1) - g(i-1)) + cB (g(k+1) - g(k-1)),
+// f the part from the interior row and the wall flux.  One block owns a
+// TX x TK tile of one wall and carries a halo of 2*passes points each side
+// (periodic wrap), runs `passes` red/black passes on it in LDS and writes its
+// tile with the images: the same passes as the launch-per-colour fill, in one
+// launch.  Dynamic LDS: 4 (TX + 4 passes)(TK + 4 passes) values of T.
+template <class T, class TM, int TX, int TK>
+__global__ void __launch_bounds__(256) k_eta_ghosts(Geo G, Met<TM> M, T* __restrict__ p, BCs bc, int passes) {
+    extern __shared__ __align__(16) unsigned char eta_lds[];
+    const int Hw = 2 * passes, WX = TX + 2 * Hw, WK = TK + 2 * Hw, NW = WX * WK;
+    T* sf = (T*)eta_lds;
+    T* sa = sf + NW;
+    T* sb = sa + NW;
+    T* sg = sb + NW;
+    const int tx = (G.nx + TX - 1) / TX, tk = (G.nz + TK - 1) / TK;
+    const int side = blockIdx.x / (tx * tk), rem = blockIdx.x % (tx * tk);
+    const int ti = (rem % tx) * TX, tkk = (rem / tx) * TK;  // tile origin, 0-based
+    const int jr = side == 0 ? 1 : G.ny;                    // interior row at the wall
+    const int jg = side == 0 ? 0 : G.ny + 1;                // ghost row
+    const int jm = side == 0 ? 0 : G.ny;                    // row of the wall-face metrics
+    const T sgn = side == 0 ? (T)1 : (T)-1;
+    const T q = (T)0.25;
+    const double an = bc.an_over_dt;
+    const double* cs = bc.cs[2 + side];
+    auto wrap = [](int a, int n) { return ((a % n) + n) % n + 1; };  // 0-based, any sign -> 1..n
+    for (int t = threadIdx.x; t < NW; t += blockDim.x) {
+        const int i = wrap(ti + (t % WX) - Hw, G.nx), k = wrap(tkk + (t / WX) - Hw, G.nz);
+        const long cm = G.idx(i, jm, k);
+        const T inv = (T)1 / (T)M.g[4][cm];
+        const T cA = sgn * q * inv * (T)M.g[3][cm], cB = sgn * q * inv * (T)M.g[5][cm];
+        const T csv = cs ? (T)cs[(i - 1) + (long)G.nx * (k - 1)] : (T)0;
+        const int ip1 = i == G.nx ? 1 : i + 1, im1 = i == 1 ? G.nx : i - 1;
+        const int kp1 = k == G.nz ? 1 : k + 1, km1 = k == 1 ? G.nz : k - 1;
+        sf[t] = (T)p[G.idx(i, jr, k)] - sgn * (T)an * csv * inv +
+                cA * ((T)p[G.idx(ip1, jr, k)] - (T)p[G.idx(im1, jr, k)]) +
+                cB * ((T)p[G.idx(i, jr, kp1)] - (T)p[G.idx(i, jr, km1)]);
+        sa[t] = cA;
+        sb[t] = cB;
+        sg[t] = (T)p[G.idx(i, jg, k)];
+    }
+    __syncthreads();
+    for (int ps = 0; ps < passes; ++ps)
+        for (int color = 0; color < 2; ++color) {
+            for (int t = threadIdx.x; t < NW; t += blockDim.x) {
+                const int wi = t % WX, wk = t / WX;
+                if (wi == 0 || wi == WX - 1 || wk == 0 || wk == WK - 1 || ((wi + wk) & 1) != color) continue;
+                sg[t] = sf[t] + sa[t] * (sg[t + 1] - sg[t - 1]) + sb[t] * (sg[t + WX] - sg[t - WX]);
+            }
+            __syncthreads();
+        }
+    for (int t = threadIdx.x; t < TX * TK; t += blockDim.x) {
+        const int i = ti + (t % TX) + 1, k = tkk + (t / TX) + 1;
+        if (i > G.nx || k > G.nz) continue;
+        const T v = sg[(t % TX + Hw) + WX * (t / TX + Hw)];
+        p[G.idx(i, jg, k)] = v;
+        write_images(G, p, i, jg, k, v);
+    }
+}
+
 template <class T, class TM>
 __global__ void k_halo_face(Geo G, Met<TM> M, T* __restrict__ p, int color, BCs bc, int set, int images) {
     halo_face_point(G, M, p, (long)blockIdx.x * blockDim.x + threadIdx.x, color, bc, set, images);

@@ -266,12 +266,40 @@ long face_set_size(const Geo& G, int set) {
     return G.kp == 1 ? 2L * G.nx * G.ny : 0L;
 }
 
+// Eta-wall ghosts of a level periodic in xi and zeta by one k_eta_ghosts
+// launch (the passes of every tile in LDS); false where that does not apply
+// (xi or zeta walls, or a tile halo beyond 64 KB of LDS).
+template <class T, class TM>
+bool eta_ghosts(H* h, const Geo& G, const Met<TM>& m, T* p, const BCs& bc, int passes) {
+    if (G.ip != 0 || G.kp != 0 || G.jp != 1 || passes < 1) return false;
+    constexpr int TX = 32, TK = 8;
+    const int hw = 2 * passes;
+    const size_t lds = (size_t)(TX + 2 * hw) * (TK + 2 * hw) * 4 * sizeof(T);
+    if (lds > 65536) return false;
+    const int nb = 2 * ((G.nx + TX - 1) / TX) * ((G.nz + TK - 1) / TK);
+    hipLaunchKernelGGL((k_eta_ghosts<T, TM, TX, TK>), dim3(nb), dim3(256), lds, h->s, G, m, p, bc, passes);
+    return true;
+}
+
+// interior_images: the periodic images of the interior are already valid
+// (a smoother output), only the wall ghosts and their images are needed
 template <class T>
-int halo(H* h, const Level& L, T* p, double an_over_dt, bool with_cs, int passes) {
+int halo(H* h, const Level& L, T* p, double an_over_dt, bool with_cs, int passes, bool interior_images = false) {
     const Geo& G = L.G;
     BCs bc;
     for (int q = 0; q < 6; ++q) bc.cs[q] = (with_cs && h->have_cs) ? h->cs[q] : nullptr;
     bc.an_over_dt = an_over_dt;
+    {
+        const long NX = G.nx + 2, NY = G.ny + 2, NZ = G.nz + 2;
+        const long ne = 2 * NY * NZ + 2 * NX * NZ + 2 * NX * NY;
+        KT kt(h, PLES_MG_K_HALO, (double)(2L * G.nx * G.nz * 8 + (interior_images ? 0 : ne * 2)) * sizeof(T));
+        if (eta_ghosts<T, T>(h, G, met<T>(L), p, bc, passes)) {
+            if (!interior_images)
+                hipLaunchKernelGGL((k_halo_edges<T>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, h->s, G, p);
+            CK(hipGetLastError());
+            return PLES_MG_OK;
+        }
+    }
     long nf = 2L * G.nx * G.nz;
     if (G.ip == 1) nf += 2L * G.ny * G.nz;
     if (G.kp == 1) nf += 2L * G.nx * G.ny;
@@ -314,6 +342,16 @@ int halo64(H* h, double* p, bool with_bc, bool fast = false) {
     // every pass counts: the outer GCR needs the ghosts at their fixed point
     // to machine precision, or the attainable residual stalls
     const int passes = std::max(1, h->o.halo_passes);
+    {
+        KT kt(h, PLES_MG_K_HALO, (double)(2L * G.nx * G.nz * 8 + (fast ? 0 : ne * 2)) * 8.0);
+        if (eta_ghosts<double, double>(h, G, m, p, bc, passes)) {
+            if (!fast)
+                hipLaunchKernelGGL((k_halo_edges<double>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, h->s, G,
+                                   p);
+            CK(hipGetLastError());
+            return PLES_MG_OK;
+        }
+    }
     KT kt(h, PLES_MG_K_HALO, (double)(nf * 6 + (fast ? 0 : ne * 2)) * passes * 8.0);
     for (int ps = 0; ps < passes; ++ps) {
         for (int set = 0; set < 3; ++set) {
@@ -703,7 +741,7 @@ int step32_body(H* h, int q) {
         hipLaunchKernelGGL((k_convert<float, float>), dim3(nb), dim3(256), 0, h->s, n, (const float*)L0.p[L0.cur],
                            Zq);
     }
-    CKS(halo<float>(h, L0, Zq, 0.0, false, std::max(1, h->o.halo_passes)));
+    CKS(halo<float>(h, L0, Zq, 0.0, false, std::max(1, h->o.halo_passes), true));
     const double N0 = (double)L0.ncell();
     const bool kry = h->o.krylov != 0;
     {

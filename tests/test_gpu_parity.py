@@ -210,12 +210,13 @@ def test_tuned_launch_shapes_same_solution():
             assert r.ok
             out.append(s.get_fi())
             if tune:
+                # nx ny nz ip kp f32 fine half -> P rP rbx use16
                 table = {}
                 for line in open(mg.TUNING_TABLE):
                     if not line.startswith("#"):
                         v = list(map(int, line.split()))
-                        table[tuple(v[:7])] = tuple(v[7:])
-                assert s.tiles(1) == table[(64, 32, 64, 0, 0, 1, 1)]
+                        table[tuple(v[:8])] = tuple(v[8:11])
+                assert s.tiles(1) == table[(64, 32, 64, 0, 0, 1, 1, 1)]
     assert cases.rel_l2_meanfree(out[1], out[0]) < 1e-9
 
 
