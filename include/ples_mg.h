@@ -98,6 +98,10 @@ typedef struct ples_mg_opts {
                         solution.  Used on the fine level (the autotuner may
                         choose per level).  0 = FP32 metrics everywhere.
                         Default 1 */
+    int coarse_exact; /* 1 = a coarsest level of at most 512 cells is solved
+                        exactly (a dense map built at create from the level's
+                        operator with its wall ghosts eliminated, applied as one
+                        matrix-vector product; default); 0 = nu_coarse sweeps */
 } ples_mg_opts;
 
 typedef struct ples_mg_result {

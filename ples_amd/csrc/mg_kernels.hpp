@@ -75,7 +75,7 @@ struct Win {
 };
 
 template <class T, class P>
-__device__ inline void win_init(Win<T>& w, const P* __restrict__ p, long c0, long sy, long sz) {
+__host__ __device__ inline void win_init(Win<T>& w, const P* __restrict__ p, long c0, long sy, long sz) {
     // c0 = idx(i,0,k): rows 0 (j-1 of j=1) and 1 (j)
     w.pm = (T)p[c0];
     w.p0 = (T)p[c0 + sy];
@@ -92,7 +92,7 @@ __device__ inline void win_init(Win<T>& w, const P* __restrict__ p, long c0, lon
 // load row j+1 of the five face columns and row j of the four diagonals;
 // c = idx(i,j,k)
 template <class T, class P>
-__device__ inline void win_load(Win<T>& w, const P* __restrict__ p, long c, long sy, long sz) {
+__host__ __device__ inline void win_load(Win<T>& w, const P* __restrict__ p, long c, long sy, long sz) {
     w.pp = (T)p[c + sy];
     w.xmp = (T)p[c - 1 + sy];
     w.xpp = (T)p[c + 1 + sy];
@@ -105,7 +105,7 @@ __device__ inline void win_load(Win<T>& w, const P* __restrict__ p, long c, long
 }
 
 template <class T>
-__device__ inline void win_shift(Win<T>& w) {
+__host__ __device__ inline void win_shift(Win<T>& w) {
     w.pm = w.p0; w.p0 = w.pp;
     w.xmm = w.xm0; w.xm0 = w.xmp;
     w.xpm = w.xp0; w.xp0 = w.xpp;
@@ -122,7 +122,7 @@ struct G2 {
 // s * sum of face-flux differences at (i,j,k), the reference expression
 // of fsrc/resid_sndrcv.f03:85-147 (without the "- rh").
 template <class T, class TM>
-__device__ inline T stencil(const Win<T>& w, const Met<TM>& M, long c, long sz, const G2<T>& gm,
+__host__ __device__ inline T stencil(const Win<T>& w, const Met<TM>& M, long c, long sz, const G2<T>& gm,
                             const G2<T>& g0, T inv_pot) {
     const T q = (T)0.25;
     const T g11 = (T)M.g[0][c], g12 = (T)M.g[1][c], g13 = (T)M.g[2][c];
@@ -153,7 +153,7 @@ __device__ inline T stencil(const Win<T>& w, const Met<TM>& M, long c, long sz, 
 }
 
 template <class T, class TM>
-__device__ inline G2<T> load_g2(const Met<TM>& M, long c) {
+__host__ __device__ inline G2<T> load_g2(const Met<TM>& M, long c) {
     return G2<T>{(T)M.g[3][c], (T)M.g[4][c], (T)M.g[5][c]};
 }
 
@@ -165,7 +165,7 @@ __device__ inline bool column_of(const Geo& G, int& i, int& k) {
 
 // write value v of row j of column (i,k) also to its periodic images
 template <class T>
-__device__ inline void write_images(const Geo& G, T* __restrict__ p, int i, int j, int k, T v) {
+__host__ __device__ inline void write_images(const Geo& G, T* __restrict__ p, int i, int j, int k, T v) {
     int ia[3], ka[3];
     int ni = 0, nk = 0;
     ia[ni++] = i;
@@ -926,14 +926,14 @@ struct BCs {
     double an_over_dt;    // an / dt
 };
 
-__device__ inline int wrapi(const Geo& G, int i) {
+__host__ __device__ inline int wrapi(const Geo& G, int i) {
     if (G.ip == 0) {
         if (i == 0) return G.nx;
         if (i == G.nx + 1) return 1;
     }
     return i;
 }
-__device__ inline int wrapk(const Geo& G, int k) {
+__host__ __device__ inline int wrapk(const Geo& G, int k) {
     if (G.kp == 0) {
         if (k == 0) return G.nz;
         if (k == G.nz + 1) return 1;
@@ -944,7 +944,7 @@ __device__ inline int wrapk(const Geo& G, int k) {
 // one launch per face set (0 eta, 1 xi, 2 zeta) and colour, so that no
 // ghost read in a launch is written by the same launch
 template <class T, class TM>
-__device__ inline void halo_face_point(const Geo& G, const Met<TM>& M, T* __restrict__ p, long t, int color,
+__host__ __device__ inline void halo_face_point(const Geo& G, const Met<TM>& M, T* __restrict__ p, long t, int color,
                                        const BCs& bc, int set, int images) {
     if (set == 1) t += 2L * G.nx * G.nz;
     if (set == 2) t += 2L * G.nx * G.nz + (G.ip == 1 ? 2L * G.ny * G.nz : 0L);
@@ -999,21 +999,23 @@ __device__ inline void halo_face_point(const Geo& G, const Met<TM>& M, T* __rest
                 const long cb = G.idx(0, j, k);
                 const T inv = (T)1 / (T)M.g[0][cb];
                 const T csv = bc.cs[0] ? (T)bc.cs[0][(j - 1) + (long)G.ny * (k - 1)] : (T)0;
-                p[cb] = PV(1, j, k) - (T)an * csv * inv +
+                const T val = PV(1, j, k) - (T)an * csv * inv +
                         q * inv *
                             ((T)M.g[1][cb] * (PV(1, j + 1, k) + PV(0, j + 1, k) - PV(1, j - 1, k) - PV(0, j - 1, k)) +
                              (T)M.g[2][cb] * (PV(1, j, k + 1) + PV(0, j, k + 1) - PV(1, j, k - 1) - PV(0, j, k - 1)));
+                p[cb] = val;
             } else {
                 const int nx = G.nx;
                 const long ct = G.idx(nx, j, k);
                 const T inv = (T)1 / (T)M.g[0][ct];
                 const T csv = bc.cs[1] ? (T)bc.cs[1][(j - 1) + (long)G.ny * (k - 1)] : (T)0;
-                p[ct + 1] = PV(nx, j, k) + (T)an * csv * inv -
+                const T val = PV(nx, j, k) + (T)an * csv * inv -
                             q * inv *
                                 ((T)M.g[1][ct] * (PV(nx + 1, j + 1, k) + PV(nx, j + 1, k) - PV(nx + 1, j - 1, k) -
                                                   PV(nx, j - 1, k)) +
                                  (T)M.g[2][ct] * (PV(nx + 1, j, k + 1) + PV(nx, j, k + 1) - PV(nx + 1, j, k - 1) -
                                                   PV(nx, j, k - 1)));
+                p[ct + 1] = val;
             }
             return;
         }
@@ -1031,21 +1033,23 @@ __device__ inline void halo_face_point(const Geo& G, const Met<TM>& M, T* __rest
                 const long cb = G.idx(i, j, 0);
                 const T inv = (T)1 / (T)M.g[8][cb];
                 const T csv = bc.cs[4] ? (T)bc.cs[4][(i - 1) + (long)G.nx * (j - 1)] : (T)0;
-                p[cb] = PV(i, j, 1) - (T)an * csv * inv +
+                const T val = PV(i, j, 1) - (T)an * csv * inv +
                         q * inv *
                             ((T)M.g[6][cb] * (PV(i + 1, j, 1) + PV(i + 1, j, 0) - PV(i - 1, j, 1) - PV(i - 1, j, 0)) +
                              (T)M.g[7][cb] * (PV(i, j + 1, 1) + PV(i, j + 1, 0) - PV(i, j - 1, 1) - PV(i, j - 1, 0)));
+                p[cb] = val;
             } else {
                 const int nz = G.nz;
                 const long ct = G.idx(i, j, nz);
                 const T inv = (T)1 / (T)M.g[8][ct];
                 const T csv = bc.cs[5] ? (T)bc.cs[5][(i - 1) + (long)G.nx * (j - 1)] : (T)0;
-                p[ct + G.sz] = PV(i, j, nz) + (T)an * csv * inv -
+                const T val = PV(i, j, nz) + (T)an * csv * inv -
                                q * inv *
                                    ((T)M.g[6][ct] * (PV(i + 1, j, nz + 1) + PV(i + 1, j, nz) - PV(i - 1, j, nz + 1) -
                                                      PV(i - 1, j, nz)) +
                                     (T)M.g[7][ct] * (PV(i, j + 1, nz + 1) + PV(i, j + 1, nz) - PV(i, j - 1, nz + 1) -
                                                      PV(i, j - 1, nz)));
+                p[ct + G.sz] = val;
             }
         }
     }
@@ -1055,7 +1059,7 @@ __device__ inline void halo_face_point(const Geo& G, const Met<TM>& M, T* __rest
 // Source of a non-face ghost position: periodic wrap first, then the
 // wall-wall edge rules (fsrc/multigrid.f03:1640-1738 for kp = 1,
 // 1358-1635 for kp = 0).  Returns false for interior / face positions.
-__device__ inline bool ghost_source(const Geo& G, int i, int j, int k, int& si, int& sj, int& sk) {
+__host__ __device__ inline bool ghost_source(const Geo& G, int i, int j, int k, int& si, int& sj, int& sk) {
     const bool gi = (i == 0 || i == G.nx + 1), gj = (j == 0 || j == G.ny + 1),
                gk = (k == 0 || k == G.nz + 1);
     const int ng = (int)gi + (int)gj + (int)gk;
@@ -1087,7 +1091,7 @@ __device__ inline bool ghost_source(const Geo& G, int i, int j, int k, int& si, 
 }
 
 template <class T>
-__device__ inline void halo_edge_point(const Geo& G, T* __restrict__ p, long t) {
+__host__ __device__ inline void halo_edge_point(const Geo& G, T* __restrict__ p, long t) {
     // every ghost position: enumerate the six ghost planes of the box, each
     // position counted once by giving planes i-ghost > j-ghost > k-ghost
     const long NX = G.nx + 2, NY = G.ny + 2, NZ = G.nz + 2;
@@ -1189,6 +1193,25 @@ __global__ void k_halo_face(Geo G, Met<TM> M, T* __restrict__ p, int color, BCs 
 template <class T>
 __global__ void k_halo_edges(Geo G, T* __restrict__ p) {
     halo_edge_point(G, p, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Exact coarsest-level solve: p (every ghosted position) = K b, K the dense
+// map built at set-up (rows: ghosted positions, columns: interior cells, i
+// fastest), b the interior of the ghosted rhs.  One wavefront per row, lanes
+// stride the columns, fixed shuffle tree: deterministic.
+__global__ void __launch_bounds__(256) k_coarse_dense(Geo G, const double* __restrict__ K, int N,
+                                                      const double* __restrict__ b, double* __restrict__ p) {
+    const int lane = threadIdx.x & 63;
+    const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= G.n) return;  // whole wavefronts leave together
+    const double* kr = K + row * (long)N;
+    double acc = 0.0;
+    for (int c = lane; c < N; c += 64) {
+        const int i = c % G.nx + 1, j = (c / G.nx) % G.ny + 1, k = c / (G.nx * G.ny) + 1;
+        acc += kr[c] * b[G.idx(i, j, k)];
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) p[row] = acc;
 }
 
 

@@ -31,6 +31,7 @@ struct Level {
     _Float16* g16[9] = {};  // smoother copy of g32 * 4^m (FP32 levels, smoother_fp16)
     double h16s = 1.0;      // 2^-m: 1/pot of the smoother reading g16
     bool use16 = false;     // the smoother reads g16
+    double* kd = nullptr;   // coarsest level: dense exact solve map (ghosted x interior)
     float* invm = nullptr;
     float* cpr = nullptr;
     float* lv_v = nullptr;  // segmented line-solve responses
@@ -528,6 +529,9 @@ int lv_prolong(H* h, int n) {  // n+1 -> n
 
 // coarsest level in one launch of one workgroup when its columns fit a block
 // and it has no xi/zeta walls (whose ghosts need the separate halo launches)
+// coarsest levels up to this many cells get the exact dense solve
+constexpr long kCoarseDenseMax = 512;
+
 bool coarse_one_block(const Level& L) {
     return !L.f32 && L.G.ip == 0 && L.G.kp == 0 && (long)L.G.nx * L.G.nz <= 1024;
 }
@@ -535,6 +539,14 @@ bool coarse_one_block(const Level& L) {
 size_t coarse_lds_bytes(const Level& L) { return (size_t)L.G.n * (12 * sizeof(double) + 2 * sizeof(float)); }
 
 int coarse_solve(H* h, Level& L, const double* rhs, int nu) {
+    if (L.kd) {  // exact: one matrix-vector product gives the ghosted solution
+        KT kt(h, PLES_MG_K_SMOOTH, (double)L.G.n * L.ncell() * 8.0);
+        hipLaunchKernelGGL(k_coarse_dense, dim3((unsigned)((L.G.n + 3) / 4)), dim3(256), 0, h->s, L.G,
+                           (const double*)L.kd, (int)L.ncell(), rhs, (double*)L.p[1 - L.cur]);
+        CK(hipGetLastError());
+        L.cur ^= 1;
+        return PLES_MG_OK;
+    }
     const int ncol = L.G.nx * L.G.nz;
     KT kt(h, PLES_MG_K_SMOOTH, (double)L.ncell() * (9 * 8 + 8 + 8 + 8 + 8));
     const size_t lds = coarse_lds_bytes(L);
@@ -562,7 +574,7 @@ int vcycle(H* h, const double* r0, bool fine_zero = true) {
     for (int n = 0; n < L; ++n) {
         const int nu = (n == L - 1) ? h->o.nu_coarse : h->o.nu_pre;
         Level& lvn = h->lv[n];
-        if (n == L - 1 && coarse_one_block(lvn)) {
+        if (n == L - 1 && (lvn.kd || coarse_one_block(lvn))) {
             CKS(coarse_solve(h, lvn, n == 0 ? r0 : (const double*)lvn.b, std::max(1, nu)));
             continue;
         }
@@ -1087,7 +1099,7 @@ bool coarse_one_block(const Level& L);
 
 int tune_level(H* h, int n) {
     Level& L = h->lv[n];
-    if (n == h->L - 1 && coarse_one_block(L)) return PLES_MG_OK;  // whole-level LDS solver
+    if (n == h->L - 1 && (L.kd || coarse_one_block(L))) return PLES_MG_OK;  // one-launch coarse solver
     const bool half = L.g16[0] != nullptr;
     const TuneKey key{L.G.nx, L.G.ny, L.G.nz, L.G.ip, L.G.kp, L.f32 ? 1 : 0, n == 0 ? 1 : 0, half ? 1 : 0};
     std::lock_guard<std::mutex> lock(g_tune_mu);
@@ -1176,6 +1188,116 @@ int check_grid(const ples_mg_grid* g) {
     return PLES_MG_OK;
 }
 
+// ---------------------------------------------------------------- exact coarse solve
+// The coarsest level's operator as a dense matrix, built on the host with the
+// device's own stencil and ghost rules (compiled for both): column m is the
+// 19-point operator applied to the unit vector e_m whose wall ghosts sit at
+// the fixed point of the homogeneous face relations and edge rules (the
+// coarse-level boundary condition).  The operator is singular (constants);
+// the flux form sums to zero over the box, so 1 is also its left null vector
+// and the system bordered with the zero-mean constraint, [[A, 1], [1^T, 0]],
+// is regular.  Its inverse's N x N block K maps a rhs b to the zero-mean
+// solution of A p = b - mean(b).  Stored: E K, E the map from the interior to
+// the ghosted vector (ghosts by the same fixed point), so one matrix-vector
+// product gives the whole ghosted solution.
+int build_coarse_dense(H* h, Level& L) {
+    const Geo& G = L.G;
+    const long N = L.ncell(), ng = G.n;
+    std::vector<std::vector<double>> gm(9, std::vector<double>((size_t)ng));
+    for (int q = 0; q < 9; ++q)
+        CK(hipMemcpyAsync(gm[q].data(), L.g64[q], (size_t)ng * 8, hipMemcpyDeviceToHost, h->s));
+    CK(hipStreamSynchronize(h->s));
+    Met<double> M;
+    for (int q = 0; q < 9; ++q) M.g[q] = gm[q].data();
+    BCs bc0;
+    for (int q = 0; q < 6; ++q) bc0.cs[q] = nullptr;
+    bc0.an_over_dt = 0.0;
+    const long NX = G.nx + 2, NY = G.ny + 2, NZ = G.nz + 2;
+    const long ne = 2 * NY * NZ + 2 * NX * NZ + 2 * NX * NY;
+    auto cell = [&](long c) { return G.idx((int)(c % G.nx) + 1, (int)((c / G.nx) % G.ny) + 1, (int)(c / ((long)G.nx * G.ny)) + 1); };
+    std::vector<double> A((size_t)(N * N), 0.0);   // column-major
+    std::vector<double> E((size_t)(ng * N), 0.0);  // column-major: ghosted image of e_m
+    std::vector<double> v((size_t)ng), prev((size_t)ng);
+    const double inv_pot = 1.0 / L.pot;
+    for (long m = 0; m < N; ++m) {
+        std::fill(v.begin(), v.end(), 0.0);
+        v[(size_t)cell(m)] = 1.0;
+        for (int pass = 0; pass < 400; ++pass) {
+            prev = v;
+            for (int set = 0; set < 3; ++set) {
+                const long nset = face_set_size(G, set);
+                for (int color = 0; color < 2; ++color)
+                    for (long t = 0; t < nset; ++t) halo_face_point(G, M, v.data(), t, color, bc0, set, 0);
+            }
+            for (long t = 0; t < ne; ++t) halo_edge_point(G, v.data(), t);
+            double d = 0.0;
+            for (long t = 0; t < ng; ++t) d = std::max(d, std::fabs(v[(size_t)t] - prev[(size_t)t]));
+            if (d <= 1e-17) break;  // unit-scale vector: converged to rounding
+        }
+        std::copy(v.begin(), v.end(), E.begin() + (size_t)(m * ng));
+        for (long r = 0; r < N; ++r) {
+            const long c = cell(r);
+            Win<double> w;
+            win_init(w, v.data(), c - G.sy, G.sy, G.sz);
+            win_load(w, v.data(), c, G.sy, G.sz);
+            const G2<double> g0 = load_g2<double>(M, c), gmv = load_g2<double>(M, c - G.sy);
+            A[(size_t)(r + N * m)] = stencil(w, M, c, G.sz, gmv, g0, inv_pot);
+        }
+    }
+    // Gauss-Jordan inverse of the bordered matrix, partial pivoting
+    const long nb = N + 1;
+    std::vector<double> B((size_t)(nb * nb), 0.0), I((size_t)(nb * nb), 0.0);  // row-major
+    for (long r = 0; r < N; ++r) {
+        for (long c = 0; c < N; ++c) B[(size_t)(r * nb + c)] = A[(size_t)(r + N * c)];
+        B[(size_t)(r * nb + N)] = 1.0;
+        B[(size_t)(N * nb + r)] = 1.0;
+    }
+    for (long r = 0; r < nb; ++r) I[(size_t)(r * nb + r)] = 1.0;
+    for (long col = 0; col < nb; ++col) {
+        long piv = col;
+        for (long r = col + 1; r < nb; ++r)
+            if (std::fabs(B[(size_t)(r * nb + col)]) > std::fabs(B[(size_t)(piv * nb + col)])) piv = r;
+        if (!(std::fabs(B[(size_t)(piv * nb + col)]) > 0.0)) return PLES_MG_ERR_NONFINITE;
+        if (piv != col)
+            for (long c = 0; c < nb; ++c) {
+                std::swap(B[(size_t)(piv * nb + c)], B[(size_t)(col * nb + c)]);
+                std::swap(I[(size_t)(piv * nb + c)], I[(size_t)(col * nb + c)]);
+            }
+        const double d = 1.0 / B[(size_t)(col * nb + col)];
+        for (long c = 0; c < nb; ++c) {
+            B[(size_t)(col * nb + c)] *= d;
+            I[(size_t)(col * nb + c)] *= d;
+        }
+        for (long r = 0; r < nb; ++r) {
+            if (r == col) continue;
+            const double f = B[(size_t)(r * nb + col)];
+            if (f == 0.0) continue;
+            for (long c = 0; c < nb; ++c) {
+                B[(size_t)(r * nb + c)] -= f * B[(size_t)(col * nb + c)];
+                I[(size_t)(r * nb + c)] -= f * I[(size_t)(col * nb + c)];
+            }
+        }
+    }
+    // K_full = E K (row-major, ghosted rows x interior columns)
+    std::vector<double> KF((size_t)(ng * N), 0.0);
+    for (long m = 0; m < N; ++m) {
+        const double* e = E.data() + (size_t)(m * ng);
+        const double* krow = I.data() + (size_t)(m * nb);  // row m of K
+        for (long g = 0; g < ng; ++g) {
+            const double em = e[(size_t)g];
+            if (em == 0.0) continue;
+            double* out = KF.data() + (size_t)(g * N);
+            for (long c = 0; c < N; ++c) out[(size_t)c] += em * krow[(size_t)c];
+        }
+    }
+    for (double x : KF)
+        if (!std::isfinite(x)) return PLES_MG_ERR_NONFINITE;
+    CKS(dalloc(h, (void**)&L.kd, (size_t)(ng * N) * 8));
+    CK(hipMemcpyAsync(L.kd, KF.data(), (size_t)(ng * N) * 8, hipMemcpyHostToDevice, h->s));
+    CK(hipStreamSynchronize(h->s));
+    return PLES_MG_OK;
+}
+
 // 16-bit copy of an FP32 level's metrics for its smoother, scaled by 4^m so
 // that the largest normal metric lands in [2^8, 2^10): far from the FP16
 // overflow, and values down to 2^-24 of it stay normal numbers.  The scale
@@ -1223,6 +1345,8 @@ size_t estimate(const ples_mg_grid* g, const ples_mg_opts* o) {
         tot += cells * sp * 2;                  // ping-pong
         if (n > 0) tot += cells * sp;           // rhs
         if (n < g->nlevel - 1) tot += cells * sp;  // residual
+        const size_t ni = (size_t)(g->jx / st) * (g->jy / st) * (g->jz / st);
+        if (n == g->nlevel - 1 && o && o->coarse_exact && (long)ni <= kCoarseDenseMax) tot += cells * ni * 8;
     }
     const size_t c0 = (size_t)(g->jx + 2) * (g->jy + 2) * (g->jz + 2);
     tot += c0 * 8 * 7;                                             // x, b, r, Z[2], W[2]
@@ -1262,6 +1386,7 @@ void ples_mg_default_opts(ples_mg_opts* o) {
     o->autotune = 0;
     o->fp32_krylov = 1;
     o->smoother_fp16 = 1;
+    o->coarse_exact = 1;
 }
 
 const char* ples_mg_status_string(int s) {
@@ -1403,6 +1528,7 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
             CKS(dalloc(h, (void**)&L.lv_w, cells * 4));
         }
         CKS(level_factors(h, L));
+        if (n == h->L - 1 && o.coarse_exact && L.ncell() <= (long)kCoarseDenseMax) CKS(build_coarse_dense(h, L));
         const size_t sp = L.f32 ? 4 : 8;
         CKS(dalloc(h, &L.p[0], cells * sp));
         CKS(dalloc(h, &L.p[1], cells * sp));
