@@ -69,7 +69,9 @@ typedef struct ples_mg_opts {
     int nu_pre;      /* smoothing sweeps per level on the way down (default 3) */
     int nu_post;     /* ... on the way up (default 3) */
     int nu_coarse;   /* sweeps on the coarsest level (default 16) */
-    double omega;    /* damping of the eta-line Jacobi smoother (default 0.8) */
+    double omega;    /* damping of the eta-line Jacobi smoother (default 0.8):
+                        every sweep with sweep_schedule = 0, the coarsest-level
+                        sweeps otherwise */
     int krylov;      /* 1 = GCR acceleration of the refinement loop (default), 0 = plain */
     int stop_mode;   /* PLES_MG_STOP_REL_L2 (default) or PLES_MG_STOP_ABS_MAX */
     int project_rhs; /* 1 = FP64 compatibility projection of the rhs (default) */
@@ -102,6 +104,11 @@ typedef struct ples_mg_opts {
                         exactly (a dense map built at create from the level's
                         operator with its wall ghosts eliminated, applied as one
                         matrix-vector product; default); 0 = nu_coarse sweeps */
+    int sweep_schedule; /* damping of the nu sweeps of a visit: 1 = 1 / (the
+                        roots of the degree-nu Chebyshev polynomial on
+                        [0.48, 2.4]), which damps that part of the smoother's
+                        spectrum by 1 / T_nu(1.5) together (default); 0 = omega
+                        for every sweep */
 } ples_mg_opts;
 
 typedef struct ples_mg_result {

@@ -458,8 +458,17 @@ struct YlineOtf {
     static constexpr bool value = sizeof(T) == 4 && MR * P * BX * 4 <= 32768;
 };
 
+// minimum waves per SIMD of the FP32 smoother with FP32 / 16-bit metrics
+// (register budget per lane: 512 / waves)
+#ifndef PLES_YLINE_WPE_F32
+#define PLES_YLINE_WPE_F32 4
+#endif
+#ifndef PLES_YLINE_WPE_F16
+#define PLES_YLINE_WPE_F16 4
+#endif
 template <int P, int MR, class T, class TM, class TR>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
+    sizeof(T) == 4 ? (sizeof(TM) == 2 ? PLES_YLINE_WPE_F16 : PLES_YLINE_WPE_F32) : 1)))
 k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin, T* __restrict__ pout, const TR* __restrict__ b,
         T omega, T inv_pot, int zero_in) {
     constexpr int BX = SegBX<P>::value;

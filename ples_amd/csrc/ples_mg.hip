@@ -68,6 +68,8 @@ struct ples_mg_handle {
     double* cs[6] = {};
     bool have_cs = false;
     double dt = 1.0;
+    double omega_now = 0.8;               // damping of the sweep being launched
+    double cheb_lo = 0.0, cheb_hi = 0.0;  // Chebyshev sweep schedule interval (0 = constant omega)
     double* S = nullptr;   // device scalars
     double* Sh = nullptr;  // pinned host mirror
     double* part = nullptr;
@@ -383,10 +385,10 @@ void yline_go(H* h, Level& L, const Met<TM>& m, double inv_pot, const TR* rhs, i
     const ColLaunch cl = seg_launch(L.G, P);
     if constexpr (MR > 0)
         hipLaunchKernelGGL((k_yline<P, MR, T, TM, TR>), cl.grid, cl.block, 0, h->s, L.G, m, line_f(L),
-                           (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->o.omega, (T)inv_pot, zin);
+                           (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->omega_now, (T)inv_pot, zin);
     else
         hipLaunchKernelGGL((k_yline_staged<P, T, TM, TR>), cl.grid, cl.block, 0, h->s, L.G, m, line_f(L),
-                           (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->o.omega, (T)inv_pot, zin);
+                           (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->omega_now, (T)inv_pot, zin);
 }
 
 template <int P, int MR, class T, class TR>
@@ -423,6 +425,15 @@ int smooth(H* h, Level& L, const TR* rhs, int nu, bool zero_start, bool fine) {
     const double seg = L.P > 1 ? 8.0 : 0.0;  // v, w reads of the last pass
     for (int s = 0; s < nu; ++s) {
         const int zin = (zero_start && s == 0) ? 1 : 0;
+        // damping of sweep s: the constant omega, or with the Chebyshev schedule
+        // 1 / (the roots of the degree-nu Chebyshev polynomial on [lo, hi]),
+        // so the nu sweeps together damp that part of the spectrum of
+        // M_line^-1 A by 1 / T_nu((hi + lo) / (hi - lo)) (1/9 for nu = 3)
+        h->omega_now = h->o.omega;
+        if (h->cheb_hi > 0.0 && nu > 1) {
+            const double c = 0.5 * (h->cheb_hi + h->cheb_lo), d = 0.5 * (h->cheb_hi - h->cheb_lo);
+            h->omega_now = 1.0 / (c + d * std::cos(M_PI * (2.0 * s + 1.0) / (2.0 * nu)));
+        }
         // forward: rhs, metrics, iterate, invm, cpr; last pass: (v, w), write.  The staged variant
         // adds the e write/read, the cpr re-read and the q write/read.  FP32 register variants
         // factor on the fly (no invm, cpr; the zero-start sweep reads g11, g33 instead).
@@ -531,6 +542,11 @@ int lv_prolong(H* h, int n) {  // n+1 -> n
 // and it has no xi/zeta walls (whose ghosts need the separate halo launches)
 // coarsest levels up to this many cells get the exact dense solve
 constexpr long kCoarseDenseMax = 512;
+// interval of the Chebyshev sweep schedule on the spectrum of M_line^-1 A
+// (measured on the terrain workloads: one V-cycle fewer than omega = 0.8 at
+// 256x128x256 in both precisions and at 512x256x512; the cycle counts are
+// sensitive to the lower end, [0.3, 2.0] costs five more)
+constexpr double kChebLo = 0.48, kChebHi = 2.4;
 
 bool coarse_one_block(const Level& L) {
     return !L.f32 && L.G.ip == 0 && L.G.kp == 0 && (long)L.G.nx * L.G.nz <= 1024;
@@ -1387,6 +1403,7 @@ void ples_mg_default_opts(ples_mg_opts* o) {
     o->fp32_krylov = 1;
     o->smoother_fp16 = 1;
     o->coarse_exact = 1;
+    o->sweep_schedule = 1;
 }
 
 const char* ples_mg_status_string(int s) {
@@ -1476,6 +1493,10 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
         CK(hipMemcpyAsync(h->nodes[c], src[c], nn * 8, hipMemcpyHostToDevice, h->s));
     }
     h->lv.resize(h->L);
+    if (o.sweep_schedule == 1) {
+        h->cheb_hi = kChebHi;
+        h->cheb_lo = kChebLo;
+    }
     for (int n = 0; n < h->L; ++n) {
         Level& L = h->lv[n];
         const int st = 1 << n;

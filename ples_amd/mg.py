@@ -57,7 +57,7 @@ class _Opts(C.Structure):
                 ("stop_mode", C.c_int), ("project_rhs", C.c_int), ("halo_passes", C.c_int),
                 ("device", C.c_int), ("graphs", C.c_int), ("inner_cycles", C.c_int),
                 ("autotune", C.c_int), ("fp32_krylov", C.c_int), ("smoother_fp16", C.c_int),
-                ("coarse_exact", C.c_int)]
+                ("coarse_exact", C.c_int), ("sweep_schedule", C.c_int)]
 
 
 class _Result(C.Structure):
