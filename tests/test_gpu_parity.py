@@ -161,6 +161,25 @@ def test_exact_coarse_solve(ip, kp):
         assert cases.rel_l2_meanfree(fi, ref) < 1e-9, prec
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_sweep_schedule_same_solution(precision):
+    """The Chebyshev sweep schedule changes the damping of each sweep, not the
+    solution: both schedules converge to the oracle's pressure, the schedule
+    in no more cycles than constant damping."""
+    g = cases.grid(32, 16, 32, gamma=1.5, amp=0.2)
+    rhs_g = O.case_rhs(g, 1002)
+    ref = _oracle_solution(g, 3, rhs_g)
+    cyc = {}
+    for sched in (0, 1):
+        with _solver(g, 3, precision, sweep_schedule=sched) as s:
+            fi = np.zeros(s.fi_shape)
+            res = s.multi(fi, cases.interior(rhs_g), eps=1e-12, ficycle=200)
+            assert res.ok, (sched, res)
+            cyc[sched] = res.cycles
+        assert cases.rel_l2_meanfree(fi, ref) < 1e-9, sched
+    assert cyc[1] <= cyc[0], cyc
+
+
 def test_mixed_equals_fp64_c1_shape():
     g = cases.grid(64, 32, 64, gamma=2.0, amp=0.0)
     rhs = cases.interior(O.case_rhs(g, 1001))
