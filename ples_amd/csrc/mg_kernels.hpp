@@ -228,7 +228,16 @@ struct LineF {
     const float* cpr;
     const float* v;
     const float* w;
+    const _Float16* v16;  // 16-bit copies of v, w: read by the smoother with 16-bit metrics
+    const _Float16* w16;
 };
+
+// spike value of row c: the 16-bit copy for the 16-bit-metric smoother
+template <class TM>
+__device__ inline float spike(const float* a, const _Float16* a16, long c) {
+    if constexpr (sizeof(TM) == 2) return (float)a16[c];
+    else return a[c];
+}
 
 // ------------------------------------------------------------------------
 // eta-line Jacobi smoother:  p_new = p + omega * M_line^{-1} (b - A p)
@@ -421,7 +430,7 @@ __global__ void __launch_bounds__(256) k_coarse_lds(Geo G, Met<T> M, LineF F, T*
     Met<T> ML;
     for (int q = 0; q < 9; ++q) ML.g[q] = base + (3 + q) * n;
     float* fl = reinterpret_cast<float*>(base + 12 * n);
-    LineF FL{fl, fl + n, nullptr, nullptr};
+    LineF FL{fl, fl + n, nullptr, nullptr, nullptr, nullptr};
     for (long t = threadIdx.x; t < n; t += blockDim.x) {
         p0[t] = 0;
         p1[t] = 0;
@@ -564,10 +573,10 @@ k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin, T* __restrict__ po
         yL = elast - plast;
         if (P > 1) {
             const long cF = cb + sy, cL = cb + sy * mr;
-            vF = (T)F.v[cF];
-            vL = (T)F.v[cL];
-            wF = (T)F.w[cF];
-            wL = (T)F.w[cL];
+            vF = (T)spike<TM>(F.v, F.v16, cF);
+            vL = (T)spike<TM>(F.v, F.v16, cL);
+            wF = (T)spike<TM>(F.w, F.w16, cF);
+            wL = (T)spike<TM>(F.w, F.w16, cL);
         }
     }
     T uLm = 0, uFp = 0;
@@ -588,7 +597,7 @@ k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin, T* __restrict__ po
     for (int jj = 0; jj < MR; ++jj) {
         const long c = cb + sy * (jj + 1);
         T pn = e[jj];
-        if (P > 1) pn -= uLm * (T)F.v[c] + uFp * (T)F.w[c];
+        if (P > 1) pn -= uLm * (T)spike<TM>(F.v, F.v16, c) + uFp * (T)spike<TM>(F.w, F.w16, c);
         e[jj] = pn;
         pout[c] = pn;
         write_images(G, pout, i, q.j0 + jj, k, pn);
@@ -653,10 +662,10 @@ __global__ void __launch_bounds__(512) k_yline_staged(Geo G, Met<TM> M, LineF F,
         yF = qn - pfirst;
         if (P > 1) {
             const long cF = cb + sy, cL = cb + sy * (q.j1 - q.j0 + 1);
-            vF = (T)F.v[cF];
-            vL = (T)F.v[cL];
-            wF = (T)F.w[cF];
-            wL = (T)F.w[cL];
+            vF = (T)spike<TM>(F.v, F.v16, cF);
+            vL = (T)spike<TM>(F.v, F.v16, cL);
+            wF = (T)spike<TM>(F.w, F.w16, cF);
+            wL = (T)spike<TM>(F.w, F.w16, cL);
         }
     }
     T uLm = 0, uFp = 0;
@@ -676,7 +685,7 @@ __global__ void __launch_bounds__(512) k_yline_staged(Geo G, Met<TM> M, LineF F,
     for (int j = q.j0; j <= q.j1; ++j) {
         const long c = cb + sy * (j - q.j0 + 1);
         T pn = pout[c];
-        if (P > 1) pn -= uLm * (T)F.v[c] + uFp * (T)F.w[c];
+        if (P > 1) pn -= uLm * (T)spike<TM>(F.v, F.v16, c) + uFp * (T)spike<TM>(F.w, F.w16, c);
         pout[c] = pn;
         write_images(G, pout, i, j, k, pn);
         if (j == q.j0) pn1 = pn;

@@ -32,6 +32,8 @@ struct Level {
     double h16s = 1.0;      // 2^-m: 1/pot of the smoother reading g16
     bool use16 = false;     // the smoother reads g16
     double* kd = nullptr;   // coarsest level: dense exact solve map (ghosted x interior)
+    _Float16* v16 = nullptr;  // 16-bit copies of lv_v, lv_w (levels with g16)
+    _Float16* w16 = nullptr;
     float* invm = nullptr;
     float* cpr = nullptr;
     float* lv_v = nullptr;  // segmented line-solve responses
@@ -223,7 +225,7 @@ int choose_segments(int ny) {
     return P;
 }
 
-LineF line_f(const Level& L) { return LineF{L.invm, L.cpr, L.lv_v, L.lv_w}; }
+LineF line_f(const Level& L) { return LineF{L.invm, L.cpr, L.lv_v, L.lv_w, L.v16, L.w16}; }
 
 // per-launch event timing
 struct KT {
@@ -422,7 +424,7 @@ template <class T, class TR>
 int smooth(H* h, Level& L, const TR* rhs, int nu, bool zero_start, bool fine) {
     const double sm = (L.use16 && L.g16[0]) ? 2.0 : sizeof(T), sp = sizeof(T), sr = sizeof(TR);
     const double N = (double)L.ncell();
-    const double seg = L.P > 1 ? 8.0 : 0.0;  // v, w reads of the last pass
+    const double seg = L.P > 1 ? ((L.use16 && L.g16[0]) ? 4.0 : 8.0) : 0.0;  // v, w reads of the last pass
     for (int s = 0; s < nu; ++s) {
         const int zin = (zero_start && s == 0) ? 1 : 0;
         // damping of sweep s: the constant omega, or with the Chebyshev schedule
@@ -1060,6 +1062,15 @@ int level_factors(H* h, Level& L) {
         hipLaunchKernelGGL((k_line_factors<double>), cl.grid, cl.block, 0, h->s, L.G, met<double>(L),
                            1.0 / L.pot, L.P, L.invm, L.cpr, L.lv_v, L.lv_w);
     CK(hipGetLastError());
+    if (L.g16[0] && L.P > 1) {  // the 16-bit-metric smoother reads 16-bit spikes
+        const long n = L.G.n;
+        if (!L.v16) CKS(dalloc(h, (void**)&L.v16, (size_t)n * 2));
+        if (!L.w16) CKS(dalloc(h, (void**)&L.w16, (size_t)n * 2));
+        const int nb = (int)std::min<long>((n + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_to_half, dim3(nb), dim3(256), 0, h->s, n, (const float*)L.lv_v, L.v16, 1.0f);
+        hipLaunchKernelGGL(k_to_half, dim3(nb), dim3(256), 0, h->s, n, (const float*)L.lv_w, L.w16, 1.0f);
+        CK(hipGetLastError());
+    }
     return PLES_MG_OK;
 }
 
@@ -1355,7 +1366,7 @@ size_t estimate(const ples_mg_grid* g, const ples_mg_opts* o) {
         const bool f32 = mixed && n < g->nlevel - 1;
         const size_t sp = f32 ? 4 : 8;
         tot += cells * 9 * sp;                  // V-cycle metrics
-        if (f32 && o->smoother_fp16) tot += cells * 9 * 2;  // smoother's 16-bit copy
+        if (f32 && o->smoother_fp16) tot += cells * (9 * 2 + 4);  // smoother's 16-bit metrics, spikes
         if (n == 0 && f32) tot += cells * 9 * 8;  // FP64 outer metrics
         tot += cells * 16;                      // invm, cpr, v, w
         tot += cells * sp * 2;                  // ping-pong
