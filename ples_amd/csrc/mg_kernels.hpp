@@ -163,6 +163,15 @@ __device__ inline bool column_of(const Geo& G, int& i, int& k) {
     return i <= G.nx && k <= G.nz;
 }
 
+// interior cell of a one-thread-per-cell launch (grid nx/bx x ny/by x nz,
+// rows along the block's x: coalesced)
+__device__ inline bool cell_of(const Geo& G, long& c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > G.nx || j > G.ny) return false;
+    c = G.idx(i, j, (int)blockIdx.z + 1);
+    return true;
+}
+
 // write value v of row j of column (i,k) also to its periodic images
 template <class T>
 __host__ __device__ inline void write_images(const Geo& G, T* __restrict__ p, int i, int j, int k, T v) {
@@ -1635,46 +1644,35 @@ __global__ void k_fin_sum(const double* __restrict__ part, int nb, int nv, doubl
 // a[interior] -= (S[S_SUM] - bflux) / ncell
 __global__ void __launch_bounds__(256) k_sub_mean(Geo G, double* __restrict__ a, const double* S,
                                                   double bflux, double ncell) {
-    int i, k;
-    if (!column_of(G, i, k)) return;
-    const double m = (S[S_SUM] - bflux) / ncell;
-    for (int j = 1; j <= G.ny; ++j) a[G.idx(i, j, k)] -= m;
+    long c;
+    if (!cell_of(G, c)) return;
+    a[c] -= (S[S_SUM] - bflux) / ncell;
 }
 
 // interior perturbation p += rel * std * N(0,1)
 __global__ void __launch_bounds__(256) k_perturb(Geo G, double* __restrict__ a, const double* S,
                                                  uint64_t seed, double rel, double ncell) {
-    int i, k;
-    if (!column_of(G, i, k)) return;
+    long c;
+    if (!cell_of(G, c)) return;
     const double mean = S[S_SUM] / ncell;
     const double var = fmax(S[S_SUM + 1] / ncell - mean * mean, 0.0);
-    const double sd = sqrt(var);
-    for (int j = 1; j <= G.ny; ++j) {
-        const long c = G.idx(i, j, k);
-        a[c] += rel * sd * gauss(seed, 7u, (int64_t)c);
-    }
+    a[c] += rel * sqrt(var) * gauss(seed, 7u, (int64_t)c);
 }
 
 // r32 = s * r64 over the interior (the FP32 inner problem's rhs)
 __global__ void __launch_bounds__(256) k_scale_to32(Geo G, const double* __restrict__ r, double s,
                                                     float* __restrict__ r32) {
-    int i, k;
-    if (!column_of(G, i, k)) return;
-    for (int j = 1; j <= G.ny; ++j) {
-        const long c = G.idx(i, j, k);
-        r32[c] = (float)(s * r[c]);
-    }
+    long c;
+    if (!cell_of(G, c)) return;
+    r32[c] = (float)(s * r[c]);
 }
 
 // x += s * e32 over the interior (FP64 iterative-refinement update)
 __global__ void __launch_bounds__(256) k_add_back(Geo G, double* __restrict__ x, double s,
                                                   const float* __restrict__ e32) {
-    int i, k;
-    if (!column_of(G, i, k)) return;
-    for (int j = 1; j <= G.ny; ++j) {
-        const long c = G.idx(i, j, k);
-        x[c] += s * (double)e32[c];
-    }
+    long c;
+    if (!cell_of(G, c)) return;
+    x[c] += s * (double)e32[c];
 }
 
 // plain 16-byte copy (bandwidth probe)

@@ -180,6 +180,15 @@ struct ColLaunch {
     dim3 grid, block;
     int nblocks;
 };
+// one thread per interior cell (kernels using cell_of)
+ColLaunch cell_launch(const Geo& G) {
+    ColLaunch c;
+    c.block = dim3(64, 4, 1);
+    c.grid = dim3((unsigned)((G.nx + 63) / 64), (unsigned)((G.ny + 3) / 4), (unsigned)G.nz);
+    c.nblocks = (int)(c.grid.x * c.grid.y * c.grid.z);
+    return c;
+}
+
 ColLaunch col_launch(const Geo& G) {
     ColLaunch c;
     const int bx = std::min(64, pow2ceil(G.nx));
@@ -924,15 +933,14 @@ int solve_ir32(H* h, double eps, int maxcyc, int& cycles, double& rr, double& rm
     const bool absmode = h->o.stop_mode == PLES_MG_STOP_ABS_MAX;
     auto measure = [&](double r2, double rm) { return absmode ? rm : std::sqrt(r2) / bnorm; };
     const Level& L0 = h->lv[0];
-    const ColLaunch cc = col_launch(L0.G);
+    const ColLaunch ce = cell_launch(L0.G);
     int stall = 0;
     while (cycles < maxcyc) {
         const double m = measure(rr, rmax);
         if (m < eps) return PLES_MG_OK;
         const double rn = std::sqrt(rr);
         if (!(rn > 0.0)) return PLES_MG_OK;
-        hipLaunchKernelGGL(k_scale_to32, cc.grid, cc.block, 0, h->s, L0.G, (const double*)h->rvec, 1.0 / rn,
-                           h->rr32);
+        hipLaunchKernelGGL(k_scale_to32, ce.grid, ce.block, 0, h->s, L0.G, (const double*)h->rvec, 1.0 / rn, h->rr32);
         CK(hipMemsetAsync(h->e32, 0, (size_t)L0.G.n * sizeof(float), h->s));
         CK(hipGetLastError());
         // relative reduction the outer test still needs (L2 terms), FP32-bounded
@@ -944,7 +952,7 @@ int solve_ir32(H* h, double eps, int maxcyc, int& cycles, double& rr, double& rm
         CKS(run_steps(h, 1, h->S2, 1.0, tol, std::min(40, maxcyc - cycles), rel, run));
         cycles += run.applied;
         if (run.nonfinite) return PLES_MG_ERR_NONFINITE;
-        hipLaunchKernelGGL(k_add_back, cc.grid, cc.block, 0, h->s, L0.G, h->x, rn, (const float*)h->e32);
+        hipLaunchKernelGGL(k_add_back, ce.grid, ce.block, 0, h->s, L0.G, h->x, rn, (const float*)h->e32);
         CK(hipGetLastError());
         CKS(outer_residual(h, h->x, h->rvec));
         CKS(read_S(h));
@@ -1623,9 +1631,11 @@ static int project_rhs(H* h, double bflux) {
     hipLaunchKernelGGL(k_sum, cl.grid, cl.block, 0, h->s, L0.G, (const double*)h->bvec, h->part, cl.nblocks);
     hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1), 0, h->s, (const double*)h->part, cl.nblocks, 2, h->S);
     const double ncell = (double)L0.ncell();
-    if (h->o.project_rhs)
-        hipLaunchKernelGGL(k_sub_mean, cl.grid, cl.block, 0, h->s, L0.G, h->bvec, (const double*)h->S, bflux,
+    if (h->o.project_rhs) {
+        const ColLaunch ce = cell_launch(L0.G);
+        hipLaunchKernelGGL(k_sub_mean, ce.grid, ce.block, 0, h->s, L0.G, h->bvec, (const double*)h->S, bflux,
                            ncell);
+    }
     CK(hipGetLastError());
     CKS(read_S(h));
     h->last_removed = h->o.project_rhs ? (h->Sh[S_SUM] - bflux) / ncell : 0.0;
@@ -1906,7 +1916,8 @@ int ples_mg_perturb_fi(ples_mg_handle* h, uint64_t seed, double rel) {
     const ColLaunch cl = col_launch(L0.G);
     hipLaunchKernelGGL(k_sum, cl.grid, cl.block, 0, h->s, L0.G, (const double*)h->x, h->part, cl.nblocks);
     hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1), 0, h->s, (const double*)h->part, cl.nblocks, 2, h->S);
-    hipLaunchKernelGGL(k_perturb, cl.grid, cl.block, 0, h->s, L0.G, h->x, (const double*)h->S, seed, rel,
+    const ColLaunch ce = cell_launch(L0.G);
+    hipLaunchKernelGGL(k_perturb, ce.grid, ce.block, 0, h->s, L0.G, h->x, (const double*)h->S, seed, rel,
                        (double)L0.ncell());
     CK(hipGetLastError());
     CK(hipStreamSynchronize(h->s));
@@ -2782,7 +2793,8 @@ int try_admit(SV* sv, SReq& r) {
         const ColLaunch cl = col_launch(L0.G);
         hipLaunchKernelGGL(k_sum, cl.grid, cl.block, 0, h->s, L0.G, (const double*)h->x, h->part, cl.nblocks);
         hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1), 0, h->s, (const double*)h->part, cl.nblocks, 2, h->S);
-        hipLaunchKernelGGL(k_perturb, cl.grid, cl.block, 0, h->s, L0.G, h->x, (const double*)h->S, r.seed, r.rel,
+        const ColLaunch ce = cell_launch(L0.G);
+        hipLaunchKernelGGL(k_perturb, ce.grid, ce.block, 0, h->s, L0.G, h->x, (const double*)h->S, r.seed, r.rel,
                            (double)L0.ncell());
     } else {
         CK(hipMemsetAsync(h->x, 0, (size_t)h->lv[0].G.n * 8, h->s));
