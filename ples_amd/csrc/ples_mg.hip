@@ -765,16 +765,28 @@ int arm_stop(H* h, double* Sd, double thr, int mode) {
 int step32_body(H* h, int q) {
     Level& L0 = h->lv[0];
     const ColLaunch cl = seg_launch(L0.G, L0.P);
-    h->rhs32 = h->rr32;
-    int st = PLES_MG_OK;
-    for (int c = 0; c < h->inner && st == PLES_MG_OK; ++c) st = vcycle(h, nullptr, c == 0);
-    h->rhs32 = nullptr;
-    CKS(st);
     float* Zq = h->Z32[q];
     float* Wq = h->W32[q];
     float* Zp = h->Z32[1 - q];
     float* Wp = h->W32[1 - q];
-    {
+    // One V-cycle from zero flips the fine ping-pong pair max(1, nu_pre) +
+    // nu_post times: the buffer its last sweep writes is known, so Zq stands in
+    // for it and receives z directly (no copy); Zp is not touched.
+    const bool direct = h->inner == 1 && h->L >= 2;
+    const int fin = L0.cur ^ ((std::max(1, h->o.nu_pre) + std::max(0, h->o.nu_post)) & 1);
+    void* held = L0.p[fin];
+    if (direct) L0.p[fin] = Zq;
+    h->rhs32 = h->rr32;
+    int st = PLES_MG_OK;
+    for (int c = 0; c < h->inner && st == PLES_MG_OK; ++c) st = vcycle(h, nullptr, c == 0);
+    h->rhs32 = nullptr;
+    if (direct) {
+        const bool landed = L0.cur == fin;
+        L0.p[fin] = held;
+        if (!landed) return PLES_MG_ERR_ARG;  // the sweep count did not match (internal)
+    }
+    CKS(st);
+    if (!direct) {
         const long n = L0.G.n;
         const int nb = (int)std::min<long>((n + 255) / 256, 8192);
         hipLaunchKernelGGL((k_convert<float, float>), dim3(nb), dim3(256), 0, h->s, n, (const float*)L0.p[L0.cur],
