@@ -1160,7 +1160,7 @@ __host__ __device__ inline void halo_edge_point(const Geo& G, T* __restrict__ p,
 // tile with the images: the same passes as the launch-per-colour fill, in one
 // launch.  Dynamic LDS: 4 (TX + 4 passes)(TK + 4 passes) values of T.
 template <class T, class TM, int TX, int TK>
-__global__ void __launch_bounds__(256) k_eta_ghosts(Geo G, Met<TM> M, T* __restrict__ p, BCs bc, int passes) {
+__global__ void __launch_bounds__(1024) k_eta_ghosts(Geo G, Met<TM> M, T* __restrict__ p, BCs bc, int passes) {
     extern __shared__ __align__(16) unsigned char eta_lds[];
     const int Hw = 2 * passes, WX = TX + 2 * Hw, WK = TK + 2 * Hw, NW = WX * WK;
     T* sf = (T*)eta_lds;
@@ -1194,11 +1194,14 @@ __global__ void __launch_bounds__(256) k_eta_ghosts(Geo G, Met<TM> M, T* __restr
         sg[t] = (T)p[G.idx(i, jg, k)];
     }
     __syncthreads();
+    // each colour phase walks only that colour's points (WX is even: half a row each)
+    const int hx = WX / 2;
     for (int ps = 0; ps < passes; ++ps)
         for (int color = 0; color < 2; ++color) {
-            for (int t = threadIdx.x; t < NW; t += blockDim.x) {
-                const int wi = t % WX, wk = t / WX;
-                if (wi == 0 || wi == WX - 1 || wk == 0 || wk == WK - 1 || ((wi + wk) & 1) != color) continue;
+            for (int u = threadIdx.x; u < hx * WK; u += blockDim.x) {
+                const int wk = u / hx, wi = 2 * (u % hx) + ((wk + color) & 1);
+                if (wi == 0 || wi == WX - 1 || wk == 0 || wk == WK - 1) continue;
+                const int t = wi + WX * wk;
                 sg[t] = sf[t] + sa[t] * (sg[t + 1] - sg[t - 1]) + sb[t] * (sg[t + WX] - sg[t - WX]);
             }
             __syncthreads();

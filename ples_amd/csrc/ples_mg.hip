@@ -291,7 +291,7 @@ bool eta_ghosts(H* h, const Geo& G, const Met<TM>& m, T* p, const BCs& bc, int p
     const size_t lds = (size_t)(TX + 2 * hw) * (TK + 2 * hw) * 4 * sizeof(T);
     if (lds > 65536) return false;
     const int nb = 2 * ((G.nx + TX - 1) / TX) * ((G.nz + TK - 1) / TK);
-    hipLaunchKernelGGL((k_eta_ghosts<T, TM, TX, TK>), dim3(nb), dim3(256), lds, h->s, G, m, p, bc, passes);
+    hipLaunchKernelGGL((k_eta_ghosts<T, TM, TX, TK>), dim3(nb), dim3(1024), lds, h->s, G, m, p, bc, passes);
     return true;
 }
 
