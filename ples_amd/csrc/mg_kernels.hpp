@@ -766,6 +766,56 @@ __global__ void __launch_bounds__(512) k_resid(Geo G, Met<TM> M, const T* __rest
     }
 }
 
+// V-cycle residual of a level restricted straight into the next coarser
+// level's rhs, without storing the fine residual: rc(I,J,K) = -(1/8) * the
+// sum of the 2x2x2 fine residuals r = A p - b (k_resid MODE 0 then
+// k_restrict).  Each thread walks the two zeta planes 2K-1, 2K of its column
+// together, one row pair at a time (two stencil windows), so the 2x2 (j, k)
+// part of a child block stays in registers; the i pair is the neighbouring
+// lane (bx even).  Segments have an even number of rows and start at an odd
+// row.  Block: bx lanes along i x P segments; XCD-aware block order.
+template <class T, class TM, class TR, class TC>
+__global__ void __launch_bounds__(512) k_resid_restrict(Geo G, Met<TM> M, const T* __restrict__ p,
+                                                        const TR* __restrict__ b, Geo Gc, TC* __restrict__ rc,
+                                                        T inv_pot, int P) {
+    int bxi, bk;
+    xcd_tile(bxi, bk);
+    const int bx = blockDim.x, tx = threadIdx.x, s = threadIdx.y;
+    const int m = G.ny / P;
+    const int i = bxi * bx + tx + 1, kc = bk + 1, j0 = s * m + 1;
+    const bool act = i <= G.nx;
+    const long sy = G.sy, sz = G.sz;
+    const long cb0 = act ? G.idx(i, j0 - 1, 2 * kc - 1) : 0, cb1 = cb0 + sz;
+    Win<T> w0, w1;
+    G2<T> gm0{}, gm1{};
+    if (act) {
+        win_init(w0, p, cb0, sy, sz);
+        win_init(w1, p, cb1, sy, sz);
+        gm0 = load_g2<T>(M, cb0);
+        gm1 = load_g2<T>(M, cb1);
+    }
+    for (int rp = 0; rp < m / 2; ++rp) {
+        T v = 0;
+        if (act) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const long c0 = cb0 + sy * (2 * rp + h + 1), c1 = c0 + sz;
+                win_load(w0, p, c0, sy, sz);
+                win_load(w1, p, c1, sy, sz);
+                const G2<T> g00 = load_g2<T>(M, c0), g01 = load_g2<T>(M, c1);
+                v += stencil(w0, M, c0, sz, gm0, g00, inv_pot) - (T)b[c0];
+                v += stencil(w1, M, c1, sz, gm1, g01, inv_pot) - (T)b[c1];
+                gm0 = g00;
+                gm1 = g01;
+                win_shift(w0);
+                win_shift(w1);
+            }
+        }
+        v += __shfl_down(v, 1, 64);
+        if (act && !(tx & 1)) rc[Gc.idx((i + 1) >> 1, (j0 + 1) / 2 + rp, kc)] = (TC)(-v * (T)0.125);
+    }
+}
+
 // ------------------------------------------------------------------------
 // Outer-loop scalar logic on the device (no host round trip per iteration).
 // ------------------------------------------------------------------------

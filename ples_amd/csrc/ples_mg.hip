@@ -503,6 +503,27 @@ int restrict_to(H* h, Level& F, Level& Cc) {
     return PLES_MG_OK;
 }
 
+// V-cycle residual of level F restricted straight into Cc.b (one launch, no
+// stored fine residual); returns 1 where the launch shape does not apply (odd
+// segment rows, planes or lanes) and the caller runs the two kernels instead
+template <class T, class TR, class TC>
+int resid_restrict(H* h, Level& F, Level& Cc, const TR* rhs, bool fine) {
+    const int P = F.rP > 0 ? F.rP : F.P;
+    int bx = F.rbx > 0 ? F.rbx : seg_bx(P);
+    const int m = F.G.ny / P;
+    if (m < 2 || (m & 1) || (F.G.nz & 1) || (F.G.nx & 1) || F.G.ny % P) return 1;
+    while (bx * P > 512 && bx > 2) bx >>= 1;
+    if ((bx * P) % 64 != 0 || (bx & 1)) return 1;
+    const dim3 block((unsigned)bx, (unsigned)P, 1), grid((unsigned)((F.G.nx + bx - 1) / bx), (unsigned)(F.G.nz / 2));
+    const double N = (double)F.ncell();
+    KT kt(h, fine ? PLES_MG_K_RESID_FINE : PLES_MG_K_RESID,
+          N * (9 * sizeof(T) + sizeof(T) + sizeof(TR)) + (double)Cc.ncell() * sizeof(TC));
+    hipLaunchKernelGGL((k_resid_restrict<T, T, TR, TC>), grid, block, 0, h->s, F.G, met<T>(F),
+                       (const T*)F.p[F.cur], rhs, Cc.G, (TC*)Cc.b, (T)(1.0 / F.pot), P);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
 template <class TC, class TF>
 int prolong_to(H* h, Level& Cc, Level& F) {
     dim3 block(std::min(64, pow2ceil(F.G.nx + 2)), 4);
@@ -531,6 +552,30 @@ int lv_resid(H* h, int n, const double* r0) {
     if (n == 0) return L.f32 ? vresid<float, double>(h, L, r0, true) : vresid<double, double>(h, L, r0, true);
     return L.f32 ? vresid<float, float>(h, L, (const float*)L.b, false)
                  : vresid<double, double>(h, L, (const double*)L.b, false);
+}
+
+int lv_restrict(H* h, int n);
+
+// residual of level n restricted to level n+1's rhs: fused where it applies
+int lv_resid_restrict(H* h, int n, const double* r0) {
+    Level &F = h->lv[n], &Cc = h->lv[n + 1];
+    int st = 1;
+    if (n == 0 && h->rhs32)
+        st = Cc.f32 ? resid_restrict<float, float, float>(h, F, Cc, h->rhs32, true)
+                    : resid_restrict<float, float, double>(h, F, Cc, h->rhs32, true);
+    else if (n == 0 && F.f32)
+        st = Cc.f32 ? resid_restrict<float, double, float>(h, F, Cc, r0, true)
+                    : resid_restrict<float, double, double>(h, F, Cc, r0, true);
+    else if (n == 0)
+        st = resid_restrict<double, double, double>(h, F, Cc, r0, true);
+    else if (F.f32)
+        st = Cc.f32 ? resid_restrict<float, float, float>(h, F, Cc, (const float*)F.b, false)
+                    : resid_restrict<float, float, double>(h, F, Cc, (const float*)F.b, false);
+    else
+        st = resid_restrict<double, double, double>(h, F, Cc, (const double*)F.b, false);
+    if (st != 1) return st;
+    CKS(lv_resid(h, n, r0));
+    return lv_restrict(h, n);
 }
 
 int lv_restrict(H* h, int n) {
@@ -606,10 +651,7 @@ int vcycle(H* h, const double* r0, bool fine_zero = true) {
             continue;
         }
         CKS(lv_smooth(h, n, r0, std::max(1, nu), n > 0 || fine_zero));
-        if (n < L - 1) {
-            CKS(lv_resid(h, n, r0));
-            CKS(lv_restrict(h, n));
-        }
+        if (n < L - 1) CKS(lv_resid_restrict(h, n, r0));
     }
     for (int n = L - 2; n >= 0; --n) {
         CKS(lv_prolong(h, n));
