@@ -1184,6 +1184,7 @@ int time_launches(H* h, int reps, double& ms, F&& f) {
 
 int lv_smooth(H* h, int n, const double* r0, int nu, bool zero);
 int lv_resid(H* h, int n, const double* r0);
+int lv_resid_restrict(H* h, int n, const double* r0);
 bool coarse_one_block(const Level& L);
 
 int tune_level(H* h, int n) {
@@ -1252,7 +1253,8 @@ int tune_level(H* h, int n) {
         L.rP = P;
         L.rbx = bx;
         double ms = 0;
-        CKS(time_launches(h, 3, ms, [&] { return lv_resid(h, n, r0); }));
+        // the V-cycle runs the residual fused with the restriction below the coarsest level
+        CKS(time_launches(h, 3, ms, [&] { return n < h->L - 1 ? lv_resid_restrict(h, n, r0) : lv_resid(h, n, r0); }));
         if (ms < 0.97 * tbest || tbest == 1e30) {
             tbest = ms;
             best.rP = P;
