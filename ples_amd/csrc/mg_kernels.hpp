@@ -991,6 +991,48 @@ __global__ void __launch_bounds__(256) k_prolong(Geo Gc, Geo Gf, const TC* __res
     pf[f] = pf[f] + s * (TF)(1.0 / 64.0);
 }
 
+// The same interpolation by coarse blocks: fine cells 2I, 2I+1 (and likewise
+// in j, k) take their values from the same coarse cells I, I+1, so one thread
+// per (I, J, K), I in 0..ncx (ghosts included), loads that 2x2x2 coarse block
+// once and updates its 8 fine children, two consecutive i at a time (the fine
+// strides are even, so the pair is aligned).
+template <class TC, class TF>
+__global__ void __launch_bounds__(256) k_prolong_blk(Geo Gc, Geo Gf, const TC* __restrict__ ec,
+                                                     TF* __restrict__ pf) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int Kc = blockIdx.z;
+    if (I > Gc.nx || J > Gc.ny) return;
+    const long c = Gc.idx(I, J, Kc);
+    const long sy = Gc.sy, sz = Gc.sz;
+    TF e[2][2][2];  // [k][j][i] offsets
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int d = 0; d < 2; ++d) e[a][bb][d] = (TF)ec[c + d + sy * bb + sz * a];
+    using V2 = TF __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+            // weights 3 for the coarse cell on the fine child's side, 1 for the other
+            const TF wk0 = dk ? (TF)1 : (TF)3, wk1 = dk ? (TF)3 : (TF)1;
+            const TF wj0 = dj ? (TF)1 : (TF)3, wj1 = dj ? (TF)3 : (TF)1;
+            TF s2[2];
+#pragma unroll
+            for (int d = 0; d < 2; ++d)
+                s2[d] = wk0 * (wj0 * e[0][0][d] + wj1 * e[0][1][d]) + wk1 * (wj0 * e[1][0][d] + wj1 * e[1][1][d]);
+            // fine i = 2I: 3 * (coarse I) + 1 * (I+1); fine 2I+1: 1 * I + 3 * (I+1)
+            V2* f = (V2*)(pf + Gf.idx(2 * I, 2 * J + dj, 2 * Kc + dk));
+            V2 v = *f;
+            v.x += ((TF)3 * s2[0] + s2[1]) * (TF)(1.0 / 64.0);
+            v.y += (s2[0] + (TF)3 * s2[1]) * (TF)(1.0 / 64.0);
+            *f = v;
+        }
+}
+
 // ------------------------------------------------------------------------
 // Exact ghost fill (outer FP64 operator, walls in xi/zeta): face relations
 // of fsrc/multigrid.f03:1113-1317 solved by red/black passes over each wall

@@ -526,10 +526,11 @@ int resid_restrict(H* h, Level& F, Level& Cc, const TR* rhs, bool fine) {
 
 template <class TC, class TF>
 int prolong_to(H* h, Level& Cc, Level& F) {
-    dim3 block(std::min(64, pow2ceil(F.G.nx + 2)), 4);
-    dim3 grid((F.G.nx + 2 + block.x - 1) / block.x, (F.G.ny + 2 + 3) / 4, F.G.nz + 2);
+    // one thread per coarse block (I, J, K) in 0..nc (ghosts included): 8 fine children each
+    dim3 block(std::min(64, pow2ceil(Cc.G.nx + 1)), 4);
+    dim3 grid((Cc.G.nx + 1 + block.x - 1) / block.x, (Cc.G.ny + 1 + 3) / 4, Cc.G.nz + 1);
     KT kt(h, PLES_MG_K_TRANSFER, 2.0 * F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC));
-    hipLaunchKernelGGL((k_prolong<TC, TF>), grid, block, 0, h->s, Cc.G, F.G, (const TC*)Cc.p[Cc.cur],
+    hipLaunchKernelGGL((k_prolong_blk<TC, TF>), grid, block, 0, h->s, Cc.G, F.G, (const TC*)Cc.p[Cc.cur],
                        (TF*)F.p[F.cur]);
     CK(hipGetLastError());
     return PLES_MG_OK;
