@@ -120,15 +120,18 @@ struct G2 {
 };
 
 // s * sum of face-flux differences at (i,j,k), the reference expression
-// of fsrc/resid_sndrcv.f03:85-147 (without the "- rh").
-template <class T, class TM>
+// of fsrc/resid_sndrcv.f03:85-147 (without the "- rh").  X0: the level's
+// xi-zeta cross metrics g13, g31 are identically zero (orthogonal xi and
+// zeta grid lines, as on terrain-following grids over x-z separable meshes),
+// so they are not loaded.
+template <bool X0 = false, class T, class TM>
 __host__ __device__ inline T stencil(const Win<T>& w, const Met<TM>& M, long c, long sz, const G2<T>& gm,
                             const G2<T>& g0, T inv_pot) {
     const T q = (T)0.25;
-    const T g11 = (T)M.g[0][c], g12 = (T)M.g[1][c], g13 = (T)M.g[2][c];
-    const T h11 = (T)M.g[0][c - 1], h12 = (T)M.g[1][c - 1], h13 = (T)M.g[2][c - 1];
-    const T g31 = (T)M.g[6][c], g32 = (T)M.g[7][c], g33 = (T)M.g[8][c];
-    const T h31 = (T)M.g[6][c - sz], h32 = (T)M.g[7][c - sz], h33 = (T)M.g[8][c - sz];
+    const T g11 = (T)M.g[0][c], g12 = (T)M.g[1][c], g13 = X0 ? (T)0 : (T)M.g[2][c];
+    const T h11 = (T)M.g[0][c - 1], h12 = (T)M.g[1][c - 1], h13 = X0 ? (T)0 : (T)M.g[2][c - 1];
+    const T g31 = X0 ? (T)0 : (T)M.g[6][c], g32 = (T)M.g[7][c], g33 = (T)M.g[8][c];
+    const T h31 = X0 ? (T)0 : (T)M.g[6][c - sz], h32 = (T)M.g[7][c - sz], h33 = (T)M.g[8][c - sz];
     T res_dx = (g11 * (w.xp0 - w.p0) +
                 q * (g12 * (w.xpp + w.pp - w.xpm - w.pm) + g13 * (w.dpp + w.zp0 - w.dpm - w.zm0))) *
                inv_pot;
@@ -484,7 +487,7 @@ struct YlineOtf {
 #ifndef PLES_YLINE_WPE_F16
 #define PLES_YLINE_WPE_F16 4
 #endif
-template <int P, int MR, class T, class TM, class TR>
+template <int P, int MR, class T, class TM, class TR, bool X0 = false>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     sizeof(T) == 4 ? (sizeof(TM) == 2 ? PLES_YLINE_WPE_F16 : PLES_YLINE_WPE_F32) : 1)))
 k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin, T* __restrict__ pout, const TR* __restrict__ b,
@@ -550,7 +553,7 @@ k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin, T* __restrict__ po
                 const long c = cb + sy * (jj + 1);
                 win_load(w, pin, c, sy, sz);
                 const G2<T> g0 = load_g2<T>(M, c);
-                const T r = stencil(w, M, c, sz, gm, g0, inv_pot) - (T)b[c];
+                const T r = stencil<X0>(w, M, c, sz, gm, g0, inv_pot) - (T)b[c];
                 T a, inv, cp;
                 if constexpr (OTF) {
                     a = factor(jj, c, gm.g22, g0.g22, inv);
@@ -709,7 +712,7 @@ __global__ void __launch_bounds__(512) k_yline_staged(Geo G, Met<TM> M, LineF F,
 //   MODE 1: r = b - A p, partial sums of r.r, max|r|, b.b
 //   MODE 2: w = A p, partial sums of w.w, w.wp, r.w, r.wp
 // ------------------------------------------------------------------------
-template <int MODE, class T, class TM, class TR, class TO, class TV = double>
+template <int MODE, class T, class TM, class TR, class TO, class TV = double, bool X0 = false>
 __global__ void __launch_bounds__(512) k_resid(Geo G, Met<TM> M, const T* __restrict__ p,
                                                const TR* __restrict__ b, TO* __restrict__ out,
                                                T inv_pot, const TV* __restrict__ rvec,
@@ -728,7 +731,7 @@ __global__ void __launch_bounds__(512) k_resid(Geo G, Met<TM> M, const T* __rest
             const long c = cb + sy * (j - q.j0 + 1);
             win_load(w, p, c, sy, sz);
             const G2<T> g0 = load_g2<T>(M, c);
-            const T ap = stencil(w, M, c, sz, gm, g0, inv_pot);
+            const T ap = stencil<X0>(w, M, c, sz, gm, g0, inv_pot);
             if (MODE == 0) {
                 out[c] = (TO)(ap - (T)b[c]);
             } else if (MODE == 1) {
@@ -774,7 +777,7 @@ __global__ void __launch_bounds__(512) k_resid(Geo G, Met<TM> M, const T* __rest
 // part of a child block stays in registers; the i pair is the neighbouring
 // lane (bx even).  Segments have an even number of rows and start at an odd
 // row.  Block: bx lanes along i x P segments; XCD-aware block order.
-template <class T, class TM, class TR, class TC>
+template <class T, class TM, class TR, class TC, bool X0 = false>
 __global__ void __launch_bounds__(512) k_resid_restrict(Geo G, Met<TM> M, const T* __restrict__ p,
                                                         const TR* __restrict__ b, Geo Gc, TC* __restrict__ rc,
                                                         T inv_pot, int P) {
@@ -803,8 +806,8 @@ __global__ void __launch_bounds__(512) k_resid_restrict(Geo G, Met<TM> M, const 
                 win_load(w0, p, c0, sy, sz);
                 win_load(w1, p, c1, sy, sz);
                 const G2<T> g00 = load_g2<T>(M, c0), g01 = load_g2<T>(M, c1);
-                v += stencil(w0, M, c0, sz, gm0, g00, inv_pot) - (T)b[c0];
-                v += stencil(w1, M, c1, sz, gm1, g01, inv_pot) - (T)b[c1];
+                v += stencil<X0>(w0, M, c0, sz, gm0, g00, inv_pot) - (T)b[c0];
+                v += stencil<X0>(w1, M, c1, sz, gm1, g01, inv_pot) - (T)b[c1];
                 gm0 = g00;
                 gm1 = g01;
                 win_shift(w0);
@@ -1250,9 +1253,25 @@ __host__ __device__ inline void halo_edge_point(const Geo& G, T* __restrict__ p,
 // TX x TK tile of one wall and carries a halo of 2*passes points each side
 // (periodic wrap), runs `passes` red/black passes on it in LDS and writes its
 // tile with the images: the same passes as the launch-per-colour fill, in one
-// launch.  Dynamic LDS: 4 (TX + 4 passes)(TK + 4 passes) values of T.
+// launch.  The starting ghost values come from a snapshot of the ghost rows
+// (k_ghost_rows), so the result does not depend on the order in which
+// blocks run.  Dynamic LDS: 4 (TX + 4 passes)(TK + 4 passes) values of T.
+// Copy of the two eta-wall ghost rows (snap[side][k][i], ghosted i and k):
+// the starting guess of k_eta_ghosts, taken before the launch so no block
+// reads a ghost that another block of that launch writes.
+template <class T>
+__global__ void __launch_bounds__(256) k_ghost_rows(Geo G, const T* __restrict__ p, T* __restrict__ snap) {
+    const long NX = G.nx + 2, NZ = G.nz + 2;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * NX * NZ) return;
+    const int side = (int)(t / (NX * NZ));
+    const long r = t % (NX * NZ);
+    snap[t] = p[G.idx((int)(r % NX), side ? G.ny + 1 : 0, (int)(r / NX))];
+}
+
 template <class T, class TM, int TX, int TK>
-__global__ void __launch_bounds__(1024) k_eta_ghosts(Geo G, Met<TM> M, T* __restrict__ p, BCs bc, int passes) {
+__global__ void __launch_bounds__(1024) k_eta_ghosts(Geo G, Met<TM> M, T* __restrict__ p, const T* __restrict__ snap,
+                                                     BCs bc, int passes) {
     extern __shared__ __align__(16) unsigned char eta_lds[];
     const int Hw = 2 * passes, WX = TX + 2 * Hw, WK = TK + 2 * Hw, NW = WX * WK;
     T* sf = (T*)eta_lds;
@@ -1283,7 +1302,7 @@ __global__ void __launch_bounds__(1024) k_eta_ghosts(Geo G, Met<TM> M, T* __rest
                 cB * ((T)p[G.idx(i, jr, kp1)] - (T)p[G.idx(i, jr, km1)]);
         sa[t] = cA;
         sb[t] = cB;
-        sg[t] = (T)p[G.idx(i, jg, k)];
+        sg[t] = snap[(long)side * (G.nx + 2) * (G.nz + 2) + i + (long)(G.nx + 2) * k];
     }
     __syncthreads();
     // each colour phase walks only that colour's points (WX is even: half a row each)
@@ -1793,7 +1812,8 @@ __global__ void k_to_half(long n, const float* __restrict__ a, _Float16* __restr
 }
 
 // per-block maxima of |a| over n values (grid-stride), one partial per block
-__global__ void __launch_bounds__(256) k_absmax(long n, const float* __restrict__ a, double* partials) {
+template <class TI>
+__global__ void __launch_bounds__(256) k_absmax(long n, const TI* __restrict__ a, double* partials) {
     double v[1] = {0.0};
     const bool mx[1] = {true};
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -31,6 +31,7 @@ struct Level {
     _Float16* g16[9] = {};  // smoother copy of g32 * 4^m (FP32 levels, smoother_fp16)
     double h16s = 1.0;      // 2^-m: 1/pot of the smoother reading g16
     bool use16 = false;     // the smoother reads g16
+    bool x0 = false;        // g13 and g31 identically zero: stencil kernels skip them
     double* kd = nullptr;   // coarsest level: dense exact solve map (ghosted x interior)
     _Float16* v16 = nullptr;  // 16-bit copies of lv_v, lv_w (levels with g16)
     _Float16* w16 = nullptr;
@@ -71,6 +72,8 @@ struct ples_mg_handle {
     bool have_cs = false;
     double dt = 1.0;
     double omega_now = 0.8;               // damping of the sweep being launched
+    void* snap = nullptr;                 // fine-level eta-wall ghost rows (k_eta_ghosts start)
+    size_t snap_bytes = 0;
     double cheb_lo = 0.0, cheb_hi = 0.0;  // Chebyshev sweep schedule interval (0 = constant omega)
     double* S = nullptr;   // device scalars
     double* Sh = nullptr;  // pinned host mirror
@@ -258,6 +261,9 @@ struct KT {
     }
 };
 
+// metric arrays the level's stencil kernels read
+double nmet(const Level& L) { return L.x0 ? 7.0 : 9.0; }
+
 template <class T>
 Met<T> met(const Level& L);
 template <>
@@ -289,9 +295,13 @@ bool eta_ghosts(H* h, const Geo& G, const Met<TM>& m, T* p, const BCs& bc, int p
     constexpr int TX = 32, TK = 8;
     const int hw = 2 * passes;
     const size_t lds = (size_t)(TX + 2 * hw) * (TK + 2 * hw) * 4 * sizeof(T);
-    if (lds > 65536) return false;
+    const long nrow = 2L * (G.nx + 2) * (G.nz + 2);
+    if (lds > 65536 || (size_t)nrow * sizeof(T) > h->snap_bytes) return false;
+    hipLaunchKernelGGL((k_ghost_rows<T>), dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, h->s, G,
+                       (const T*)p, (T*)h->snap);
     const int nb = 2 * ((G.nx + TX - 1) / TX) * ((G.nz + TK - 1) / TK);
-    hipLaunchKernelGGL((k_eta_ghosts<T, TM, TX, TK>), dim3(nb), dim3(1024), lds, h->s, G, m, p, bc, passes);
+    hipLaunchKernelGGL((k_eta_ghosts<T, TM, TX, TK>), dim3(nb), dim3(1024), lds, h->s, G, m, p, (const T*)h->snap,
+                       bc, passes);
     return true;
 }
 
@@ -394,10 +404,20 @@ int yline_mr(int m) { return (m == 4 || m == 8 || m == 16 || m == 32) ? m : 0; }
 template <int P, int MR, class T, class TM, class TR>
 void yline_go(H* h, Level& L, const Met<TM>& m, double inv_pot, const TR* rhs, int zin) {
     const ColLaunch cl = seg_launch(L.G, P);
-    if constexpr (MR > 0)
-        hipLaunchKernelGGL((k_yline<P, MR, T, TM, TR>), cl.grid, cl.block, 0, h->s, L.G, m, line_f(L),
-                           (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->omega_now, (T)inv_pot, zin);
-    else
+    if constexpr (MR > 0) {
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, cl.grid, cl.block, 0, h->s, L.G, m, line_f(L), (const T*)L.p[L.cur],
+                               (T*)L.p[1 - L.cur], rhs, (T)h->omega_now, (T)inv_pot, zin);
+        };
+        // the variant without the g13, g31 loads: measured faster for the 16-bit-metric
+        // and FP64 smoothers, slower for the FP32-metric one (kept with the loads)
+        if constexpr (sizeof(TM) == 2 || sizeof(T) == 8) {
+            if (L.x0) go(k_yline<P, MR, T, TM, TR, true>);
+            else go(k_yline<P, MR, T, TM, TR, false>);
+        } else {
+            go(k_yline<P, MR, T, TM, TR, false>);
+        }
+    } else
         hipLaunchKernelGGL((k_yline_staged<P, T, TM, TR>), cl.grid, cl.block, 0, h->s, L.G, m, line_f(L),
                            (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->omega_now, (T)inv_pot, zin);
 }
@@ -454,7 +474,8 @@ int smooth(H* h, Level& L, const TR* rhs, int nu, bool zero_start, bool fine) {
         const double stage = MR > 0 ? 0.0 : (4 * sp + 4);
         const double fac = otf ? 0.0 : 8.0;
         const double bytes = zin ? N * (sr + sm + (otf ? 2 * sm : 8.0) + sp + seg + stage)
-                                 : N * (9 * sm + sp + sr + fac + sp + seg + stage);
+                                 : N * (((MR > 0 && (sm == 2.0 || sizeof(T) == 8)) ? nmet(L) : 9.0) * sm + sp + sr +
+                                        fac + sp + seg + stage);
         {
             KT kt(h, fine ? PLES_MG_K_SMOOTH_FINE : PLES_MG_K_SMOOTH, bytes);
             switch (L.P) {
@@ -485,10 +506,13 @@ int vresid(H* h, Level& L, const TR* rhs, bool fine) {
         cl.grid = dim3((L.G.nx + L.rbx - 1) / L.rbx, L.G.nz, 1);
     }
     const double N = (double)L.ncell();
-    KT kt(h, fine ? PLES_MG_K_RESID_FINE : PLES_MG_K_RESID, N * (9 * sizeof(T) + 2 * sizeof(T) + sizeof(TR)));
-    hipLaunchKernelGGL((k_resid<0, T, T, TR, T>), cl.grid, cl.block, 0, h->s, L.G, met<T>(L),
-                       (const T*)L.p[L.cur], rhs, (T*)L.r, (T)(1.0 / L.pot), (const double*)nullptr,
-                       (const double*)nullptr, (double*)nullptr, 0, P);
+    KT kt(h, fine ? PLES_MG_K_RESID_FINE : PLES_MG_K_RESID, N * (nmet(L) * sizeof(T) + 2 * sizeof(T) + sizeof(TR)));
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, cl.grid, cl.block, 0, h->s, L.G, met<T>(L), (const T*)L.p[L.cur], rhs, (T*)L.r,
+                           (T)(1.0 / L.pot), (const double*)nullptr, (const double*)nullptr, (double*)nullptr, 0, P);
+    };
+    if (L.x0) go(k_resid<0, T, T, TR, T, double, true>);
+    else go(k_resid<0, T, T, TR, T, double, false>);
     CK(hipGetLastError());
     return PLES_MG_OK;
 }
@@ -517,9 +541,13 @@ int resid_restrict(H* h, Level& F, Level& Cc, const TR* rhs, bool fine) {
     const dim3 block((unsigned)bx, (unsigned)P, 1), grid((unsigned)((F.G.nx + bx - 1) / bx), (unsigned)(F.G.nz / 2));
     const double N = (double)F.ncell();
     KT kt(h, fine ? PLES_MG_K_RESID_FINE : PLES_MG_K_RESID,
-          N * (9 * sizeof(T) + sizeof(T) + sizeof(TR)) + (double)Cc.ncell() * sizeof(TC));
-    hipLaunchKernelGGL((k_resid_restrict<T, T, TR, TC>), grid, block, 0, h->s, F.G, met<T>(F),
-                       (const T*)F.p[F.cur], rhs, Cc.G, (TC*)Cc.b, (T)(1.0 / F.pot), P);
+          N * (nmet(F) * sizeof(T) + sizeof(T) + sizeof(TR)) + (double)Cc.ncell() * sizeof(TC));
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, grid, block, 0, h->s, F.G, met<T>(F), (const T*)F.p[F.cur], rhs, Cc.G, (TC*)Cc.b,
+                           (T)(1.0 / F.pot), P);
+    };
+    if (F.x0) go(k_resid_restrict<T, T, TR, TC, true>);
+    else go(k_resid_restrict<T, T, TR, TC, false>);
     CK(hipGetLastError());
     return PLES_MG_OK;
 }
@@ -680,10 +708,14 @@ int outer_residual(H* h, double* x, double* r) {
     const Level& L0 = h->lv[0];
     const ColLaunch cl = seg_launch(L0.G, L0.P);
     {
-        KT kt(h, PLES_MG_K_OUTER_APPLY, (double)L0.ncell() * (72 + 24));
-        hipLaunchKernelGGL((k_resid<1, double, double, double, double>), cl.grid, cl.block, 0, h->s, L0.G,
-                           met<double>(L0), (const double*)x, (const double*)h->bvec, r, 1.0,
-                           (const double*)nullptr, (const double*)nullptr, h->part, cl.nblocks, L0.P);
+        KT kt(h, PLES_MG_K_OUTER_APPLY, (double)L0.ncell() * (8 * nmet(L0) + 24));
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, cl.grid, cl.block, 0, h->s, L0.G, met<double>(L0), (const double*)x,
+                               (const double*)h->bvec, r, 1.0, (const double*)nullptr, (const double*)nullptr,
+                               h->part, cl.nblocks, L0.P);
+        };
+        if (L0.x0) go(k_resid<1, double, double, double, double, double, true>);
+        else go(k_resid<1, double, double, double, double, double, false>);
     }
     hipLaunchKernelGGL(k_fin_resid, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S);
     CK(hipGetLastError());
@@ -717,11 +749,14 @@ int step_body(H* h, int q) {
     CKS(halo64(h, Zq, false, true));
     const bool kry = h->o.krylov != 0;
     {
-        KT kt(h, PLES_MG_K_OUTER_APPLY, N0 * (72 + 24 + (kry ? 8 : 0)));
-        hipLaunchKernelGGL((k_resid<2, double, double, double, double>), cl.grid, cl.block, 0, h->s, L0.G,
-                           met<double>(L0), (const double*)Zq, (const double*)nullptr, Wq, 1.0,
-                           (const double*)h->rvec, (const double*)(kry ? Wp : nullptr), h->part, cl.nblocks,
-                           L0.P);
+        KT kt(h, PLES_MG_K_OUTER_APPLY, N0 * (8 * nmet(L0) + 24 + (kry ? 8 : 0)));
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, cl.grid, cl.block, 0, h->s, L0.G, met<double>(L0), (const double*)Zq,
+                               (const double*)nullptr, Wq, 1.0, (const double*)h->rvec,
+                               (const double*)(kry ? Wp : nullptr), h->part, cl.nblocks, L0.P);
+        };
+        if (L0.x0) go(k_resid<2, double, double, double, double, double, true>);
+        else go(k_resid<2, double, double, double, double, double, false>);
     }
     hipLaunchKernelGGL(k_fin_apply, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S,
                        h->o.krylov);
@@ -839,10 +874,14 @@ int step32_body(H* h, int q) {
     const double N0 = (double)L0.ncell();
     const bool kry = h->o.krylov != 0;
     {
-        KT kt(h, PLES_MG_K_OUTER_APPLY, N0 * (36 + 12 + (kry ? 4 : 0)));
-        hipLaunchKernelGGL((k_resid<2, float, float, float, float, float>), cl.grid, cl.block, 0, h->s, L0.G,
-                           met<float>(L0), (const float*)Zq, (const float*)nullptr, Wq, 1.0f,
-                           (const float*)h->rr32, (const float*)(kry ? Wp : nullptr), h->part, cl.nblocks, L0.P);
+        KT kt(h, PLES_MG_K_OUTER_APPLY, N0 * (4 * nmet(L0) + 12 + (kry ? 4 : 0)));
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, cl.grid, cl.block, 0, h->s, L0.G, met<float>(L0), (const float*)Zq,
+                               (const float*)nullptr, Wq, 1.0f, (const float*)h->rr32,
+                               (const float*)(kry ? Wp : nullptr), h->part, cl.nblocks, L0.P);
+        };
+        if (L0.x0) go(k_resid<2, float, float, float, float, float, true>);
+        else go(k_resid<2, float, float, float, float, float, false>);
     }
     hipLaunchKernelGGL(k_fin_apply, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S2,
                        h->o.krylov);
@@ -1390,6 +1429,28 @@ int build_coarse_dense(H* h, Level& L) {
     return PLES_MG_OK;
 }
 
+// Whether the xi-zeta cross metrics g13 and g31 are identically zero on a
+// level (orthogonal xi and zeta grid lines everywhere, e.g. terrain-following
+// grids over x-z separable meshes): then the stencil kernels are launched in
+// their variant that does not load them; the operator is the same.
+int find_x0(H* h, Level& L) {
+    const long n = L.G.n;
+    const int nb = (int)std::min<long>((n + 255) / 256, 1024);
+    double* part = nullptr;
+    CKS(dalloc(h, (void**)&part, (size_t)nb * 2 * sizeof(double)));
+    hipLaunchKernelGGL(k_absmax<double>, dim3(nb), dim3(256), 0, h->s, n, (const double*)L.g64[2], part);
+    hipLaunchKernelGGL(k_absmax<double>, dim3(nb), dim3(256), 0, h->s, n, (const double*)L.g64[6], part + nb);
+    CK(hipGetLastError());
+    std::vector<double> hp((size_t)nb * 2);
+    CK(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, h->s));
+    CK(hipStreamSynchronize(h->s));
+    dfree(h, part);
+    double mx = 0.0;
+    for (double v : hp) mx = std::max(mx, v);
+    L.x0 = mx == 0.0;
+    return PLES_MG_OK;
+}
+
 // 16-bit copy of an FP32 level's metrics for its smoother, scaled by 4^m so
 // that the largest normal metric lands in [2^8, 2^10): far from the FP16
 // overflow, and values down to 2^-24 of it stay normal numbers.  The scale
@@ -1401,7 +1462,8 @@ int make_half_metrics(H* h, Level& L) {
     CKS(dalloc(h, (void**)&part, (size_t)nb * 3 * sizeof(double)));
     const int diag[3] = {0, 4, 8};
     for (int d = 0; d < 3; ++d)
-        hipLaunchKernelGGL(k_absmax, dim3(nb), dim3(256), 0, h->s, cells, (const float*)L.g32[diag[d]], part + d * nb);
+        hipLaunchKernelGGL(k_absmax<float>, dim3(nb), dim3(256), 0, h->s, cells, (const float*)L.g32[diag[d]],
+                           part + d * nb);
     CK(hipGetLastError());
     std::vector<double> hp((size_t)nb * 3);
     CK(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, h->s));
@@ -1595,6 +1657,7 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
             CK(hipStreamSynchronize(h->s));
             dfree(h, staging);
         }
+        CKS(find_x0(h, L));
         if (L.f32) {
             for (int q = 0; q < 9; ++q) {
                 CKS(dalloc(h, (void**)&L.g32[q], cells * 4));
@@ -1633,6 +1696,9 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
         if (n < h->L - 1) CKS(dalloc(h, &L.r, cells * sp));
     }
     const size_t c0 = (size_t)h->lv[0].G.n;
+    h->snap_bytes = 2 * (size_t)(h->lv[0].G.nx + 2) * (h->lv[0].G.nz + 2) * sizeof(double);
+    CKS(dalloc(h, &h->snap, h->snap_bytes));
+    CK(hipGetLastError());
     CKS(dalloc(h, (void**)&h->x, c0 * 8));
     CKS(dalloc(h, (void**)&h->bvec, c0 * 8));
     CKS(dalloc(h, (void**)&h->rvec, c0 * 8));
