@@ -76,7 +76,9 @@ typedef struct ples_mg_opts {
     int stop_mode;   /* PLES_MG_STOP_REL_L2 (default) or PLES_MG_STOP_ABS_MAX */
     int project_rhs; /* 1 = FP64 compatibility projection of the rhs (default) */
     int halo_passes; /* red/black passes of the exact wall-ghost solve (0 = auto:
-                        6 with eta walls only, 24 with xi/zeta walls) */
+                        6 with eta walls only; with xi/zeta walls measured at
+                        create from the contraction of the passes on a test
+                        field, 24 to 400) */
     int device;      /* HIP device ordinal (default 0) */
     int graphs;      /* 1 = replay each refinement step from a captured HIP graph
                         (default), 0 = launch its kernels one by one; steps are

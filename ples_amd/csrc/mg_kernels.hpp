@@ -1773,6 +1773,23 @@ __global__ void __launch_bounds__(256) k_perturb(Geo G, double* __restrict__ a, 
     a[c] += rel * sqrt(var) * gauss(seed, 7u, (int64_t)c);
 }
 
+// interior = N(0,1) per (seed, cell) (a test field)
+__global__ void __launch_bounds__(256) k_fill_gauss(Geo G, double* __restrict__ a, uint64_t seed) {
+    long c;
+    if (!cell_of(G, c)) return;
+    a[c] = gauss(seed, 9u, (int64_t)c);
+}
+
+// per-block maxima of |a - b| over n values (grid-stride)
+__global__ void __launch_bounds__(256) k_absdiff_max(long n, const double* __restrict__ a,
+                                                     const double* __restrict__ b, double* partials) {
+    double v[1] = {0.0};
+    const bool mx[1] = {true};
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        v[0] = fmax(v[0], fabs(a[t] - b[t]));
+    block_reduce_store<1>(v, partials, gridDim.x, mx);
+}
+
 // r32 = s * r64 over the interior (the FP32 inner problem's rhs)
 __global__ void __launch_bounds__(256) k_scale_to32(Geo G, const double* __restrict__ r, double s,
                                                     float* __restrict__ r32) {

@@ -72,6 +72,7 @@ struct ples_mg_handle {
     bool have_cs = false;
     double dt = 1.0;
     double omega_now = 0.8;               // damping of the sweep being launched
+    int halo_passes32 = 6;                // wall-ghost passes of the FP32 inner direction
     void* snap = nullptr;                 // fine-level eta-wall ghost rows (k_eta_ghosts start)
     size_t snap_bytes = 0;
     double cheb_lo = 0.0, cheb_hi = 0.0;  // Chebyshev sweep schedule interval (0 = constant omega)
@@ -350,7 +351,7 @@ int halo(H* h, const Level& L, T* p, double an_over_dt, bool with_cs, int passes
 // fast: periodic xi and zeta with valid interior images -- only the eta-wall
 // ghosts are iterated (neighbours read through the periodic wrap) and each
 // pass writes their images, no edge launch.
-int halo64(H* h, double* p, bool with_bc, bool fast = false) {
+int halo64(H* h, double* p, bool with_bc, bool fast = false, int passes_override = 0) {
     const Level& L0 = h->lv[0];
     Met<double> m = met<double>(L0);
     const Geo& G = L0.G;
@@ -365,7 +366,7 @@ int halo64(H* h, double* p, bool with_bc, bool fast = false) {
     fast = fast && G.ip == 0 && G.kp == 0;
     // every pass counts: the outer GCR needs the ghosts at their fixed point
     // to machine precision, or the attainable residual stalls
-    const int passes = std::max(1, h->o.halo_passes);
+    const int passes = passes_override > 0 ? passes_override : std::max(1, h->o.halo_passes);
     {
         KT kt(h, PLES_MG_K_HALO, (double)(2L * G.nx * G.nz * 8 + (fast ? 0 : ne * 2)) * 8.0);
         if (eta_ghosts<double, double>(h, G, m, p, bc, passes)) {
@@ -870,7 +871,7 @@ int step32_body(H* h, int q) {
         hipLaunchKernelGGL((k_convert<float, float>), dim3(nb), dim3(256), 0, h->s, n, (const float*)L0.p[L0.cur],
                            Zq);
     }
-    CKS(halo<float>(h, L0, Zq, 0.0, false, std::max(1, h->o.halo_passes), true));
+    CKS(halo<float>(h, L0, Zq, 0.0, false, std::max(1, h->halo_passes32), true));
     const double N0 = (double)L0.ncell();
     const bool kry = h->o.krylov != 0;
     {
@@ -1451,6 +1452,64 @@ int find_x0(H* h, Level& L) {
     return PLES_MG_OK;
 }
 
+// Wall-ghost passes with xi or zeta walls (automatic count).  The red/black
+// passes contract slowly where the tangential cross metrics are large
+// against the normal one (strongly warped walled grids), and a fixed count
+// then leaves the outer residual short of its fixed point.  At create, one
+// pass at a time is applied to the ghosts of a Gaussian test field; the
+// largest ratio of successive ghost changes over the later passes estimates
+// the contraction rho, and the counts are those that take the first change
+// down to 1e-15 (FP64 fills) and 1e-8 (the FP32 inner direction), within
+// [24, 400] and [6, 400].
+int auto_halo_passes(H* h) {
+    const Level& L0 = h->lv[0];
+    const Geo& G = L0.G;
+    double* a = h->Z[0];
+    double* b = h->Z[1];
+    const long n = G.n;
+    const int nb = (int)std::min<long>((n + 255) / 256, 1024);
+    double* part = nullptr;
+    CKS(dalloc(h, (void**)&part, (size_t)nb * sizeof(double)));
+    CK(hipMemsetAsync(a, 0, (size_t)n * 8, h->s));
+    const ColLaunch ce = cell_launch(G);
+    hipLaunchKernelGGL(k_fill_gauss, ce.grid, ce.block, 0, h->s, G, a, (uint64_t)0x5eedULL);
+    CK(hipGetLastError());
+    std::vector<double> hp((size_t)nb);
+    double d1 = 0.0, dprev = 0.0, rho = 0.0;
+    int used = 0;
+    for (int ps = 1; ps <= 48; ++ps) {
+        CK(hipMemcpyAsync(b, a, (size_t)n * 8, hipMemcpyDeviceToDevice, h->s));
+        CKS(halo64(h, a, false, false, 1));
+        hipLaunchKernelGGL(k_absdiff_max, dim3(nb), dim3(256), 0, h->s, n, (const double*)a, (const double*)b, part);
+        CK(hipGetLastError());
+        CK(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, h->s));
+        CK(hipStreamSynchronize(h->s));
+        double d = 0.0;
+        for (double v : hp) d = std::max(d, v);
+        if (!std::isfinite(d)) {
+            dfree(h, part);
+            return PLES_MG_ERR_NONFINITE;
+        }
+        if (ps == 1) d1 = d;
+        else if (ps >= 4 && dprev > 0.0) rho = std::max(rho, d / dprev);
+        dprev = d;
+        used = ps;
+        if (d <= 1e-15 * std::max(d1, 1e-300)) break;
+    }
+    dfree(h, part);
+    CK(hipMemsetAsync(a, 0, (size_t)n * 8, h->s));
+    CK(hipMemsetAsync(b, 0, (size_t)n * 8, h->s));
+    auto count = [&](double target, int lo) {
+        if (rho <= 0.0) return std::max(lo, used);
+        if (rho >= 0.999) return 400;
+        const int k = (int)std::ceil(std::log(target) / std::log(rho)) + 1;
+        return std::min(400, std::max(lo, k));
+    };
+    h->o.halo_passes = count(1e-15, 24);
+    h->halo_passes32 = count(1e-8, 6);
+    return PLES_MG_OK;
+}
+
 // 16-bit copy of an FP32 level's metrics for its smoother, scaled by 4^m so
 // that the largest normal metric lands in [2^8, 2^10): far from the FP16
 // overflow, and values down to 2^-24 of it stay normal numbers.  The scale
@@ -1592,6 +1651,7 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
     if (o.precision != PLES_MG_FP64 && o.precision != PLES_MG_MIXED) return PLES_MG_ERR_ARG;
     if (o.nu_pre < 0 || o.nu_post < 0 || o.nu_coarse < 0 || !(o.omega > 0.0 && o.omega < 2.0))
         return PLES_MG_ERR_ARG;
+    const bool auto_passes = o.halo_passes <= 0;
     if (o.halo_passes <= 0) o.halo_passes = (g->ip == 1 || g->kp == 1) ? 24 : 6;
     if (o.inner_cycles < 0 || o.inner_cycles > 8) return PLES_MG_ERR_ARG;
     h->o = o;
@@ -1724,6 +1784,8 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
     h->part_n = std::max(col_launch(h->lv[0].G).nblocks, ((h->lv[0].G.nx + 31) / 32) * h->lv[0].G.nz);
     h->part_n = std::max(h->part_n, 1);
     CKS(dalloc(h, (void**)&h->part, (size_t)h->part_n * 4 * sizeof(double)));
+    h->halo_passes32 = h->o.halo_passes;
+    if (auto_passes && (g->ip == 1 || g->kp == 1)) CKS(auto_halo_passes(h));
     if (o.autotune)
         for (int n = 0; n < h->L; ++n) CKS(tune_level(h, n));
     CK(hipStreamSynchronize(h->s));

@@ -162,6 +162,22 @@ def test_exact_coarse_solve(ip, kp):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_walls_everywhere_strongly_warped(precision):
+    """Walls in every direction on a strongly warped grid (gamma 1.5, terrain
+    amplitude 0.2): the wall-ghost passes contract slowly there, and the
+    automatic pass count measured at create must carry the outer residual to
+    its fixed point (a fixed 24 passes stalls near 1e-7)."""
+    g = cases.grid(32, 16, 32, gamma=1.5, amp=0.2, ip=1, kp=1)
+    rhs_g = O.case_rhs(g, 92)
+    ref = _oracle_solution(g, 3, rhs_g)
+    with _solver(g, 3, precision) as s:
+        fi = np.zeros(s.fi_shape)
+        res = s.multi(fi, cases.interior(rhs_g), eps=1e-12, ficycle=300)
+    assert res.ok, res
+    assert cases.rel_l2_meanfree(fi, ref) < 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_solve_matches_oracle_sheared(precision):
     """Grid lines skewed in the xi-zeta plane as well (x shifted by a function
     of z, periodic over the zeta period): every cross metric is nonzero, so the
