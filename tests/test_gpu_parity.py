@@ -162,6 +162,26 @@ def test_exact_coarse_solve(ip, kp):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_is_deterministic(precision):
+    """The same solve gives bitwise the same pressure, cycles and residual when
+    repeated on one handle and on a second handle (deterministic reductions,
+    no kernel reading what another block of its launch writes)."""
+    g = cases.grid(64, 32, 64, gamma=1.5, amp=0.2)
+    rhs = cases.interior(O.case_rhs(g, 1004))
+    out = []
+    for rep in range(2):
+        with _solver(g, 4, precision) as s:
+            for _ in range(2):
+                fi = np.zeros(s.fi_shape)
+                res = s.multi(fi, rhs, eps=1e-11, ficycle=200)
+                assert res.ok, res
+                out.append((fi, res.cycles, res.rel_res))
+    for fi, cyc, rr in out[1:]:
+        assert cyc == out[0][1] and rr == out[0][2]
+        assert np.array_equal(fi, out[0][0])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_walls_everywhere_strongly_warped(precision):
     """Walls in every direction on a strongly warped grid (gamma 1.5, terrain
     amplitude 0.2): the wall-ghost passes contract slowly there, and the
