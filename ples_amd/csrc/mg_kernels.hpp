@@ -970,35 +970,12 @@ __global__ void __launch_bounds__(256) k_restrict(Geo Gc, Geo Gf, const TF* __re
 }
 
 // pf += trilinear cell-centred interpolation of ec over the ghost-inclusive
-// fine range (fsrc/prolong.f03:64-261): weight 3 per axis where the coarse
-// offset matches the fine child parity, 1 otherwise, over 64.
-template <class TC, class TF>
-__global__ void __launch_bounds__(256) k_prolong(Geo Gc, Geo Gf, const TC* __restrict__ ec,
-                                                 TF* __restrict__ pf) {
-    const int fi = blockIdx.x * blockDim.x + threadIdx.x;
-    const int fj = blockIdx.y * blockDim.y + threadIdx.y;
-    const int fk = blockIdx.z;
-    if (fi > Gf.nx + 1 || fj > Gf.ny + 1) return;
-    const int i = fi >> 1, j = fj >> 1, k = fk >> 1;
-    const int di = fi & 1, dj = fj & 1, dk = fk & 1;
-    const long c = Gc.idx(i, j, k);
-    const long sy = Gc.sy, sz = Gc.sz;
-    const TF wi0 = di ? (TF)1 : (TF)3, wi1 = di ? (TF)3 : (TF)1;
-    const TF wj0 = dj ? (TF)1 : (TF)3, wj1 = dj ? (TF)3 : (TF)1;
-    const TF wk0 = dk ? (TF)1 : (TF)3, wk1 = dk ? (TF)3 : (TF)1;
-    const TF s = (wi0 * wj0 * wk0) * (TF)ec[c] + (wi1 * wj0 * wk0) * (TF)ec[c + 1] +
-                 (wi0 * wj1 * wk0) * (TF)ec[c + sy] + (wi0 * wj0 * wk1) * (TF)ec[c + sz] +
-                 (wi1 * wj1 * wk0) * (TF)ec[c + 1 + sy] + (wi1 * wj0 * wk1) * (TF)ec[c + 1 + sz] +
-                 (wi0 * wj1 * wk1) * (TF)ec[c + sy + sz] + (wi1 * wj1 * wk1) * (TF)ec[c + 1 + sy + sz];
-    const long f = Gf.idx(fi, fj, fk);
-    pf[f] = pf[f] + s * (TF)(1.0 / 64.0);
-}
-
-// The same interpolation by coarse blocks: fine cells 2I, 2I+1 (and likewise
-// in j, k) take their values from the same coarse cells I, I+1, so one thread
-// per (I, J, K), I in 0..ncx (ghosts included), loads that 2x2x2 coarse block
-// once and updates its 8 fine children, two consecutive i at a time (the fine
-// strides are even, so the pair is aligned).
+// fine range (fsrc/prolong.f03:64-261): weight 3 per axis for the coarse cell
+// on the fine child's side, 1 for the other, over 64.  Fine cells 2I, 2I+1
+// (and likewise in j, k) take their values from the same coarse cells I, I+1,
+// so one thread per (I, J, K), I in 0..ncx (ghosts included), loads that
+// 2x2x2 coarse block once and updates its 8 fine children, two consecutive i
+// at a time (the fine strides are even, so the pair is aligned).
 template <class TC, class TF>
 __global__ void __launch_bounds__(256) k_prolong_blk(Geo Gc, Geo Gf, const TC* __restrict__ ec,
                                                      TF* __restrict__ pf) {
