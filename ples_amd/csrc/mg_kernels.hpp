@@ -775,12 +775,14 @@ __global__ void __launch_bounds__(512) k_resid(Geo G, Met<TM> M, const T* __rest
 // k_restrict).  Each thread walks the two zeta planes 2K-1, 2K of its column
 // together, one row pair at a time (two stencil windows), so the 2x2 (j, k)
 // part of a child block stays in registers; the i pair is the neighbouring
-// lane (bx even).  Segments have an even number of rows and start at an odd
-// row.  Block: bx lanes along i x P segments; XCD-aware block order.
+// lane (bx even); fx = 1 (a coarse level with the fine xi count): -(1/4) of
+// the four (j, k) children of the thread's own column.  Segments have an even
+// number of rows and start at an odd row.  Block: bx lanes along i x P
+// segments; XCD-aware block order.
 template <class T, class TM, class TR, class TC, bool X0 = false>
 __global__ void __launch_bounds__(512) k_resid_restrict(Geo G, Met<TM> M, const T* __restrict__ p,
                                                         const TR* __restrict__ b, Geo Gc, TC* __restrict__ rc,
-                                                        T inv_pot, int P) {
+                                                        T inv_pot, int P, int fx) {
     int bxi, bk;
     xcd_tile(bxi, bk);
     const int bx = blockDim.x, tx = threadIdx.x, s = threadIdx.y;
@@ -813,6 +815,10 @@ __global__ void __launch_bounds__(512) k_resid_restrict(Geo G, Met<TM> M, const 
                 win_shift(w0);
                 win_shift(w1);
             }
+        }
+        if (fx == 1) {  // coarse level with the fine xi count: four children
+            if (act) rc[Gc.idx(i, (j0 + 1) / 2 + rp, kc)] = (TC)(-v * (T)0.25);
+            continue;
         }
         v += __shfl_down(v, 1, 64);
         if (act && !(tx & 1)) rc[Gc.idx((i + 1) >> 1, (j0 + 1) / 2 + rp, kc)] = (TC)(-v * (T)0.125);
@@ -954,16 +960,24 @@ __global__ void k_gather_S(const double* const* __restrict__ S, int n, double* _
 // ------------------------------------------------------------------------
 // Transfers
 // ------------------------------------------------------------------------
-// rc = -(1/8) * sum of the eight children (fsrc/restrict_sndrcv.f03:32-53)
+// rc = -(1/8) * sum of the eight children (fsrc/restrict_sndrcv.f03:32-53);
+// fx = 1 (a coarse level with the fine xi count): -(1/4) * sum of the four
+// children in eta and zeta, the same average
 template <class TF, class TC>
 __global__ void __launch_bounds__(256) k_restrict(Geo Gc, Geo Gf, const TF* __restrict__ rf,
-                                                  TC* __restrict__ rc) {
+                                                  TC* __restrict__ rc, int fx) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
     if (i > Gc.nx || j > Gc.ny) return;
-    const long f = Gf.idx(2 * i, 2 * j, 2 * k);
     const long sy = Gf.sy, sz = Gf.sz;
+    if (fx == 1) {
+        const long f = Gf.idx(i, 2 * j, 2 * k);
+        const TF s = rf[f] + rf[f - sy] + rf[f - sz] + rf[f - sy - sz];
+        rc[Gc.idx(i, j, k)] = (TC)(-s * (TF)0.25);
+        return;
+    }
+    const long f = Gf.idx(2 * i, 2 * j, 2 * k);
     TF s = rf[f] + rf[f - 1] + rf[f - sy] + rf[f - 1 - sy] + rf[f - sz] + rf[f - 1 - sz] +
            rf[f - sy - sz] + rf[f - 1 - sy - sz];
     rc[Gc.idx(i, j, k)] = (TC)(-s * (TF)0.125);
@@ -1010,6 +1024,32 @@ __global__ void __launch_bounds__(256) k_prolong_blk(Geo Gc, Geo Gf, const TC* _
             v.x += ((TF)3 * s2[0] + s2[1]) * (TF)(1.0 / 64.0);
             v.y += (s2[0] + (TF)3 * s2[1]) * (TF)(1.0 / 64.0);
             *f = v;
+        }
+}
+
+// pf += bilinear (eta, zeta) interpolation of ec for a coarse level with the
+// fine xi count: the prolongation above without its xi factor (weights 3 and
+// 1 per axis over 16, the fine xi index taking the coarse one's value).  One
+// thread per coarse (i, J, K), i in 0..nx+1, J in 0..ncy, K in 0..ncz (ghosts
+// included), updating its four fine children.
+template <class TC, class TF>
+__global__ void __launch_bounds__(256) k_prolong_yz(Geo Gc, Geo Gf, const TC* __restrict__ ec,
+                                                    TF* __restrict__ pf) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int Kc = blockIdx.z;
+    if (I > Gc.nx + 1 || J > Gc.ny) return;
+    const long c = Gc.idx(I, J, Kc);
+    const long sy = Gc.sy, sz = Gc.sz;
+    const TF e00 = (TF)ec[c], e01 = (TF)ec[c + sy], e10 = (TF)ec[c + sz], e11 = (TF)ec[c + sy + sz];  // [k][j]
+#pragma unroll
+    for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+            const TF wk0 = dk ? (TF)1 : (TF)3, wk1 = dk ? (TF)3 : (TF)1;
+            const TF wj0 = dj ? (TF)1 : (TF)3, wj1 = dj ? (TF)3 : (TF)1;
+            pf[Gf.idx(I, 2 * J + dj, 2 * Kc + dk)] +=
+                (wk0 * (wj0 * e00 + wj1 * e01) + wk1 * (wj0 * e10 + wj1 * e11)) * (TF)(1.0 / 16.0);
         }
 }
 
@@ -1492,9 +1532,13 @@ __device__ inline void wall_deriv(const double x0[3], const double x1[3], const 
     }
 }
 
-// dir 0: xi faces (i = 0..nx), 1: eta faces (j = 0..ny), 2: zeta faces
+// dir 0: xi faces (i = 0..nx), 1: eta faces (j = 0..ny), 2: zeta faces.
+// st: node stride of the level in eta and zeta, sx: in xi (st, or st / 2 below
+// a level coarsened in eta and zeta only).  The metrics are divided by sx: with
+// the operator's 1/st^2 this scales the coarse-index flux sum by the number of
+// fine cells per coarse cell (sx st st), the reference's 1/pot for sx = st.
 template <class TO>
-__global__ void k_metrics(Nodes N, Geo G, int st, int dir, TO* g0, TO* g1, TO* g2, double* v0,
+__global__ void k_metrics(Nodes N, Geo G, int st, int sx, int dir, TO* g0, TO* g1, TO* g2, double* v0,
                           double* v1, double* v2) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y * blockDim.y + threadIdx.y;
@@ -1510,25 +1554,25 @@ __global__ void k_metrics(Nodes N, Geo G, int st, int dir, TO* g0, TO* g1, TO* g
         i = a + 1; j = b + 1; k = cidx;
         if (i > G.nx || j > G.ny) return;
     }
-    const double pot = (double)st;
+    const double pot = (double)sx;
     double c[3], e[3], t[3];
     if (dir == 0) {
-        const int j2 = j * st, j1 = (j - 1) * st, k2 = k * st, k1 = (k - 1) * st, ic = i * st;
+        const int j2 = j * st, j1 = (j - 1) * st, k2 = k * st, k1 = (k - 1) * st, ic = i * sx;
         auto face = [&](int aa, double o[3]) {
             const int q[4][3] = {{aa, j2, k2}, {aa, j2, k1}, {aa, j1, k2}, {aa, j1, k1}};
             avg4(N, q, o);
         };
         if (i == 0 && G.ip == 1) {
             double x0[3], x1[3], x2[3];
-            face(0, x0); face(st, x1); face(2 * st, x2);
+            face(0, x0); face(sx, x1); face(2 * sx, x2);
             wall_deriv(x0, x1, x2, 1.0, st, c);
         } else if (i == G.nx && G.ip == 1) {
             double x0[3], x1[3], x2[3];
-            face(N.nx, x0); face((i - 1) * st, x1); face((i - 2) * st, x2);
+            face(N.nx, x0); face((i - 1) * sx, x1); face((i - 2) * sx, x2);
             wall_deriv(x0, x1, x2, -1.0, st, c);
         } else {
             double xd[3], xs[3];
-            face((i + 1) * st, xd); face((i - 1) * st, xs);
+            face((i + 1) * sx, xd); face((i - 1) * sx, xs);
             for (int m = 0; m < 3; ++m) c[m] = .5 * (xd[m] - xs[m]);
         }
         hdiff(N, ic, j2, k2, ic, j2, k1, ic, j1, k2, ic, j1, k1, e);
@@ -1540,7 +1584,7 @@ __global__ void k_metrics(Nodes N, Geo G, int st, int dir, TO* g0, TO* g1, TO* g
         g2[id] = (TO)((dot3(F.cs, F.zt) / F.giac) / pot);
         if (v0) { v0[id] = F.cs[0]; v1[id] = F.cs[1]; v2[id] = F.cs[2]; }
     } else if (dir == 1) {
-        const int i2 = i * st, i1 = (i - 1) * st, k2 = k * st, k1 = (k - 1) * st, jc = j * st;
+        const int i2 = i * sx, i1 = (i - 1) * sx, k2 = k * st, k1 = (k - 1) * st, jc = j * st;
         auto face = [&](int bb, double o[3]) {
             const int q[4][3] = {{i2, bb, k2}, {i2, bb, k1}, {i1, bb, k1}, {i1, bb, k2}};
             avg4(N, q, o);
@@ -1567,7 +1611,7 @@ __global__ void k_metrics(Nodes N, Geo G, int st, int dir, TO* g0, TO* g1, TO* g
         g2[id] = (TO)((dot3(F.et, F.zt) / F.giac) / pot);
         if (v0) { v0[id] = F.et[0]; v1[id] = F.et[1]; v2[id] = F.et[2]; }
     } else {
-        const int i2 = i * st, i1 = (i - 1) * st, j2 = j * st, j1 = (j - 1) * st, kc = k * st;
+        const int i2 = i * sx, i1 = (i - 1) * sx, j2 = j * st, j1 = (j - 1) * st, kc = k * st;
         auto face = [&](int cc, double o[3]) {
             const int q[4][3] = {{i2, j2, cc}, {i2, j1, cc}, {i1, j1, cc}, {i1, j2, cc}};
             avg4(N, q, o);

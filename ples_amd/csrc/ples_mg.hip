@@ -25,7 +25,8 @@ namespace {
 struct Level {
     Geo G{};
     bool f32 = false;  // V-cycle precision of this level
-    double pot = 1.0;
+    double pot = 1.0;  // node stride in eta and zeta (2^n); the operator's 1/s
+    int sx = 1;        // node stride in xi: pot, or pot / 2 below a level coarsened in eta and zeta only
     double* g64[9] = {};
     float* g32[9] = {};
     _Float16* g16[9] = {};  // smoother copy of g32 * 4^m (FP32 levels, smoother_fp16)
@@ -523,7 +524,8 @@ int restrict_to(H* h, Level& F, Level& Cc) {
     dim3 block(std::min(64, pow2ceil(Cc.G.nx)), 4);
     dim3 grid((Cc.G.nx + block.x - 1) / block.x, (Cc.G.ny + 3) / 4, Cc.G.nz);
     KT kt(h, PLES_MG_K_TRANSFER, (double)F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC));
-    hipLaunchKernelGGL((k_restrict<TF, TC>), grid, block, 0, h->s, Cc.G, F.G, (const TF*)F.r, (TC*)Cc.b);
+    hipLaunchKernelGGL((k_restrict<TF, TC>), grid, block, 0, h->s, Cc.G, F.G, (const TF*)F.r, (TC*)Cc.b,
+                       F.G.nx / Cc.G.nx);
     CK(hipGetLastError());
     return PLES_MG_OK;
 }
@@ -536,16 +538,17 @@ int resid_restrict(H* h, Level& F, Level& Cc, const TR* rhs, bool fine) {
     const int P = F.rP > 0 ? F.rP : F.P;
     int bx = F.rbx > 0 ? F.rbx : seg_bx(P);
     const int m = F.G.ny / P;
-    if (m < 2 || (m & 1) || (F.G.nz & 1) || (F.G.nx & 1) || F.G.ny % P) return 1;
+    const int fx = F.G.nx / Cc.G.nx;  // 1: the coarse level keeps the xi count
+    if (m < 2 || (m & 1) || (F.G.nz & 1) || (fx == 2 && (F.G.nx & 1)) || F.G.ny % P) return 1;
     while (bx * P > 512 && bx > 2) bx >>= 1;
-    if ((bx * P) % 64 != 0 || (bx & 1)) return 1;
+    if ((bx * P) % 64 != 0 || (fx == 2 && (bx & 1))) return 1;
     const dim3 block((unsigned)bx, (unsigned)P, 1), grid((unsigned)((F.G.nx + bx - 1) / bx), (unsigned)(F.G.nz / 2));
     const double N = (double)F.ncell();
     KT kt(h, fine ? PLES_MG_K_RESID_FINE : PLES_MG_K_RESID,
           N * (nmet(F) * sizeof(T) + sizeof(T) + sizeof(TR)) + (double)Cc.ncell() * sizeof(TC));
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, grid, block, 0, h->s, F.G, met<T>(F), (const T*)F.p[F.cur], rhs, Cc.G, (TC*)Cc.b,
-                           (T)(1.0 / F.pot), P);
+                           (T)(1.0 / F.pot), P, fx);
     };
     if (F.x0) go(k_resid_restrict<T, T, TR, TC, true>);
     else go(k_resid_restrict<T, T, TR, TC, false>);
@@ -555,6 +558,15 @@ int resid_restrict(H* h, Level& F, Level& Cc, const TR* rhs, bool fine) {
 
 template <class TC, class TF>
 int prolong_to(H* h, Level& Cc, Level& F) {
+    if (Cc.G.nx == F.G.nx) {  // coarsened in eta and zeta only: 4 fine children per coarse cell
+        dim3 block(std::min(64, pow2ceil(Cc.G.nx + 2)), 4);
+        dim3 grid((Cc.G.nx + 2 + block.x - 1) / block.x, (Cc.G.ny + 1 + 3) / 4, Cc.G.nz + 1);
+        KT kt(h, PLES_MG_K_TRANSFER, 2.0 * F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC));
+        hipLaunchKernelGGL((k_prolong_yz<TC, TF>), grid, block, 0, h->s, Cc.G, F.G, (const TC*)Cc.p[Cc.cur],
+                           (TF*)F.p[F.cur]);
+        CK(hipGetLastError());
+        return PLES_MG_OK;
+    }
     // one thread per coarse block (I, J, K) in 0..nc (ghosts included): 8 fine children each
     dim3 block(std::min(64, pow2ceil(Cc.G.nx + 1)), 4);
     dim3 grid((Cc.G.nx + 1 + block.x - 1) / block.x, (Cc.G.ny + 1 + 3) / 4, Cc.G.nz + 1);
@@ -633,6 +645,10 @@ constexpr long kCoarseDenseMax = 512;
 // 256x128x256 in both precisions and at 512x256x512; the cycle counts are
 // sensitive to the lower end, [0.3, 2.0] costs five more)
 constexpr double kChebLo = 0.48, kChebHi = 2.4;
+
+// zeta-to-xi coupling ratio (sum g33 / sum g11 on the fine level) from which
+// the first coarse level keeps the xi count (opts.semicoarsen)
+constexpr double kSemiRatio = 2.0;
 
 bool coarse_one_block(const Level& L) {
     return !L.f32 && L.G.ip == 0 && L.G.kp == 0 && (long)L.G.nx * L.G.nz <= 1024;
@@ -1138,7 +1154,7 @@ Nodes nodes_of(const H* h) {
 // FP64 metrics of level n into g[9] (ghosted, zero elsewhere); optional face vectors
 int compute_metrics(H* h, int n, double* const g[9], double* const v[9]) {
     const Geo& G = h->lv[n].G;
-    const int st = 1 << n;
+    const int st = 1 << n, sx = h->lv[n].sx;
     const Nodes N = nodes_of(h);
     for (int dir = 0; dir < 3; ++dir) {
         const int na = (dir == 0) ? G.nx + 1 : G.nx;
@@ -1146,7 +1162,7 @@ int compute_metrics(H* h, int n, double* const g[9], double* const v[9]) {
         const int nc = (dir == 2) ? G.nz + 1 : G.nz;
         dim3 block(std::min(64, pow2ceil(na)), 4);
         dim3 grid((na + block.x - 1) / block.x, (nbv + 3) / 4, nc);
-        hipLaunchKernelGGL((k_metrics<double>), grid, block, 0, h->s, N, G, st, dir, g[3 * dir],
+        hipLaunchKernelGGL((k_metrics<double>), grid, block, 0, h->s, N, G, st, sx, dir, g[3 * dir],
                            g[3 * dir + 1], g[3 * dir + 2], v ? v[3 * dir] : nullptr, v ? v[3 * dir + 1] : nullptr,
                            v ? v[3 * dir + 2] : nullptr);
     }
@@ -1452,6 +1468,22 @@ int find_x0(H* h, Level& L) {
     return PLES_MG_OK;
 }
 
+// interior sum of metric m of level L (FP64 metrics), column sums on the device
+int metric_sum(H* h, const Level& L, int m, double& out) {
+    const ColLaunch cl = col_launch(L.G);
+    double* part = nullptr;
+    CKS(dalloc(h, (void**)&part, (size_t)cl.nblocks * 2 * sizeof(double)));
+    hipLaunchKernelGGL(k_sum, cl.grid, cl.block, 0, h->s, L.G, (const double*)L.g64[m], part, cl.nblocks);
+    CK(hipGetLastError());
+    std::vector<double> hp((size_t)cl.nblocks);
+    CK(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, h->s));
+    CK(hipStreamSynchronize(h->s));
+    dfree(h, part);
+    out = 0.0;
+    for (double v : hp) out += v;
+    return PLES_MG_OK;
+}
+
 // Wall-ghost passes with xi or zeta walls (automatic count).  The red/black
 // passes contract slowly where the tangential cross metrics are large
 // against the normal one (strongly warped walled grids), and a fixed count
@@ -1543,12 +1575,15 @@ int make_half_metrics(H* h, Level& L) {
     return PLES_MG_OK;
 }
 
+// An upper bound where the first coarse level may keep the xi count
+// (opts.semicoarsen: decided at create from the metrics).
 size_t estimate(const ples_mg_grid* g, const ples_mg_opts* o) {
     size_t tot = 0;
     const bool mixed = o && o->precision == PLES_MG_MIXED;
+    const bool semi = o && o->semicoarsen;
     for (int n = 0; n < g->nlevel; ++n) {
-        const int st = 1 << n;
-        const size_t cells = (size_t)(g->jx / st + 2) * (g->jy / st + 2) * (g->jz / st + 2);
+        const int st = 1 << n, sx = (semi && n > 0) ? st / 2 : st;
+        const size_t cells = (size_t)(g->jx / sx + 2) * (g->jy / st + 2) * (g->jz / st + 2);
         const bool f32 = mixed && n < g->nlevel - 1;
         const size_t sp = f32 ? 4 : 8;
         tot += cells * 9 * sp;                  // V-cycle metrics
@@ -1558,8 +1593,15 @@ size_t estimate(const ples_mg_grid* g, const ples_mg_opts* o) {
         tot += cells * sp * 2;                  // ping-pong
         if (n > 0) tot += cells * sp;           // rhs
         if (n < g->nlevel - 1) tot += cells * sp;  // residual
-        const size_t ni = (size_t)(g->jx / st) * (g->jy / st) * (g->jz / st);
-        if (n == g->nlevel - 1 && o && o->coarse_exact && (long)ni <= kCoarseDenseMax) tot += cells * ni * 8;
+        if (n == g->nlevel - 1 && o && o->coarse_exact) {  // dense map of either hierarchy, the larger
+            size_t dense = 0;
+            for (int f : {sx, st}) {
+                const size_t ni = (size_t)(g->jx / f) * (g->jy / st) * (g->jz / st);
+                const size_t cf = (size_t)(g->jx / f + 2) * (g->jy / st + 2) * (g->jz / st + 2);
+                if ((long)ni <= kCoarseDenseMax) dense = std::max(dense, cf * ni * 8);
+            }
+            tot += dense;
+        }
     }
     const size_t c0 = (size_t)(g->jx + 2) * (g->jy + 2) * (g->jz + 2);
     tot += c0 * 8 * 7;                                             // x, b, r, Z[2], W[2]
@@ -1601,6 +1643,7 @@ void ples_mg_default_opts(ples_mg_opts* o) {
     o->smoother_fp16 = 1;
     o->coarse_exact = 1;
     o->sweep_schedule = 1;
+    o->semicoarsen = 0;
 }
 
 const char* ples_mg_status_string(int s) {
@@ -1695,10 +1738,16 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
         h->cheb_hi = kChebHi;
         h->cheb_lo = kChebLo;
     }
+    // first coarse level coarsened in eta and zeta only (xi count kept) when the
+    // fine zeta coupling dominates the xi coupling: the eta-line smoother treats
+    // both explicitly and leaves error smooth in zeta but oscillatory in xi,
+    // which a level with half the xi count cannot represent
+    bool semi = false;
     for (int n = 0; n < h->L; ++n) {
         Level& L = h->lv[n];
         const int st = 1 << n;
-        L.G = make_geo(g->jx / st, g->jy / st, g->jz / st, g->ip, g->jp, g->kp);
+        L.sx = (semi && n > 0) ? st / 2 : st;
+        L.G = make_geo(g->jx / L.sx, g->jy / st, g->jz / st, g->ip, g->jp, g->kp);
         L.pot = (double)st;
         L.f32 = mixed && n < h->L - 1;
         const size_t cells = (size_t)L.G.n;
@@ -1718,6 +1767,12 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
             dfree(h, staging);
         }
         CKS(find_x0(h, L));
+        if (n == 0 && o.semicoarsen && h->L >= 2) {
+            double s11 = 0.0, s33 = 0.0;
+            CKS(metric_sum(h, L, 0, s11));
+            CKS(metric_sum(h, L, 8, s33));
+            semi = s33 >= kSemiRatio * s11;
+        }
         if (L.f32) {
             for (int q = 0; q < 9; ++q) {
                 CKS(dalloc(h, (void**)&L.g32[q], cells * 4));
@@ -2830,6 +2885,15 @@ int ples_mg_batch_solve(ples_mg_batch* b, double eps, int maxcycles, int steps_p
     return worst;
 }
 
+
+int ples_mg_level_shape(const ples_mg_handle* h, int level, int* nx, int* ny, int* nz) {
+    if (!h || level < 1 || level > h->L) return PLES_MG_ERR_ARG;
+    const Geo& G = h->lv[level - 1].G;
+    if (nx) *nx = G.nx;
+    if (ny) *ny = G.ny;
+    if (nz) *nz = G.nz;
+    return PLES_MG_OK;
+}
 
 int ples_mg_tiles(const ples_mg_handle* h, int level, int* P, int* rP, int* rbx) {
     if (!h || level < 1 || level > h->L) return PLES_MG_ERR_ARG;

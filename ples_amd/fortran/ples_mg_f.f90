@@ -35,7 +35,7 @@ module ples_mg_f
       integer(c_int) :: nu_pre, nu_post, nu_coarse
       real(c_double) :: omega
       integer(c_int) :: krylov, stop_mode, project_rhs, halo_passes, device, graphs, inner_cycles
-      integer(c_int) :: autotune, fp32_krylov, smoother_fp16, coarse_exact, sweep_schedule
+      integer(c_int) :: autotune, fp32_krylov, smoother_fp16, coarse_exact, sweep_schedule, semicoarsen
    end type ples_mg_opts
 
    type, bind(C), public :: ples_mg_result

@@ -57,7 +57,7 @@ class _Opts(C.Structure):
                 ("stop_mode", C.c_int), ("project_rhs", C.c_int), ("halo_passes", C.c_int),
                 ("device", C.c_int), ("graphs", C.c_int), ("inner_cycles", C.c_int),
                 ("autotune", C.c_int), ("fp32_krylov", C.c_int), ("smoother_fp16", C.c_int),
-                ("coarse_exact", C.c_int), ("sweep_schedule", C.c_int)]
+                ("coarse_exact", C.c_int), ("sweep_schedule", C.c_int), ("semicoarsen", C.c_int)]
 
 
 class _Result(C.Structure):
@@ -111,6 +111,7 @@ def lib():
             [C.c_double, _dp]
         L.ples_mg_metrica.argtypes = [C.c_void_p, C.POINTER(_dp), C.POINTER(_dp), _dp]
         L.ples_mg_level_metrics.argtypes = [C.c_void_p, C.c_int, C.POINTER(_dp)]
+        L.ples_mg_level_shape.argtypes = [C.c_void_p, C.c_int] + [C.POINTER(C.c_int)] * 3
         L.ples_mg_diver.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, _dp]
         L.ples_mg_gradie.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_double] + [_dp] * 6
         L.ples_mg_update.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp]
@@ -360,10 +361,16 @@ class Solver:
                                      (_dp * 9)(*[_p(a) for a in v]), _p(giac)), "ples_mg_metrica")
         return g, v, giac
 
+    def level_shape(self, level: int):
+        """(nx, ny, nz) cell counts of level `level` (1-based)."""
+        v = [C.c_int(), C.c_int(), C.c_int()]
+        _check(lib().ples_mg_level_shape(self._h, level, *[C.byref(a) for a in v]), "ples_mg_level_shape")
+        return tuple(a.value for a in v)
+
     def level_metrics(self, level: int):
         """FP64 metrics of level `level` (1-based) in the ghosted layout."""
-        st = 1 << (level - 1)
-        sh = (self.jz // st + 2, self.jy // st + 2, self.jx // st + 2)
+        nx, ny, nz = self.level_shape(level)
+        sh = (nz + 2, ny + 2, nx + 2)
         g = [np.zeros(sh) for _ in range(9)]
         _check(lib().ples_mg_level_metrics(self._h, level, (_dp * 9)(*[_p(a) for a in g])),
                "ples_mg_level_metrics")

@@ -34,7 +34,7 @@ def test_status_strings_and_defaults():
     o = mg.default_opts()
     assert o["precision"] == mg.FP64 and o["krylov"] == 1 and o["nu_pre"] == 3 and o["nu_post"] == 3
     assert o["graphs"] == 1 and o["fp32_krylov"] == 1 and o["autotune"] == 0 and o["smoother_fp16"] == 1
-    assert o["coarse_exact"] == 1 and o["sweep_schedule"] == 1
+    assert o["coarse_exact"] == 1 and o["sweep_schedule"] == 1 and o["semicoarsen"] == 0
 
 
 def test_tuning_table_parses():
