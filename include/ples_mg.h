@@ -78,7 +78,8 @@ typedef struct ples_mg_opts {
     int halo_passes; /* red/black passes of the exact wall-ghost solve (0 = auto:
                         6 with eta walls only; with xi/zeta walls measured at
                         create from the contraction of the passes on a test
-                        field, 24 to 400) */
+                        field, 24 to 400; create fails with
+                        PLES_MG_ERR_UNSUPPORTED where the passes diverge) */
     int device;      /* HIP device ordinal (default 0) */
     int graphs;      /* 1 = replay each refinement step from a captured HIP graph
                         (default), 0 = launch its kernels one by one; steps are

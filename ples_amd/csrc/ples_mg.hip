@@ -1507,9 +1507,10 @@ int auto_halo_passes(H* h) {
     hipLaunchKernelGGL(k_fill_gauss, ce.grid, ce.block, 0, h->s, G, a, (uint64_t)0x5eedULL);
     CK(hipGetLastError());
     std::vector<double> hp((size_t)nb);
-    double d1 = 0.0, dprev = 0.0, rho = 0.0;
+    double d1 = 0.0, d4 = 0.0, dprev = 0.0, rho = 0.0;
     int used = 0;
-    for (int ps = 1; ps <= 48; ++ps) {
+    constexpr int kMaxProbe = 48;
+    for (int ps = 1; ps <= kMaxProbe; ++ps) {
         CK(hipMemcpyAsync(b, a, (size_t)n * 8, hipMemcpyDeviceToDevice, h->s));
         CKS(halo64(h, a, false, false, 1));
         hipLaunchKernelGGL(k_absdiff_max, dim3(nb), dim3(256), 0, h->s, n, (const double*)a, (const double*)b, part);
@@ -1524,6 +1525,7 @@ int auto_halo_passes(H* h) {
         }
         if (ps == 1) d1 = d;
         else if (ps >= 4 && dprev > 0.0) rho = std::max(rho, d / dprev);
+        if (ps == 4) d4 = d;
         dprev = d;
         used = ps;
         if (d <= 1e-15 * std::max(d1, 1e-300)) break;
@@ -1531,6 +1533,11 @@ int auto_halo_passes(H* h) {
     dfree(h, part);
     CK(hipMemsetAsync(a, 0, (size_t)n * 8, h->s));
     CK(hipMemsetAsync(b, 0, (size_t)n * 8, h->s));
+    // changes still growing after all probe passes: the face relations have no
+    // reachable fixed point on this grid (the reference's ghost iteration,
+    // fsrc/multigrid.f03:1109-1317, diverges there too); refuse the grid here
+    // rather than return non-finite pressures from the solve
+    if (used == kMaxProbe && dprev > d4) return PLES_MG_ERR_UNSUPPORTED;
     auto count = [&](double target, int lo) {
         if (rho <= 0.0) return std::max(lo, used);
         if (rho >= 0.999) return 400;
