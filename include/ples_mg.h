@@ -112,14 +112,14 @@ typedef struct ples_mg_opts {
                         [0.48, 2.4]), which damps that part of the smoother's
                         spectrum by 1 / T_nu(1.5) together (default); 0 = omega
                         for every sweep */
-    int semicoarsen; /* 1 = the first coarse level keeps the fine xi count when
-                        the fine level's zeta coupling is at least twice its xi
-                        coupling (sum g33 >= 2 sum g11, e.g. dz <= dx / 1.41),
-                        coarsening eta and zeta only: fewer V-cycles on
-                        anisotropic grids without terrain, more on the stretched
-                        terrain grids; 0 = every level halves all three counts,
-                        the reference's hierarchy (fsrc/multigrid.f03:2293-3052)
-                        (default).  The converged pressure is the same. */
+    int semicoarsen; /* first coarse level when the fine level's zeta coupling
+                        is at least twice its xi coupling (sum g33 >= 2 sum g11,
+                        e.g. dz <= dx / 1.41): 1 = halves zeta only (keeps the
+                        xi and eta counts), 2 = halves eta and zeta (keeps xi);
+                        the levels below halve all three.  0 = every level
+                        halves all three counts, the reference's hierarchy
+                        (fsrc/multigrid.f03:2293-3052) (default).  The
+                        converged pressure is the same. */
 } ples_mg_opts;
 
 typedef struct ples_mg_result {
@@ -173,10 +173,11 @@ int ples_mg_metrica(ples_mg_handle* h, double* const g[9], double* const csv[9],
  * f.  For parity tests. */
 int ples_mg_level_metrics(ples_mg_handle* h, int level, double* const g[9]);
 
-/* Cell counts of multigrid level n (1-based): jy/2^(n-1), jz/2^(n-1) and
- * jx/2^(n-1), or jx/2^(n-2) below a level coarsened in eta and zeta only
- * (opts.semicoarsen); the reference's jxc/jyc/jzc (fsrc/iniz_metrica.f03:22-47)
- * for a full-coarsening hierarchy.  Outputs may be NULL. */
+/* Cell counts of multigrid level n (1-based): jz/2^(n-1), and jx/2^(n-1),
+ * jy/2^(n-1), or jx/2^(n-2) (and jy/2^(n-2)) below a first coarse level that
+ * kept those counts (opts.semicoarsen); the reference's jxc/jyc/jzc
+ * (fsrc/iniz_metrica.f03:22-47) for a full-coarsening hierarchy.  Outputs may
+ * be NULL. */
 int ples_mg_level_shape(const ples_mg_handle* h, int level, int* nx, int* ny, int* nz);
 
 /* rhs = (div of contravariant fluxes)/dt (fsrc/diver.f03:20-28) with

@@ -961,16 +961,21 @@ __global__ void k_gather_S(const double* const* __restrict__ S, int n, double* _
 // Transfers
 // ------------------------------------------------------------------------
 // rc = -(1/8) * sum of the eight children (fsrc/restrict_sndrcv.f03:32-53);
-// fx = 1 (a coarse level with the fine xi count): -(1/4) * sum of the four
-// children in eta and zeta, the same average
+// a coarse level that keeps the xi (fx = 1) or the xi and eta counts (fx = fy
+// = 1) averages its four or two children the same way
 template <class TF, class TC>
 __global__ void __launch_bounds__(256) k_restrict(Geo Gc, Geo Gf, const TF* __restrict__ rf,
-                                                  TC* __restrict__ rc, int fx) {
+                                                  TC* __restrict__ rc, int fx, int fy) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
     if (i > Gc.nx || j > Gc.ny) return;
     const long sy = Gf.sy, sz = Gf.sz;
+    if (fx == 1 && fy == 1) {
+        const long f = Gf.idx(i, j, 2 * k);
+        rc[Gc.idx(i, j, k)] = (TC)(-(rf[f] + rf[f - sz]) * (TF)0.5);
+        return;
+    }
     if (fx == 1) {
         const long f = Gf.idx(i, 2 * j, 2 * k);
         const TF s = rf[f] + rf[f - sy] + rf[f - sz] + rf[f - sy - sz];
@@ -1051,6 +1056,23 @@ __global__ void __launch_bounds__(256) k_prolong_yz(Geo Gc, Geo Gf, const TC* __
             pf[Gf.idx(I, 2 * J + dj, 2 * Kc + dk)] +=
                 (wk0 * (wj0 * e00 + wj1 * e01) + wk1 * (wj0 * e10 + wj1 * e11)) * (TF)(1.0 / 16.0);
         }
+}
+
+// pf += linear zeta interpolation of ec for a coarse level with the fine xi
+// and eta counts (weights 3 and 1 over 4).  One thread per coarse (i, j, K),
+// i in 0..nx+1, j in 0..ny+1, K in 0..ncz (ghosts included), updating its two
+// fine children (i, j, 2K) and (i, j, 2K+1).
+template <class TC, class TF>
+__global__ void __launch_bounds__(256) k_prolong_z(Geo Gc, Geo Gf, const TC* __restrict__ ec,
+                                                   TF* __restrict__ pf) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int Kc = blockIdx.z;
+    if (I > Gc.nx + 1 || J > Gc.ny + 1) return;
+    const long c = Gc.idx(I, J, Kc);
+    const TF e0 = (TF)ec[c], e1 = (TF)ec[c + Gc.sz];
+    pf[Gf.idx(I, J, 2 * Kc)] += ((TF)3 * e0 + e1) * (TF)0.25;
+    pf[Gf.idx(I, J, 2 * Kc + 1)] += (e0 + (TF)3 * e1) * (TF)0.25;
 }
 
 // ------------------------------------------------------------------------
@@ -1533,13 +1555,15 @@ __device__ inline void wall_deriv(const double x0[3], const double x1[3], const 
 }
 
 // dir 0: xi faces (i = 0..nx), 1: eta faces (j = 0..ny), 2: zeta faces.
-// st: node stride of the level in eta and zeta, sx: in xi (st, or st / 2 below
-// a level coarsened in eta and zeta only).  The metrics are divided by sx: with
-// the operator's 1/st^2 this scales the coarse-index flux sum by the number of
-// fine cells per coarse cell (sx st st), the reference's 1/pot for sx = st.
+// sx, sy, sz: node strides of the level (2^(n-1) each for the reference's
+// level n; a coarse level below one that kept the xi or the xi and eta counts
+// has half the stride there); coarse: the level is not the finest (the wall
+// derivative fallback).  The metrics are divided by div = sx sy / sz: with the
+// operator's 1/sz^2 this scales the coarse-index flux sum by the number of fine
+// cells per coarse cell (sx sy sz), the reference's 1/pot when all are pot.
 template <class TO>
-__global__ void k_metrics(Nodes N, Geo G, int st, int sx, int dir, TO* g0, TO* g1, TO* g2, double* v0,
-                          double* v1, double* v2) {
+__global__ void k_metrics(Nodes N, Geo G, int sx, int sy, int sz, int coarse, int dir, TO* g0, TO* g1, TO* g2,
+                          double* v0, double* v1, double* v2) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y * blockDim.y + threadIdx.y;
     const int cidx = blockIdx.z;
@@ -1554,10 +1578,11 @@ __global__ void k_metrics(Nodes N, Geo G, int st, int sx, int dir, TO* g0, TO* g
         i = a + 1; j = b + 1; k = cidx;
         if (i > G.nx || j > G.ny) return;
     }
-    const double pot = (double)sx;
+    const double pot = (double)sx * (double)sy / (double)sz;
+    const int st = coarse ? 2 : 1;  // wall_deriv: first-order fallback on coarse levels
     double c[3], e[3], t[3];
     if (dir == 0) {
-        const int j2 = j * st, j1 = (j - 1) * st, k2 = k * st, k1 = (k - 1) * st, ic = i * sx;
+        const int j2 = j * sy, j1 = (j - 1) * sy, k2 = k * sz, k1 = (k - 1) * sz, ic = i * sx;
         auto face = [&](int aa, double o[3]) {
             const int q[4][3] = {{aa, j2, k2}, {aa, j2, k1}, {aa, j1, k2}, {aa, j1, k1}};
             avg4(N, q, o);
@@ -1584,22 +1609,22 @@ __global__ void k_metrics(Nodes N, Geo G, int st, int sx, int dir, TO* g0, TO* g
         g2[id] = (TO)((dot3(F.cs, F.zt) / F.giac) / pot);
         if (v0) { v0[id] = F.cs[0]; v1[id] = F.cs[1]; v2[id] = F.cs[2]; }
     } else if (dir == 1) {
-        const int i2 = i * sx, i1 = (i - 1) * sx, k2 = k * st, k1 = (k - 1) * st, jc = j * st;
+        const int i2 = i * sx, i1 = (i - 1) * sx, k2 = k * sz, k1 = (k - 1) * sz, jc = j * sy;
         auto face = [&](int bb, double o[3]) {
             const int q[4][3] = {{i2, bb, k2}, {i2, bb, k1}, {i1, bb, k1}, {i1, bb, k2}};
             avg4(N, q, o);
         };
         if (j == 0 && G.jp == 1) {
             double x0[3], x1[3], x2[3];
-            face(0, x0); face(st, x1); face(2 * st, x2);
+            face(0, x0); face(sy, x1); face(2 * sy, x2);
             wall_deriv(x0, x1, x2, 1.0, st, e);
         } else if (j == G.ny && G.jp == 1) {
             double x0[3], x1[3], x2[3];
-            face(N.ny, x0); face((j - 1) * st, x1); face((j - 2) * st, x2);
+            face(N.ny, x0); face((j - 1) * sy, x1); face((j - 2) * sy, x2);
             wall_deriv(x0, x1, x2, -1.0, st, e);
         } else {
             double xp[3], xm[3];
-            face((j + 1) * st, xp); face((j - 1) * st, xm);
+            face((j + 1) * sy, xp); face((j - 1) * sy, xm);
             for (int m = 0; m < 3; ++m) e[m] = .5 * (xp[m] - xm[m]);
         }
         hdiff(N, i2, jc, k2, i1, jc, k2, i2, jc, k1, i1, jc, k1, t);
@@ -1611,22 +1636,22 @@ __global__ void k_metrics(Nodes N, Geo G, int st, int sx, int dir, TO* g0, TO* g
         g2[id] = (TO)((dot3(F.et, F.zt) / F.giac) / pot);
         if (v0) { v0[id] = F.et[0]; v1[id] = F.et[1]; v2[id] = F.et[2]; }
     } else {
-        const int i2 = i * sx, i1 = (i - 1) * sx, j2 = j * st, j1 = (j - 1) * st, kc = k * st;
+        const int i2 = i * sx, i1 = (i - 1) * sx, j2 = j * sy, j1 = (j - 1) * sy, kc = k * sz;
         auto face = [&](int cc, double o[3]) {
             const int q[4][3] = {{i2, j2, cc}, {i2, j1, cc}, {i1, j1, cc}, {i1, j2, cc}};
             avg4(N, q, o);
         };
         if (k == 0 && G.kp == 1) {
             double x0[3], x1[3], x2[3];
-            face(0, x0); face(st, x1); face(2 * st, x2);
+            face(0, x0); face(sz, x1); face(2 * sz, x2);
             wall_deriv(x0, x1, x2, 1.0, st, t);
         } else if (k == G.nz && G.kp == 1) {
             double x0[3], x1[3], x2[3];
-            face(N.nz, x0); face((k - 1) * st, x1); face((k - 2) * st, x2);
+            face(N.nz, x0); face((k - 1) * sz, x1); face((k - 2) * sz, x2);
             wall_deriv(x0, x1, x2, -1.0, st, t);
         } else {
             double xa[3], xb[3];
-            face((k + 1) * st, xa); face((k - 1) * st, xb);
+            face((k + 1) * sz, xa); face((k - 1) * sz, xb);
             for (int m = 0; m < 3; ++m) t[m] = .5 * (xa[m] - xb[m]);
         }
         hdiff(N, i2, j2, kc, i2, j1, kc, i1, j2, kc, i1, j1, kc, c);

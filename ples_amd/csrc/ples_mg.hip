@@ -26,7 +26,7 @@ struct Level {
     Geo G{};
     bool f32 = false;  // V-cycle precision of this level
     double pot = 1.0;  // node stride in eta and zeta (2^n); the operator's 1/s
-    int sx = 1;        // node stride in xi: pot, or pot / 2 below a level coarsened in eta and zeta only
+    int sx = 1, sy = 1;  // node strides in xi and eta: pot, or pot / 2 below a level that kept the count
     double* g64[9] = {};
     float* g32[9] = {};
     _Float16* g16[9] = {};  // smoother copy of g32 * 4^m (FP32 levels, smoother_fp16)
@@ -525,7 +525,7 @@ int restrict_to(H* h, Level& F, Level& Cc) {
     dim3 grid((Cc.G.nx + block.x - 1) / block.x, (Cc.G.ny + 3) / 4, Cc.G.nz);
     KT kt(h, PLES_MG_K_TRANSFER, (double)F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC));
     hipLaunchKernelGGL((k_restrict<TF, TC>), grid, block, 0, h->s, Cc.G, F.G, (const TF*)F.r, (TC*)Cc.b,
-                       F.G.nx / Cc.G.nx);
+                       F.G.nx / Cc.G.nx, F.G.ny / Cc.G.ny);
     CK(hipGetLastError());
     return PLES_MG_OK;
 }
@@ -539,6 +539,7 @@ int resid_restrict(H* h, Level& F, Level& Cc, const TR* rhs, bool fine) {
     int bx = F.rbx > 0 ? F.rbx : seg_bx(P);
     const int m = F.G.ny / P;
     const int fx = F.G.nx / Cc.G.nx;  // 1: the coarse level keeps the xi count
+    if (F.G.ny == Cc.G.ny) return 1;  // eta count kept too: residual, then the two-child restriction
     if (m < 2 || (m & 1) || (F.G.nz & 1) || (fx == 2 && (F.G.nx & 1)) || F.G.ny % P) return 1;
     while (bx * P > 512 && bx > 2) bx >>= 1;
     if ((bx * P) % 64 != 0 || (fx == 2 && (bx & 1))) return 1;
@@ -558,6 +559,15 @@ int resid_restrict(H* h, Level& F, Level& Cc, const TR* rhs, bool fine) {
 
 template <class TC, class TF>
 int prolong_to(H* h, Level& Cc, Level& F) {
+    if (Cc.G.ny == F.G.ny) {  // coarsened in zeta only: 2 fine children per coarse cell
+        dim3 block(std::min(64, pow2ceil(Cc.G.nx + 2)), 4);
+        dim3 grid((Cc.G.nx + 2 + block.x - 1) / block.x, (Cc.G.ny + 2 + 3) / 4, Cc.G.nz + 1);
+        KT kt(h, PLES_MG_K_TRANSFER, 2.0 * F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC));
+        hipLaunchKernelGGL((k_prolong_z<TC, TF>), grid, block, 0, h->s, Cc.G, F.G, (const TC*)Cc.p[Cc.cur],
+                           (TF*)F.p[F.cur]);
+        CK(hipGetLastError());
+        return PLES_MG_OK;
+    }
     if (Cc.G.nx == F.G.nx) {  // coarsened in eta and zeta only: 4 fine children per coarse cell
         dim3 block(std::min(64, pow2ceil(Cc.G.nx + 2)), 4);
         dim3 grid((Cc.G.nx + 2 + block.x - 1) / block.x, (Cc.G.ny + 1 + 3) / 4, Cc.G.nz + 1);
@@ -647,7 +657,7 @@ constexpr long kCoarseDenseMax = 512;
 constexpr double kChebLo = 0.48, kChebHi = 2.4;
 
 // zeta-to-xi coupling ratio (sum g33 / sum g11 on the fine level) from which
-// the first coarse level keeps the xi count (opts.semicoarsen)
+// the first coarse level halves zeta only, or eta and zeta (opts.semicoarsen)
 constexpr double kSemiRatio = 2.0;
 
 bool coarse_one_block(const Level& L) {
@@ -1154,7 +1164,7 @@ Nodes nodes_of(const H* h) {
 // FP64 metrics of level n into g[9] (ghosted, zero elsewhere); optional face vectors
 int compute_metrics(H* h, int n, double* const g[9], double* const v[9]) {
     const Geo& G = h->lv[n].G;
-    const int st = 1 << n, sx = h->lv[n].sx;
+    const int st = 1 << n, sx = h->lv[n].sx, sy = h->lv[n].sy;
     const Nodes N = nodes_of(h);
     for (int dir = 0; dir < 3; ++dir) {
         const int na = (dir == 0) ? G.nx + 1 : G.nx;
@@ -1162,7 +1172,7 @@ int compute_metrics(H* h, int n, double* const g[9], double* const v[9]) {
         const int nc = (dir == 2) ? G.nz + 1 : G.nz;
         dim3 block(std::min(64, pow2ceil(na)), 4);
         dim3 grid((na + block.x - 1) / block.x, (nbv + 3) / 4, nc);
-        hipLaunchKernelGGL((k_metrics<double>), grid, block, 0, h->s, N, G, st, sx, dir, g[3 * dir],
+        hipLaunchKernelGGL((k_metrics<double>), grid, block, 0, h->s, N, G, sx, sy, st, n > 0 ? 1 : 0, dir, g[3 * dir],
                            g[3 * dir + 1], g[3 * dir + 2], v ? v[3 * dir] : nullptr, v ? v[3 * dir + 1] : nullptr,
                            v ? v[3 * dir + 2] : nullptr);
     }
@@ -1582,15 +1592,16 @@ int make_half_metrics(H* h, Level& L) {
     return PLES_MG_OK;
 }
 
-// An upper bound where the first coarse level may keep the xi count
-// (opts.semicoarsen: decided at create from the metrics).
+// An upper bound where the first coarse level may keep the xi (and eta)
+// count (opts.semicoarsen: decided at create from the metrics).
 size_t estimate(const ples_mg_grid* g, const ples_mg_opts* o) {
     size_t tot = 0;
     const bool mixed = o && o->precision == PLES_MG_MIXED;
     const bool semi = o && o->semicoarsen;
     for (int n = 0; n < g->nlevel; ++n) {
         const int st = 1 << n, sx = (semi && n > 0) ? st / 2 : st;
-        const size_t cells = (size_t)(g->jx / sx + 2) * (g->jy / st + 2) * (g->jz / st + 2);
+        const int sy = (semi && n > 0 && o->semicoarsen == 1) ? st / 2 : st;
+        const size_t cells = (size_t)(g->jx / sx + 2) * (g->jy / sy + 2) * (g->jz / st + 2);
         const bool f32 = mixed && n < g->nlevel - 1;
         const size_t sp = f32 ? 4 : 8;
         tot += cells * 9 * sp;                  // V-cycle metrics
@@ -1603,8 +1614,9 @@ size_t estimate(const ples_mg_grid* g, const ples_mg_opts* o) {
         if (n == g->nlevel - 1 && o && o->coarse_exact) {  // dense map of either hierarchy, the larger
             size_t dense = 0;
             for (int f : {sx, st}) {
-                const size_t ni = (size_t)(g->jx / f) * (g->jy / st) * (g->jz / st);
-                const size_t cf = (size_t)(g->jx / f + 2) * (g->jy / st + 2) * (g->jz / st + 2);
+                const int fy = f == st ? st : sy;
+                const size_t ni = (size_t)(g->jx / f) * (g->jy / fy) * (g->jz / st);
+                const size_t cf = (size_t)(g->jx / f + 2) * (g->jy / fy + 2) * (g->jz / st + 2);
                 if ((long)ni <= kCoarseDenseMax) dense = std::max(dense, cf * ni * 8);
             }
             tot += dense;
@@ -1704,6 +1716,7 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
     const bool auto_passes = o.halo_passes <= 0;
     if (o.halo_passes <= 0) o.halo_passes = (g->ip == 1 || g->kp == 1) ? 24 : 6;
     if (o.inner_cycles < 0 || o.inner_cycles > 8) return PLES_MG_ERR_ARG;
+    if (o.semicoarsen < 0 || o.semicoarsen > 2) return PLES_MG_ERR_ARG;
     h->o = o;
     h->inner = o.inner_cycles > 0 ? o.inner_cycles : 1;
     h->dev = o.device;
@@ -1745,16 +1758,18 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
         h->cheb_hi = kChebHi;
         h->cheb_lo = kChebLo;
     }
-    // first coarse level coarsened in eta and zeta only (xi count kept) when the
-    // fine zeta coupling dominates the xi coupling: the eta-line smoother treats
-    // both explicitly and leaves error smooth in zeta but oscillatory in xi,
-    // which a level with half the xi count cannot represent
+    // first coarse level coarsened in zeta only (semicoarsen 1) or in eta and
+    // zeta (2) when the fine zeta coupling dominates the xi coupling: the
+    // eta-line smoother treats xi and zeta explicitly, and the error it leaves
+    // oscillatory in xi (and, where the eta spacing is wide, in eta) is not
+    // representable on a level with half those counts
     bool semi = false;
     for (int n = 0; n < h->L; ++n) {
         Level& L = h->lv[n];
         const int st = 1 << n;
         L.sx = (semi && n > 0) ? st / 2 : st;
-        L.G = make_geo(g->jx / L.sx, g->jy / st, g->jz / st, g->ip, g->jp, g->kp);
+        L.sy = (semi && n > 0 && o.semicoarsen == 1) ? st / 2 : st;
+        L.G = make_geo(g->jx / L.sx, g->jy / L.sy, g->jz / st, g->ip, g->jp, g->kp);
         L.pot = (double)st;
         L.f32 = mixed && n < h->L - 1;
         const size_t cells = (size_t)L.G.n;

@@ -19,31 +19,36 @@ def _solver(g, nlevel, precision="fp64", metrics=None, **kw):
     return mg.Solver(g.x, g.y, g.z, g.ip, g.jp, g.kp, nlevel, metrics=metrics, precision=precision, **kw)
 
 
-def _refine_xi(a):
-    """Node array (nz+1, ny+1, nx+1) -> (nz+1, ny+1, 2nx+1): the nodes at even
-    xi indices, midpoints between them."""
-    out = np.empty(a.shape[:2] + (2 * a.shape[2] - 1,))
+def _refine(a, axis):
+    """Node array (nz+1, ny+1, nx+1) refined by two along `axis` (2: xi, 1: eta):
+    the nodes at the even indices, midpoints between them."""
+    a = np.moveaxis(a, axis, -1)
+    out = np.empty(a.shape[:-1] + (2 * a.shape[-1] - 1,))
     out[..., ::2] = a
     out[..., 1::2] = 0.5 * (a[..., 1:] + a[..., :-1])
-    return out
+    return np.ascontiguousarray(np.moveaxis(out, -1, axis))
 
 
-def _oracle_level_metrics(g, lev, nx):
-    """Oracle metrics of level `lev` (1-based) for a device level of nx xi
-    cells.  A level keeping twice the full-coarsening xi count (coarsened in
-    eta and zeta only below the fine level) has node stride 2^(lev-2) in xi and
-    2^(lev-1) in eta and zeta: the oracle's level `lev` on the nodes refined by
-    two in xi reads exactly those nodes (the midpoints never), and divides by
-    2^(lev-1) where the device divides by the xi stride, a factor 2."""
-    f = nx * (1 << (lev - 1)) // g.nx
-    if f == 1:
+def _oracle_level_metrics(g, lev, nx, ny):
+    """Oracle metrics of level `lev` (1-based) for a device level of nx, ny
+    cells.  A level keeping twice the full-coarsening xi (eta) count has node
+    stride 2^(lev-2) there and 2^(lev-1) in zeta: the oracle's level `lev` on
+    the nodes refined by two in that direction reads exactly those nodes (the
+    midpoints never), and divides by 2^(lev-1) where the device divides by
+    sx sy / sz, a factor 2 per kept direction."""
+    st = 1 << (lev - 1)
+    fx, fy = nx * st // g.nx, ny * st // g.ny
+    if fx == 1 and fy == 1:
         return O.metrics(g, lev - 1)[0]
-    assert f == 2
-    gu = O.Grid(2 * g.nx, g.ny, g.nz, g.ip, g.jp, g.kp, _refine_xi(g.x), _refine_xi(g.y), _refine_xi(g.z))
-    return [2.0 * r for r in O.metrics(gu, lev - 1)[0]]
+    x, y, z = g.x, g.y, g.z
+    for f, axis in ((fx, 2), (fy, 1)):
+        if f == 2:
+            x, y, z = _refine(x, axis), _refine(y, axis), _refine(z, axis)
+    gu = O.Grid(fx * g.nx, fy * g.ny, g.nz, g.ip, g.jp, g.kp, x, y, z)
+    return [fx * fy * r for r in O.metrics(gu, lev - 1)[0]]
 
 
-@pytest.mark.parametrize("semi", [1, 0])
+@pytest.mark.parametrize("semi", [1, 2, 0])
 @pytest.mark.parametrize("shape,gamma,amp,ip,kp,nlevel", [
     ((32, 16, 32), 1.5, 0.2, 0, 0, 3),
     ((24, 16, 16), 1.0, 0.15, 1, 1, 3),
@@ -52,18 +57,22 @@ def _oracle_level_metrics(g, lev, nx):
 ])
 def test_level_metrics_match_oracle(shape, gamma, amp, ip, kp, nlevel, semi):
     """Every level against the oracle's mul_met.  The grids with dz <= dx/1.41
-    (the first, third and fourth) get a first coarse level with the fine xi
-    count under semicoarsen=1; semicoarsen=0 is the reference's hierarchy."""
+    (the first, third and fourth) get a first coarse level that keeps the xi
+    and eta counts (semicoarsen=1) or the xi count (2); semicoarsen=0 is the
+    reference's hierarchy."""
     g = cases.grid(*shape, gamma=gamma, amp=amp, ip=ip, kp=kp)
     with _solver(g, nlevel, semicoarsen=semi) as s:
-        kept = s.level_shape(2)[0] == g.nx if nlevel >= 2 else False
-        assert kept == (semi == 1 and shape != (24, 16, 16))
+        anis = shape != (24, 16, 16)
+        kx = semi in (1, 2) and anis
+        ky = semi == 1 and anis
         for lev in range(1, nlevel + 1):
             mine = s.level_metrics(lev)
             nx, ny, nz = s.level_shape(lev)
-            assert (ny, nz) == (g.ny >> (lev - 1), g.nz >> (lev - 1))
-            assert nx == g.nx >> (lev - 1 - (1 if kept and lev > 1 else 0))
-            ref = _oracle_level_metrics(g, lev, nx)
+            up = 1 if lev > 1 else 0
+            assert nz == g.nz >> (lev - 1)
+            assert nx == g.nx >> (lev - 1 - (up if kx else 0))
+            assert ny == g.ny >> (lev - 1 - (up if ky else 0))
+            ref = _oracle_level_metrics(g, lev, nx, ny)
             for q in range(9):
                 scale = np.abs(ref[q]).max() + 1e-300
                 err = np.abs(mine[q] - ref[q]).max() / scale
@@ -276,27 +285,27 @@ def test_sweep_schedule_same_solution(precision):
     ((32, 16, 32), 1.0, 0.0, 1, 1, 3),
 ])
 def test_semicoarsening_same_solution(precision, shape, gamma, amp, ip, kp, nlevel):
-    """A first coarse level coarsened in eta and zeta only (zeta coupling four
-    times the xi coupling here) changes the hierarchy, not the solution: both
-    hierarchies converge to the oracle's pressure.  On the uniform grid the
-    xi-preserving one needs fewer cycles (measured 8 against 9 FP64 with GCR,
-    12 against 15 plain V-cycles); on the stretched terrain grid more."""
+    """A first coarse level coarsened in zeta only (semicoarsen=1) or in eta
+    and zeta (2) (zeta coupling four times the xi coupling here) changes the
+    hierarchy, not the solution: all three hierarchies converge to the
+    oracle's pressure.  On the uniform grid the eta-zeta one needs fewer
+    cycles (measured 9 against 11 FP64 with GCR; zeta only: 11)."""
     g = cases.grid(*shape, gamma=gamma, amp=amp, ip=ip, kp=kp)
     rhs_g = O.case_rhs(g, 1004)
     if ip or kp:
         O.project(g, rhs_g, None, 1.0)
     ref = _oracle_solution(g, nlevel, rhs_g)
     cyc = {}
-    for semi in (0, 1):
+    for semi in (0, 1, 2):
         with _solver(g, nlevel, precision, semicoarsen=semi) as s:
-            assert (s.level_shape(2)[0] == g.nx) == (semi == 1)
+            assert s.level_shape(2) == (g.nx >> (0 if semi else 1), g.ny >> (0 if semi == 1 else 1), g.nz >> 1)
             fi = np.zeros(s.fi_shape)
             res = s.multi(fi, cases.interior(rhs_g), eps=1e-12, ficycle=200)
             assert res.ok, (semi, res)
             cyc[semi] = res.cycles
         assert cases.rel_l2_meanfree(fi, ref) < 1e-9, semi
     if gamma == 0.0 and amp == 0.0:
-        assert cyc[1] < cyc[0], cyc
+        assert cyc[2] < cyc[0] and cyc[1] <= cyc[0], cyc
 
 
 def test_diverging_wall_ghosts_refused_at_create():
