@@ -73,6 +73,20 @@ def time_solves(s, steps, warmup, tol, prepare, timing_fams=None):
     return dt, res, cycles
 
 
+def rel_l2_meanfree(p, q):
+    """||(p - mean p) - (q - mean q)||_2 / ||q - mean q||_2 over the interior of
+    two ghosted pressures (the pressure is defined up to a constant)."""
+    a = p[1:-1, 1:-1, 1:-1]
+    b = q[1:-1, 1:-1, 1:-1]
+    a = a - a.mean()
+    b = b - b.mean()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+# tolerance of the FP64 reference pressure every result is compared with
+REF_TOL = 1e-12
+
+
 def traffic_per_launch():
     """HBM bytes per launch of the fine-level mixed smoother at the default
     workload, from the committed rocprofv3 PMC summary (FETCH_SIZE scaled by
@@ -250,7 +264,11 @@ def main():
                 fams = []
             dt, res, cyc = time_solves(s, args.steps, args.warmup, cfg.tol, from_zero, fams)
             stats = {f: s.kernel_stats(f) for f in ("smooth_fine", "resid_fine")}
-            out[prec] = dict(seconds=dt, res=res, cycles=cyc, stats=stats)
+            out[prec] = dict(seconds=dt, res=res, cycles=cyc, stats=stats, fi=s.get_fi())
+            if prec == "fp64":  # reference pressure: the FP64 solve continued to REF_TOL
+                s.set_timing(None)
+                s.solve_resident(REF_TOL, 400)
+                out["ref_fi"] = s.get_fi()
             print(f"[bench] {cfg.name} {prec}: {args.steps / dt:.2f} solves/s, {1e3 * dt / args.steps:.2f} ms/solve, "
                   f"cycles {cyc[-1]}, rel_res {res.rel_res:.2e}", file=sys.stderr, flush=True)
 
@@ -264,7 +282,7 @@ def main():
     c3 = None
     if not args.no_c3 and args.config != "c3":
         c3cfg = PC.CONFIGS["c3"]
-        c3 = {}
+        c3, c3_fi = {}, {}
         for prec in ("mixed", "fp64"):
             with solver_for(c3cfg, prec) as s:
                 s.case_rhs(c3cfg.seed)
@@ -281,8 +299,15 @@ def main():
                 c3[prec] = {"solves_per_s": steps3 / dt, "ms_per_solve": 1e3 * dt / steps3,
                             "cycles": cyc[-1], "cycles_from_zero": base.cycles, "rel_res": res.rel_res,
                             "smoother_fine_GBs": (by3 / n3) / ((ms3 / n3) * 1e-3) / 1e9 if n3 else None}
+                c3_fi[prec] = s.get_fi()
+                if prec == "fp64":
+                    s.set_timing(None)
+                    s.solve_resident(REF_TOL, 400)
+                    c3_fi["ref"] = s.get_fi()
                 print(f"[bench] c3 {prec}: {c3[prec]}", file=sys.stderr, flush=True)
         c3["speedup_mixed_vs_fp64"] = c3["mixed"]["solves_per_s"] / c3["fp64"]["solves_per_s"]
+        c3["pressure_rel_l2_vs_fp64_1e-12"] = {p: rel_l2_meanfree(c3_fi[p], c3_fi["ref"]) for p in ("mixed", "fp64")}
+        del c3_fi
 
     c4 = None
     if not args.no_c4:
@@ -320,6 +345,7 @@ def main():
         "speedup_mixed_vs_fp64": fp64["seconds"] / mixed["seconds"],
         "cycles": {"mixed": mixed["cycles"][-1], "fp64": fp64["cycles"][-1]},
         "rel_res": {"mixed": mixed["res"].rel_res, "fp64": fp64["res"].rel_res},
+        "pressure_rel_l2_vs_fp64_1e-12": {p: rel_l2_meanfree(out[p]["fi"], out["ref_fi"]) for p in ("mixed", "fp64")},
         "roofline": {
             "kernel": "k_yline<float, 16-bit metrics> fine level (eta-line smoother)",
             "bound": "hbm",

@@ -1,0 +1,114 @@
+"""Parity at the benchmark workloads' full sizes (256x128x256, 6 levels, and
+512x256x512, 7 levels: terrain, eta-stretched, periodic in xi and zeta),
+through properties that do not depend on size:
+
+- the device operator against the oracle's on random fields (<= 1e-11
+  relative, as at the small sizes);
+- the converged pressure's residual recomputed by the oracle's operator from
+  the oracle's metrics (the device's stop test, 1e-10 relative L2, holds to
+  rounding);
+- mixed precision against FP64 (1e-9 relative L2 of the mean-free pressure:
+  at 1e-10 relative residual on the 512x256x512 grid, at 1e-12 on the
+  256x128x256 one);
+- linearity: the solve of a b1 + b2 is a x1 + x2 to the solves' tolerance.
+
+The oracle's metrics and residual are plain C over the whole grid (a few
+seconds at 8.4M cells); its SOR V-cycle is not run at these sizes.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from ples_amd import cases as PC
+from ples_amd import mg
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid(name):
+    cfg = PC.CONFIGS[name]
+    x, y, z = cfg.nodes()
+    return cfg, O.Grid(cfg.nx, cfg.ny, cfg.nz, 0, 1, 0, x, y, z)
+
+
+def _solver(cfg, g, precision):
+    return mg.Solver(g.x, g.y, g.z, 0, 1, 0, cfg.nlevel, precision=precision)
+
+
+def _oracle_resid(g, gm, fi, rhs):
+    """Oracle residual A p - b of a ghosted pressure whose eta-wall ghosts are
+    first brought to the oracle's fixed point (zero wall fluxes)."""
+    p = fi.copy()
+    O.ghost_fill(g, 0, p, gm, None, 0.0, 1.0, 40)
+    q = p.copy()
+    O.ghost_fill(g, 0, q, gm, None, 0.0, 1.0, 10)
+    assert np.abs(q - p).max() <= 1e-13 * np.abs(p).max()  # ghost fixed point reached
+    return cases.interior(O.resid(g, 0, p, cases.ghosted(g, rhs), gm))
+
+
+def test_operator_matches_oracle_256x128x256():
+    cfg, g = _grid("c2")
+    gm, _, _ = O.metrics(g, 0)
+    rng = np.random.default_rng(11)
+    fi = rng.standard_normal(g.gshape())
+    rhs = rng.standard_normal((g.nz, g.ny, g.nx))
+    ref = _oracle_resid(g, gm, fi, rhs)
+    with _solver(cfg, g, "fp64") as s:
+        r = s.residual(fi, rhs)
+    err = np.abs(r - ref).max() / np.abs(ref).max()
+    assert err < 1e-11, err
+
+
+def test_solutions_checked_by_oracle_operator_256x128x256():
+    """Both precisions from zero to 1e-10 on the device-generated divergence
+    rhs: the oracle's residual of each returned pressure is within rounding of
+    the stop test.  Continued to 1e-12, the two pressures agree to 1e-9 (at
+    1e-10 they differ by about 2e-8 here: a relative residual of 1e-10 leaves
+    the smoothest error modes of this grid at that level)."""
+    cfg, g = _grid("c2")
+    gm, _, _ = O.metrics(g, 0)
+    sols = {}
+    for prec in ("fp64", "mixed"):
+        with _solver(cfg, g, prec) as s:
+            s.case_rhs(cfg.seed)
+            s.zero_fi()
+            res = s.solve_resident(cfg.tol, 200)
+            assert res.ok and res.rel_res < cfg.tol, (prec, res)
+            fi, rhs = s.get_fi(), s.get_rhs()
+            r = _oracle_resid(g, gm, fi, rhs)
+            rel = np.linalg.norm(r) / np.linalg.norm(rhs)
+            assert rel < 1.5 * cfg.tol, (prec, rel, res.rel_res)
+            res = s.solve_resident(1e-12, 200)
+            assert res.ok, (prec, res)
+            sols[prec] = s.get_fi()
+    assert cases.rel_l2_meanfree(sols["mixed"], sols["fp64"]) < 1e-9
+
+
+def test_mixed_equals_fp64_and_linearity_512x256x512():
+    """At 512x256x512: mixed and FP64 pressures agree to 1e-9; and with two
+    rhs b1 (the device-generated divergence) and b2 (seeded noise, mean
+    removed), the mixed solve of 2 b1 + b2 equals 2 x1 + x2 to 1e-9 (each
+    solve converged to 1e-12)."""
+    cfg, g = _grid("c3")
+    sols = {}
+    for prec in ("fp64", "mixed"):
+        with _solver(cfg, g, prec) as s:
+            s.case_rhs(cfg.seed)
+            s.zero_fi()
+            res = s.solve_resident(cfg.tol, 200)
+            assert res.ok, (prec, res)
+            sols[prec] = s.get_fi()
+            if prec == "mixed":
+                b1 = s.get_rhs()
+                b2 = np.random.default_rng(5).standard_normal(b1.shape)
+                b2 -= b2.mean()
+                xs = []
+                for b in (b1, b2, 2.0 * b1 + b2):
+                    s.set_rhs(b)
+                    s.zero_fi()
+                    r = s.solve_resident(1e-12, 300)
+                    assert r.ok, r
+                    xs.append(s.get_fi())
+    assert cases.rel_l2_meanfree(sols["mixed"], sols["fp64"]) < 1e-9
+    assert cases.rel_l2_meanfree(xs[2], 2.0 * xs[0] + xs[1]) < 1e-9
