@@ -307,6 +307,7 @@ def main():
                 print(f"[bench] c3 {prec}: {c3[prec]}", file=sys.stderr, flush=True)
         c3["speedup_mixed_vs_fp64"] = c3["mixed"]["solves_per_s"] / c3["fp64"]["solves_per_s"]
         c3["pressure_rel_l2_vs_fp64_1e-12"] = {p: rel_l2_meanfree(c3_fi[p], c3_fi["ref"]) for p in ("mixed", "fp64")}
+        c3["pressure_rel_l2_mixed_vs_fp64"] = rel_l2_meanfree(c3_fi["mixed"], c3_fi["fp64"])
         del c3_fi
 
     c4 = None
@@ -346,6 +347,7 @@ def main():
         "cycles": {"mixed": mixed["cycles"][-1], "fp64": fp64["cycles"][-1]},
         "rel_res": {"mixed": mixed["res"].rel_res, "fp64": fp64["res"].rel_res},
         "pressure_rel_l2_vs_fp64_1e-12": {p: rel_l2_meanfree(out[p]["fi"], out["ref_fi"]) for p in ("mixed", "fp64")},
+        "pressure_rel_l2_mixed_vs_fp64": rel_l2_meanfree(out["mixed"]["fi"], out["fp64"]["fi"]),
         "roofline": {
             "kernel": "k_yline<float, 16-bit metrics> fine level (eta-line smoother)",
             "bound": "hbm",
