@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace plesmg {
 
@@ -27,6 +28,50 @@ struct Geo {
 template <class TM>
 struct Met {
     const TM* g[9];  // g11 g12 g13 | g21 g22 g23 | g31 g32 g33
+};
+
+// 16-bit smoother metrics of a level whose g13 and g31 vanish, packed per
+// cell so a row of the smoother issues one 8-byte, one 4-byte and one 2-byte
+// metric load instead of seven 2-byte ones (more bytes in flight per
+// register): a[c] = (g11, g12, g32, g33), b[c] = (g21, g23), g22 apart.
+using H4 = _Float16 __attribute__((ext_vector_type(4)));
+using H2 = _Float16 __attribute__((ext_vector_type(2)));
+struct MetPk {
+    const H4* a;
+    const H2* b;
+    const _Float16* g22;
+};
+
+// metric q (0..8 as in Met) of cell c
+template <class TM>
+__host__ __device__ inline TM mg(const Met<TM>& M, int q, long c) {
+    return M.g[q][c];
+}
+__device__ inline _Float16 mg(const MetPk& M, int q, long c) {
+    switch (q) {
+        case 0: return M.a[c].x;
+        case 1: return M.a[c].y;
+        case 3: return M.b[c].x;
+        case 4: return M.g22[c];
+        case 5: return M.b[c].y;
+        case 7: return M.a[c].z;
+        case 8: return M.a[c].w;
+        default: return (_Float16)0;  // g13, g31
+    }
+}
+
+// metric views read as 16-bit values by the smoother
+template <class MV>
+struct MVHalf {
+    static constexpr bool value = false;
+};
+template <>
+struct MVHalf<Met<_Float16>> {
+    static constexpr bool value = true;
+};
+template <>
+struct MVHalf<MetPk> {
+    static constexpr bool value = true;
 };
 
 // ------------------------------------------------------------------------
@@ -124,14 +169,14 @@ struct G2 {
 // xi-zeta cross metrics g13, g31 are identically zero (orthogonal xi and
 // zeta grid lines, as on terrain-following grids over x-z separable meshes),
 // so they are not loaded.
-template <bool X0 = false, class T, class TM>
-__host__ __device__ inline T stencil(const Win<T>& w, const Met<TM>& M, long c, long sz, const G2<T>& gm,
+template <bool X0 = false, class T, class MV>
+__host__ __device__ inline T stencil(const Win<T>& w, const MV& M, long c, long sz, const G2<T>& gm,
                             const G2<T>& g0, T inv_pot) {
     const T q = (T)0.25;
-    const T g11 = (T)M.g[0][c], g12 = (T)M.g[1][c], g13 = X0 ? (T)0 : (T)M.g[2][c];
-    const T h11 = (T)M.g[0][c - 1], h12 = (T)M.g[1][c - 1], h13 = X0 ? (T)0 : (T)M.g[2][c - 1];
-    const T g31 = X0 ? (T)0 : (T)M.g[6][c], g32 = (T)M.g[7][c], g33 = (T)M.g[8][c];
-    const T h31 = X0 ? (T)0 : (T)M.g[6][c - sz], h32 = (T)M.g[7][c - sz], h33 = (T)M.g[8][c - sz];
+    const T g11 = (T)mg(M, 0, c), g12 = (T)mg(M, 1, c), g13 = X0 ? (T)0 : (T)mg(M, 2, c);
+    const T h11 = (T)mg(M, 0, c - 1), h12 = (T)mg(M, 1, c - 1), h13 = X0 ? (T)0 : (T)mg(M, 2, c - 1);
+    const T g31 = X0 ? (T)0 : (T)mg(M, 6, c), g32 = (T)mg(M, 7, c), g33 = (T)mg(M, 8, c);
+    const T h31 = X0 ? (T)0 : (T)mg(M, 6, c - sz), h32 = (T)mg(M, 7, c - sz), h33 = (T)mg(M, 8, c - sz);
     T res_dx = (g11 * (w.xp0 - w.p0) +
                 q * (g12 * (w.xpp + w.pp - w.xpm - w.pm) + g13 * (w.dpp + w.zp0 - w.dpm - w.zm0))) *
                inv_pot;
@@ -155,9 +200,9 @@ __host__ __device__ inline T stencil(const Win<T>& w, const Met<TM>& M, long c, 
     return (res_dx - res_sn + res_sop - res_sot + res_av - res_ind) * inv_pot;
 }
 
-template <class T, class TM>
-__host__ __device__ inline G2<T> load_g2(const Met<TM>& M, long c) {
-    return G2<T>{(T)M.g[3][c], (T)M.g[4][c], (T)M.g[5][c]};
+template <class T, class MV>
+__host__ __device__ inline G2<T> load_g2(const MV& M, long c) {
+    return G2<T>{(T)mg(M, 3, c), (T)mg(M, 4, c), (T)mg(M, 5, c)};
 }
 
 __device__ inline bool column_of(const Geo& G, int& i, int& k) {
@@ -242,12 +287,13 @@ struct LineF {
     const float* w;
     const _Float16* v16;  // 16-bit copies of v, w: read by the smoother with 16-bit metrics
     const _Float16* w16;
+    const H2* vw16;       // the same pairs packed (v, w), read with packed metrics
 };
 
 // spike value of row c: the 16-bit copy for the 16-bit-metric smoother
-template <class TM>
+template <bool HALF>
 __device__ inline float spike(const float* a, const _Float16* a16, long c) {
-    if constexpr (sizeof(TM) == 2) return (float)a16[c];
+    if constexpr (HALF) return (float)a16[c];
     else return a[c];
 }
 
@@ -325,8 +371,8 @@ struct SegBX {
 // eta-wall ghosts of column (i,k) after its new values pn1 = p(i,1,k),
 // pnn = p(i,ny,k): fsrc/multigrid.f03:1166-1186 with an = 0, neighbouring
 // columns with their input (lagged) values; writes the ghosts' images too.
-template <class T, class TM>
-__device__ inline void yline_ghosts(const Geo& G, const Met<TM>& M, const T* __restrict__ pin,
+template <class T, class MV>
+__device__ inline void yline_ghosts(const Geo& G, const MV& M, const T* __restrict__ pin,
                                     T* __restrict__ pout, int i, int k, bool lo_side, bool hi_side, T pn1,
                                     T pnn, int zero_in) {
     const T qq = (T)0.25;
@@ -335,11 +381,11 @@ __device__ inline void yline_ghosts(const Geo& G, const Met<TM>& M, const T* __r
         T lo = pn1;
         const long c0 = G.idx(i, 0, k);
         if (!zero_in) {
-            const T inv = (T)1 / (T)M.g[4][c0];
+            const T inv = (T)1 / (T)mg(M, 4, c0);
             lo = pn1 + qq * inv *
-                           ((T)M.g[3][c0] * ((T)pin[c0 + 1 + sy] + (T)pin[c0 + 1] - (T)pin[c0 - 1 + sy] -
+                           ((T)mg(M, 3, c0) * ((T)pin[c0 + 1 + sy] + (T)pin[c0 + 1] - (T)pin[c0 - 1 + sy] -
                                              (T)pin[c0 - 1]) +
-                            (T)M.g[5][c0] * ((T)pin[c0 + sy + sz] + (T)pin[c0 + sz] - (T)pin[c0 + sy - sz] -
+                            (T)mg(M, 5, c0) * ((T)pin[c0 + sy + sz] + (T)pin[c0 + sz] - (T)pin[c0 + sy - sz] -
                                              (T)pin[c0 - sz]));
         }
         pout[c0] = lo;
@@ -349,11 +395,11 @@ __device__ inline void yline_ghosts(const Geo& G, const Met<TM>& M, const T* __r
         T hi = pnn;
         const long ct = G.idx(i, G.ny, k);
         if (!zero_in) {
-            const T invt = (T)1 / (T)M.g[4][ct];
+            const T invt = (T)1 / (T)mg(M, 4, ct);
             hi = pnn - qq * invt *
-                           ((T)M.g[3][ct] * ((T)pin[ct + 1 + sy] + (T)pin[ct + 1] - (T)pin[ct - 1 + sy] -
+                           ((T)mg(M, 3, ct) * ((T)pin[ct + 1 + sy] + (T)pin[ct + 1] - (T)pin[ct - 1 + sy] -
                                              (T)pin[ct - 1]) +
-                            (T)M.g[5][ct] * ((T)pin[ct + sy + sz] + (T)pin[ct + sz] - (T)pin[ct + sy - sz] -
+                            (T)mg(M, 5, ct) * ((T)pin[ct + sy + sz] + (T)pin[ct + sz] - (T)pin[ct + sy - sz] -
                                              (T)pin[ct - sz]));
         }
         pout[ct + sy] = hi;
@@ -487,11 +533,13 @@ struct YlineOtf {
 #ifndef PLES_YLINE_WPE_F16
 #define PLES_YLINE_WPE_F16 4
 #endif
-template <int P, int MR, class T, class TM, class TR, bool X0 = false>
+// MV: the metric view, Met<T>, Met<_Float16> or the packed MetPk (X0 levels).
+template <int P, int MR, class T, class MV, class TR, bool X0 = false>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
-    sizeof(T) == 4 ? (sizeof(TM) == 2 ? PLES_YLINE_WPE_F16 : PLES_YLINE_WPE_F32) : 1)))
-k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin, T* __restrict__ pout, const TR* __restrict__ b,
+    sizeof(T) == 4 ? (MVHalf<MV>::value ? PLES_YLINE_WPE_F16 : PLES_YLINE_WPE_F32) : 1)))
+k_yline(Geo G, MV M, LineF F, const T* __restrict__ pin, T* __restrict__ pout, const TR* __restrict__ b,
         T omega, T inv_pot, int zero_in) {
+    constexpr bool PK = std::is_same<MV, MetPk>::value;
     constexpr int BX = SegBX<P>::value;
     constexpr bool OTF = YlineOtf<P, MR, T>::value;
     __shared__ T red[8][P][BX];
@@ -514,8 +562,8 @@ k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin, T* __restrict__ po
         T dprev = 0, pfirst = 0, plast = 0, cprev = 0;
         // factor of row jj: 1/m and c'; g22m = g22(j-1), g22p = g22(j)
         auto factor = [&](int jj, long c, T g22m, T g22p, T& inv) {
-            const T den = s2 * (mxp * (T)M.g[0][c] + mxm * (T)M.g[0][c - 1] + mzp * (T)M.g[8][c] +
-                                mzm * (T)M.g[8][c - sz] + ((jj == mr - 1) ? myl : (T)1) * g22p +
+            const T den = s2 * (mxp * (T)mg(M, 0, c) + mxm * (T)mg(M, 0, c - 1) + mzp * (T)mg(M, 8, c) +
+                                mzm * (T)mg(M, 8, c - sz) + ((jj == mr - 1) ? myl : (T)1) * g22p +
                                 ((jj == 0) ? myf : (T)1) * g22m);
             const T a = (jj > 0) ? -s2 * g22m : (T)0;
             const T cc = (jj < mr - 1) ? -s2 * g22p : (T)0;
@@ -526,11 +574,11 @@ k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin, T* __restrict__ po
             return a;
         };
         if (zero_in) {
-            T g22m = (T)M.g[4][cb];
+            T g22m = (T)mg(M, 4, cb);
 #pragma unroll
             for (int jj = 0; jj < MR; ++jj) {
                 const long c = cb + sy * (jj + 1);
-                const T g22p = (T)M.g[4][c];
+                const T g22p = (T)mg(M, 4, c);
                 T a, inv;
                 if constexpr (OTF) {
                     a = factor(jj, c, g22m, g22p, inv);
@@ -585,10 +633,18 @@ k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin, T* __restrict__ po
         yL = elast - plast;
         if (P > 1) {
             const long cF = cb + sy, cL = cb + sy * mr;
-            vF = (T)spike<TM>(F.v, F.v16, cF);
-            vL = (T)spike<TM>(F.v, F.v16, cL);
-            wF = (T)spike<TM>(F.w, F.w16, cF);
-            wL = (T)spike<TM>(F.w, F.w16, cL);
+            if constexpr (PK) {
+                const H2 sF = F.vw16[cF], sL = F.vw16[cL];
+                vF = (T)sF.x;
+                wF = (T)sF.y;
+                vL = (T)sL.x;
+                wL = (T)sL.y;
+            } else {
+                vF = (T)spike<MVHalf<MV>::value>(F.v, F.v16, cF);
+                vL = (T)spike<MVHalf<MV>::value>(F.v, F.v16, cL);
+                wF = (T)spike<MVHalf<MV>::value>(F.w, F.w16, cF);
+                wL = (T)spike<MVHalf<MV>::value>(F.w, F.w16, cL);
+            }
         }
     }
     T uLm = 0, uFp = 0;
@@ -609,7 +665,16 @@ k_yline(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin, T* __restrict__ po
     for (int jj = 0; jj < MR; ++jj) {
         const long c = cb + sy * (jj + 1);
         T pn = e[jj];
-        if (P > 1) pn -= uLm * (T)spike<TM>(F.v, F.v16, c) + uFp * (T)spike<TM>(F.w, F.w16, c);
+        if constexpr (PK) {
+            if (P > 1) {
+                const H2 sv = F.vw16[c];
+                pn -= uLm * (T)sv.x + uFp * (T)sv.y;
+            }
+        } else {
+            if (P > 1)
+                pn -= uLm * (T)spike<MVHalf<MV>::value>(F.v, F.v16, c) +
+                      uFp * (T)spike<MVHalf<MV>::value>(F.w, F.w16, c);
+        }
         e[jj] = pn;
         pout[c] = pn;
         write_images(G, pout, i, q.j0 + jj, k, pn);
@@ -674,10 +739,10 @@ __global__ void __launch_bounds__(512) k_yline_staged(Geo G, Met<TM> M, LineF F,
         yF = qn - pfirst;
         if (P > 1) {
             const long cF = cb + sy, cL = cb + sy * (q.j1 - q.j0 + 1);
-            vF = (T)spike<TM>(F.v, F.v16, cF);
-            vL = (T)spike<TM>(F.v, F.v16, cL);
-            wF = (T)spike<TM>(F.w, F.w16, cF);
-            wL = (T)spike<TM>(F.w, F.w16, cL);
+            vF = (T)spike<sizeof(TM) == 2>(F.v, F.v16, cF);
+            vL = (T)spike<sizeof(TM) == 2>(F.v, F.v16, cL);
+            wF = (T)spike<sizeof(TM) == 2>(F.w, F.w16, cF);
+            wL = (T)spike<sizeof(TM) == 2>(F.w, F.w16, cL);
         }
     }
     T uLm = 0, uFp = 0;
@@ -697,7 +762,8 @@ __global__ void __launch_bounds__(512) k_yline_staged(Geo G, Met<TM> M, LineF F,
     for (int j = q.j0; j <= q.j1; ++j) {
         const long c = cb + sy * (j - q.j0 + 1);
         T pn = pout[c];
-        if (P > 1) pn -= uLm * (T)spike<TM>(F.v, F.v16, c) + uFp * (T)spike<TM>(F.w, F.w16, c);
+        if (P > 1)
+            pn -= uLm * (T)spike<sizeof(TM) == 2>(F.v, F.v16, c) + uFp * (T)spike<sizeof(TM) == 2>(F.w, F.w16, c);
         pout[c] = pn;
         write_images(G, pout, i, j, k, pn);
         if (j == q.j0) pn1 = pn;
@@ -1872,6 +1938,35 @@ __global__ void k_to_half(long n, const float* __restrict__ a, _Float16* __restr
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long stride = (long)gridDim.x * blockDim.x;
     for (; t < n; t += stride) b[t] = (_Float16)(a[t] * scale);
+}
+
+// packed 16-bit smoother metrics (MetPk) of an FP32 level, scaled like k_to_half
+__global__ void k_pack_met16(long n, const float* __restrict__ g11, const float* __restrict__ g12,
+                             const float* __restrict__ g32, const float* __restrict__ g33,
+                             const float* __restrict__ g21, const float* __restrict__ g23, float scale,
+                             H4* __restrict__ a, H2* __restrict__ b) {
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        H4 v;
+        v.x = (_Float16)(g11[t] * scale);
+        v.y = (_Float16)(g12[t] * scale);
+        v.z = (_Float16)(g32[t] * scale);
+        v.w = (_Float16)(g33[t] * scale);
+        a[t] = v;
+        H2 u;
+        u.x = (_Float16)(g21[t] * scale);
+        u.y = (_Float16)(g23[t] * scale);
+        b[t] = u;
+    }
+}
+
+// packed 16-bit spikes (v, w)
+__global__ void k_pack_vw(long n, const float* __restrict__ v, const float* __restrict__ w, H2* __restrict__ vw) {
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        H2 u;
+        u.x = (_Float16)v[t];
+        u.y = (_Float16)w[t];
+        vw[t] = u;
+    }
 }
 
 // per-block maxima of |a| over n values (grid-stride), one partial per block

@@ -36,6 +36,9 @@ struct Level {
     double* kd = nullptr;   // coarsest level: dense exact solve map (ghosted x interior)
     _Float16* v16 = nullptr;  // 16-bit copies of lv_v, lv_w (levels with g16)
     _Float16* w16 = nullptr;
+    H4* pk_a = nullptr;       // packed 16-bit smoother metrics (levels with g16 and x0)
+    H2* pk_b = nullptr;
+    H2* vw16 = nullptr;       // packed 16-bit spikes (levels with pk_a)
     float* invm = nullptr;
     float* cpr = nullptr;
     float* lv_v = nullptr;  // segmented line-solve responses
@@ -239,7 +242,7 @@ int choose_segments(int ny) {
     return P;
 }
 
-LineF line_f(const Level& L) { return LineF{L.invm, L.cpr, L.lv_v, L.lv_w, L.v16, L.w16}; }
+LineF line_f(const Level& L) { return LineF{L.invm, L.cpr, L.lv_v, L.lv_w, L.v16, L.w16, L.vw16}; }
 
 // per-launch event timing
 struct KT {
@@ -412,12 +415,24 @@ void yline_go(H* h, Level& L, const Met<TM>& m, double inv_pot, const TR* rhs, i
                                (T*)L.p[1 - L.cur], rhs, (T)h->omega_now, (T)inv_pot, zin);
         };
         // the variant without the g13, g31 loads: measured faster for the 16-bit-metric
-        // and FP64 smoothers, slower for the FP32-metric one (kept with the loads)
-        if constexpr (sizeof(TM) == 2 || sizeof(T) == 8) {
-            if (L.x0) go(k_yline<P, MR, T, TM, TR, true>);
-            else go(k_yline<P, MR, T, TM, TR, false>);
+        // and FP64 smoothers, slower for the FP32-metric one (kept with the loads); the
+        // 16-bit one reads the packed metrics and spikes
+        if constexpr (sizeof(TM) == 2) {
+            if (L.x0 && L.pk_a && (P == 1 || L.vw16)) {
+                const MetPk mp{L.pk_a, L.pk_b, L.g16[4]};
+                hipLaunchKernelGGL((k_yline<P, MR, T, MetPk, TR, true>), cl.grid, cl.block, 0, h->s, L.G, mp,
+                                   line_f(L), (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->omega_now,
+                                   (T)inv_pot, zin);
+            } else if (L.x0) {
+                go(k_yline<P, MR, T, Met<TM>, TR, true>);
+            } else {
+                go(k_yline<P, MR, T, Met<TM>, TR, false>);
+            }
+        } else if constexpr (sizeof(T) == 8) {
+            if (L.x0) go(k_yline<P, MR, T, Met<TM>, TR, true>);
+            else go(k_yline<P, MR, T, Met<TM>, TR, false>);
         } else {
-            go(k_yline<P, MR, T, TM, TR, false>);
+            go(k_yline<P, MR, T, Met<TM>, TR, false>);
         }
     } else
         hipLaunchKernelGGL((k_yline_staged<P, T, TM, TR>), cl.grid, cl.block, 0, h->s, L.G, m, line_f(L),
@@ -1198,6 +1213,11 @@ int level_factors(H* h, Level& L) {
         const int nb = (int)std::min<long>((n + 255) / 256, 8192);
         hipLaunchKernelGGL(k_to_half, dim3(nb), dim3(256), 0, h->s, n, (const float*)L.lv_v, L.v16, 1.0f);
         hipLaunchKernelGGL(k_to_half, dim3(nb), dim3(256), 0, h->s, n, (const float*)L.lv_w, L.w16, 1.0f);
+        if (L.pk_a) {
+            if (!L.vw16) CKS(dalloc(h, (void**)&L.vw16, (size_t)n * sizeof(H2)));
+            hipLaunchKernelGGL(k_pack_vw, dim3(nb), dim3(256), 0, h->s, n, (const float*)L.lv_v,
+                               (const float*)L.lv_w, L.vw16);
+        }
         CK(hipGetLastError());
     }
     return PLES_MG_OK;
@@ -1588,6 +1608,13 @@ int make_half_metrics(H* h, Level& L) {
         CKS(dalloc(h, (void**)&L.g16[q], (size_t)cells * 2));
         hipLaunchKernelGGL(k_to_half, dim3(nb), dim3(256), 0, h->s, cells, (const float*)L.g32[q], L.g16[q], scale);
     }
+    if (L.x0) {  // g13 = g31 = 0: the register smoother reads the packed form
+        CKS(dalloc(h, (void**)&L.pk_a, (size_t)cells * sizeof(H4)));
+        CKS(dalloc(h, (void**)&L.pk_b, (size_t)cells * sizeof(H2)));
+        hipLaunchKernelGGL(k_pack_met16, dim3(nb), dim3(256), 0, h->s, cells, (const float*)L.g32[0],
+                           (const float*)L.g32[1], (const float*)L.g32[7], (const float*)L.g32[8],
+                           (const float*)L.g32[3], (const float*)L.g32[5], scale, L.pk_a, L.pk_b);
+    }
     CK(hipGetLastError());
     return PLES_MG_OK;
 }
@@ -1605,7 +1632,7 @@ size_t estimate(const ples_mg_grid* g, const ples_mg_opts* o) {
         const bool f32 = mixed && n < g->nlevel - 1;
         const size_t sp = f32 ? 4 : 8;
         tot += cells * 9 * sp;                  // V-cycle metrics
-        if (f32 && o->smoother_fp16) tot += cells * (9 * 2 + 4);  // smoother's 16-bit metrics, spikes
+        if (f32 && o->smoother_fp16) tot += cells * (9 * 2 + 4 + 12 + 4);  // smoother's 16-bit metrics, spikes, packed
         if (n == 0 && f32) tot += cells * 9 * 8;  // FP64 outer metrics
         tot += cells * 16;                      // invm, cpr, v, w
         tot += cells * sp * 2;                  // ping-pong
