@@ -349,7 +349,7 @@ def main():
         "pressure_rel_l2_vs_fp64_1e-12": {p: rel_l2_meanfree(out[p]["fi"], out["ref_fi"]) for p in ("mixed", "fp64")},
         "pressure_rel_l2_mixed_vs_fp64": rel_l2_meanfree(out["mixed"]["fi"], out["fp64"]["fi"]),
         "roofline": {
-            "kernel": "k_yline<float, 16-bit metrics> fine level (eta-line smoother)",
+            "kernel": "k_yline<float, packed 16-bit metrics> fine level (eta-line smoother)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
