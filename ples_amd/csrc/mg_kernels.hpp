@@ -907,12 +907,32 @@ __device__ inline void fin_reduce(const double* __restrict__ part, int nb, const
                                   double (&out)[NV]) {
     __shared__ double sh[NV][256];
     const int t = threadIdx.x;
+    // four strided accumulators per quantity, so every thread has 4 NV loads
+    // in flight instead of one dependent load at a time; fixed order
+    double a[NV][4];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        double a = 0.0;
-        for (int u = t; u < nb; u += 256) a = mx[q] ? fmax(a, part[(long)q * nb + u]) : a + part[(long)q * nb + u];
-        sh[q][t] = a;
-    }
+    for (int q = 0; q < NV; ++q)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a[q][k] = 0.0;
+    int u = t;
+    for (; u + 768 < nb; u += 1024)
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double v = part[(long)q * nb + u + 256 * k];
+                a[q][k] = mx[q] ? fmax(a[q][k], v) : a[q][k] + v;
+            }
+    for (; u < nb; u += 256)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const double v = part[(long)q * nb + u];
+            a[q][0] = mx[q] ? fmax(a[q][0], v) : a[q][0] + v;
+        }
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+        sh[q][t] = mx[q] ? fmax(fmax(a[q][0], a[q][1]), fmax(a[q][2], a[q][3]))
+                         : (a[q][0] + a[q][1]) + (a[q][2] + a[q][3]);
     __syncthreads();
     for (int wdt = 128; wdt > 0; wdt >>= 1) {
         if (t < wdt)
