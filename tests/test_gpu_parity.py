@@ -3,7 +3,7 @@
 Tolerances: FP64 kernels agree with the oracle to 1e-12 relative (the only
 differences are fused multiply-adds and summation order on the device);
 converged pressures agree to 1e-9 relative L2 after mean removal (the
-north-star parity bar), mixed precision included.
+pressure parity bar of the solver), mixed precision included.
 """
 import numpy as np
 import pytest
