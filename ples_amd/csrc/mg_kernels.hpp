@@ -37,7 +37,8 @@ struct Met {
 using H4 = _Float16 __attribute__((ext_vector_type(4)));
 using H2 = _Float16 __attribute__((ext_vector_type(2)));
 struct MetPk {
-    const H4* a;
+    const H2* a;   // (g11, g12)
+    const H2* z;   // (g32, g33)
     const H2* b;
     const _Float16* g22;
 };
@@ -54,8 +55,8 @@ __device__ inline _Float16 mg(const MetPk& M, int q, long c) {
         case 3: return M.b[c].x;
         case 4: return M.g22[c];
         case 5: return M.b[c].y;
-        case 7: return M.a[c].z;
-        case 8: return M.a[c].w;
+        case 7: return M.z[c].x;
+        case 8: return M.z[c].y;
         default: return (_Float16)0;  // g13, g31
     }
 }
@@ -1964,14 +1965,16 @@ __global__ void k_to_half(long n, const float* __restrict__ a, _Float16* __restr
 __global__ void k_pack_met16(long n, const float* __restrict__ g11, const float* __restrict__ g12,
                              const float* __restrict__ g32, const float* __restrict__ g33,
                              const float* __restrict__ g21, const float* __restrict__ g23, float scale,
-                             H4* __restrict__ a, H2* __restrict__ b) {
+                             H2* __restrict__ a, H2* __restrict__ zz, H2* __restrict__ b) {
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-        H4 v;
+        H2 v;
         v.x = (_Float16)(g11[t] * scale);
         v.y = (_Float16)(g12[t] * scale);
-        v.z = (_Float16)(g32[t] * scale);
-        v.w = (_Float16)(g33[t] * scale);
         a[t] = v;
+        H2 vz;
+        vz.x = (_Float16)(g32[t] * scale);
+        vz.y = (_Float16)(g33[t] * scale);
+        zz[t] = vz;
         H2 u;
         u.x = (_Float16)(g21[t] * scale);
         u.y = (_Float16)(g23[t] * scale);

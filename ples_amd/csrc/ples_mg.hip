@@ -36,7 +36,8 @@ struct Level {
     double* kd = nullptr;   // coarsest level: dense exact solve map (ghosted x interior)
     _Float16* v16 = nullptr;  // 16-bit copies of lv_v, lv_w (levels with g16)
     _Float16* w16 = nullptr;
-    H4* pk_a = nullptr;       // packed 16-bit smoother metrics (levels with g16 and x0)
+    H2* pk_a = nullptr;       // packed 16-bit smoother metrics (levels with g16 and x0)
+    H2* pk_z = nullptr;
     H2* pk_b = nullptr;
     H2* vw16 = nullptr;       // packed 16-bit spikes (levels with pk_a)
     float* invm = nullptr;
@@ -419,7 +420,7 @@ void yline_go(H* h, Level& L, const Met<TM>& m, double inv_pot, const TR* rhs, i
         // 16-bit one reads the packed metrics and spikes
         if constexpr (sizeof(TM) == 2) {
             if (L.x0 && L.pk_a && (P == 1 || L.vw16)) {
-                const MetPk mp{L.pk_a, L.pk_b, L.g16[4]};
+                const MetPk mp{L.pk_a, L.pk_z, L.pk_b, L.g16[4]};
                 hipLaunchKernelGGL((k_yline<P, MR, T, MetPk, TR, true>), cl.grid, cl.block, 0, h->s, L.G, mp,
                                    line_f(L), (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->omega_now,
                                    (T)inv_pot, zin);
@@ -1609,11 +1610,12 @@ int make_half_metrics(H* h, Level& L) {
         hipLaunchKernelGGL(k_to_half, dim3(nb), dim3(256), 0, h->s, cells, (const float*)L.g32[q], L.g16[q], scale);
     }
     if (L.x0) {  // g13 = g31 = 0: the register smoother reads the packed form
-        CKS(dalloc(h, (void**)&L.pk_a, (size_t)cells * sizeof(H4)));
+        CKS(dalloc(h, (void**)&L.pk_a, (size_t)cells * sizeof(H2)));
+        CKS(dalloc(h, (void**)&L.pk_z, (size_t)cells * sizeof(H2)));
         CKS(dalloc(h, (void**)&L.pk_b, (size_t)cells * sizeof(H2)));
         hipLaunchKernelGGL(k_pack_met16, dim3(nb), dim3(256), 0, h->s, cells, (const float*)L.g32[0],
                            (const float*)L.g32[1], (const float*)L.g32[7], (const float*)L.g32[8],
-                           (const float*)L.g32[3], (const float*)L.g32[5], scale, L.pk_a, L.pk_b);
+                           (const float*)L.g32[3], (const float*)L.g32[5], scale, L.pk_a, L.pk_z, L.pk_b);
     }
     CK(hipGetLastError());
     return PLES_MG_OK;
