@@ -31,14 +31,13 @@ struct Met {
 };
 
 // 16-bit smoother metrics of a level whose g13 and g31 vanish, packed per
-// cell so a row of the smoother issues one 8-byte, one 4-byte and one 2-byte
-// metric load instead of seven 2-byte ones (more bytes in flight per
-// register): a[c] = (g11, g12, g32, g33), b[c] = (g21, g23), g22 apart.
-using H4 = _Float16 __attribute__((ext_vector_type(4)));
+// cell in 4-byte pairs so a row of the smoother issues fewer, wider metric
+// loads than seven 2-byte ones (more bytes in flight per register):
+// a[c] = (g11, g12), z[c] = (g32, g33), b[c] = (g21, g23), g22 apart.
 using H2 = _Float16 __attribute__((ext_vector_type(2)));
 struct MetPk {
-    const H2* a;   // (g11, g12)
-    const H2* z;   // (g32, g33)
+    const H2* a;
+    const H2* z;
     const H2* b;
     const _Float16* g22;
 };
