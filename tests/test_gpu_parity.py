@@ -241,6 +241,22 @@ def test_walls_everywhere_strongly_warped(precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_matches_oracle_partial_blocks(precision):
+    """A fine level whose xi count (48) fills 48 of the 64 lanes of its
+    smoother and residual blocks, so the last lanes of every block row are
+    idle: the terrain grid's packed 16-bit smoother metrics and the column
+    kernels leave them out and the solve still matches the oracle."""
+    g = cases.grid(48, 16, 24, gamma=1.5, amp=0.2)
+    rhs_g = O.case_rhs(g, 1006)
+    ref = _oracle_solution(g, 3, rhs_g)
+    with _solver(g, 3, precision) as s:
+        fi = np.zeros(s.fi_shape)
+        res = s.multi(fi, cases.interior(rhs_g), eps=1e-12, ficycle=200)
+    assert res.ok, res
+    assert cases.rel_l2_meanfree(fi, ref) < 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_solve_matches_oracle_sheared(precision):
     """Grid lines skewed in the xi-zeta plane as well (x shifted by a function
     of z, periodic over the zeta period): every cross metric is nonzero, so the
