@@ -86,11 +86,14 @@ typedef struct ples_mg_opts {
                         always launched one by one while kernel timing is on */
     int inner_cycles; /* V-cycles per refinement step, the first from zero, each
                         further one continuing from the last (0 = auto = 1) */
-    int autotune;    /* 1 = at create, time the candidate launch shapes of every
-                        level (smoother segments per column, residual segments x
-                        lanes) and keep the fastest; results are cached per level
-                        shape for the process, so equal shapes get equal kernels.
-                        0 = fixed rule (default) */
+    int autotune;    /* launch shapes of every level (smoother segments per
+                        column and 16-bit metrics, residual segments x lanes):
+                        0 (default) = the process-wide table's shape where it
+                        holds the level's (ples_mg_tuning_load), else the fixed
+                        rule; 1 = as 0, and a shape the table lacks is timed at
+                        create (the candidates on the level's own arrays, the
+                        fastest kept and added to the table, so equal shapes get
+                        equal kernels); -1 = the fixed rule only */
     int fp32_krylov; /* PLES_MG_MIXED only: 1 = FP64 iterative refinement around
                         an FP32 GCR(2) preconditioned by the FP32 V-cycle (default);
                         0 = the FP32 V-cycle directly inside the FP64 GCR(2).
@@ -354,8 +357,8 @@ int ples_mg_restart_read(const char* path, int jx, int jy, int jz, int nscal, do
  * shape per line: nx ny nz ip kp f32 fine half P rP rbx use16 -- half: the
  * level has the 16-bit smoother metrics, use16: its smoother reads them;
  * lines of the older form nx ny nz ip kp f32 fine P rP rbx are read with
- * half = use16 = 0), so handles created with opts.autotune take those shapes
- * without timing; save the current table. */
+ * half = use16 = 0), so handles created afterwards (opts.autotune >= 0) take
+ * those shapes without timing; save the current table. */
 int ples_mg_tuning_load(const char* path);
 int ples_mg_tuning_save(const char* path);
 

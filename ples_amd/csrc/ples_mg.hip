@@ -1275,7 +1275,9 @@ int lv_resid(H* h, int n, const double* r0);
 int lv_resid_restrict(H* h, int n, const double* r0);
 bool coarse_one_block(const Level& L);
 
-int tune_level(H* h, int n) {
+// time_missing: time the candidates of a shape the table does not hold (else
+// the level keeps the fixed rule's shape)
+int tune_level(H* h, int n, bool time_missing = true) {
     Level& L = h->lv[n];
     if (n == h->L - 1 && (L.kd || coarse_one_block(L))) return PLES_MG_OK;  // one-launch coarse solver
     const bool half = L.g16[0] != nullptr;
@@ -1296,6 +1298,7 @@ int tune_level(H* h, int n) {
         L.use16 = half && it->second.use16 != 0;
         return level_factors(h, L);
     }
+    if (!time_missing) return PLES_MG_OK;
     const double* r0 = h->rvec;  // level-0 rhs (any content: timing only)
     // the fixed rule's shape is the incumbent; another shape must be at least
     // 3% faster to replace it, which keeps choices stable between runs
@@ -1892,8 +1895,8 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
     CKS(dalloc(h, (void**)&h->part, (size_t)h->part_n * 4 * sizeof(double)));
     h->halo_passes32 = h->o.halo_passes;
     if (auto_passes && (g->ip == 1 || g->kp == 1)) CKS(auto_halo_passes(h));
-    if (o.autotune)
-        for (int n = 0; n < h->L; ++n) CKS(tune_level(h, n));
+    if (o.autotune >= 0)  // loaded table entries (and timing with autotune > 0)
+        for (int n = 0; n < h->L; ++n) CKS(tune_level(h, n, o.autotune > 0));
     CK(hipStreamSynchronize(h->s));
     return PLES_MG_OK;
 }

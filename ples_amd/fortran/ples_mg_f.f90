@@ -67,6 +67,13 @@ module ples_mg_f
          type(c_ptr), value :: h
       end function
 
+      ! launch-shape table for the handles created afterwards; path is a
+      ! NUL-terminated string: trim(path)//c_null_char
+      integer(c_int) function ples_mg_tuning_load(path) bind(C, name="ples_mg_tuning_load")
+         import :: c_int, c_char
+         character(kind=c_char), dimension(*), intent(in) :: path
+      end function
+
       integer(c_int) function ples_mg_solve(h, fi, deepl, deepr, rhs, cs1, cs2, cs3, cs4, cs5, cs6, &
                                             dt, eps, ficycle, res) bind(C, name="ples_mg_solve")
          import :: c_int, c_ptr, c_double, ples_mg_result

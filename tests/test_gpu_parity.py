@@ -415,27 +415,30 @@ def test_warm_start_resident_and_determinism():
 
 def test_tuned_launch_shapes_same_solution():
     """Launch shapes from the tuning table change the smoother's segments and
-    the residual's blocks, not the solution: tuned and fixed-rule solves
-    agree to the parity bar, and the tuned handle reports the table's shapes."""
+    the residual's blocks, not the solution: fixed-rule (autotune=-1) and
+    table solves agree to the parity bar, and handles created with the
+    default options (autotune=0) or with timing (1) report the table's shapes
+    once it is loaded."""
     assert mg.tuning_load()
     g = cases.grid(64, 32, 64, gamma=1.5, amp=0.2)
+    # nx ny nz ip kp f32 fine half -> P rP rbx use16
+    table = {}
+    for line in open(mg.TUNING_TABLE):
+        if not line.startswith("#"):
+            v = list(map(int, line.split()))
+            table[tuple(v[:8])] = tuple(v[8:11])
     out = []
-    for tune in (0, 1):
+    for tune in (-1, 0, 1):
         with _solver(g, 4, "mixed", autotune=tune) as s:
             s.case_rhs(77)
             s.zero_fi()
             r = s.solve_resident(1e-11, 200)
             assert r.ok
             out.append(s.get_fi())
-            if tune:
-                # nx ny nz ip kp f32 fine half -> P rP rbx use16
-                table = {}
-                for line in open(mg.TUNING_TABLE):
-                    if not line.startswith("#"):
-                        v = list(map(int, line.split()))
-                        table[tuple(v[:8])] = tuple(v[8:11])
+            if tune >= 0:
                 assert s.tiles(1) == table[(64, 32, 64, 0, 0, 1, 1, 1)]
     assert cases.rel_l2_meanfree(out[1], out[0]) < 1e-9
+    assert np.array_equal(out[1], out[2])
 
 
 _SEG_REF = {}
