@@ -1837,9 +1837,13 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
             CK(hipGetLastError());
             if (o.smoother_fp16) {
                 CKS(make_half_metrics(h, L));
-                // fixed rule: the fine level only (measured: faster there, slower on
-                // the smaller, latency-bound levels); the autotuner times both
-                L.use16 = n == 0;
+                // fixed rule: the fine level, and the coarser levels (from 512
+                // cells) whose segments fit a register variant of the smoother,
+                // which reads the packed form where g13 = g31 = 0 (the autotuner
+                // chose these on the benchmark shapes; the staged variant and the
+                // smallest levels are faster with FP32 metrics); the autotuner
+                // times both
+                L.use16 = n == 0 || (L.ncell() >= 512 && yline_mr(L.G.ny / choose_segments(L.G.ny)) != 0);
             }
             if (n > 0) {
                 CK(hipStreamSynchronize(h->s));
