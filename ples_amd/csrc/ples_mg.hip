@@ -226,6 +226,27 @@ ColLaunch seg_launch(const Geo& G, int P) {
     return c;
 }
 
+// eta segments per column, fixed rule: among the counts P with a register
+// smoother variant (ny / P in {4, 8, 16, 32}, P in 1..16 dividing ny), the
+// fewest that still give the level ncol * P >= 2^17 threads (long register
+// rows where the grid fills the device), else 8 rows per segment where
+// possible (the small, latency-bound levels); the autotuner's choice on the
+// benchmark shapes.  Row counts without a register variant: choose_segments.
+int choose_segments(int ny);
+int fixed_segments(int ny, long ncol) {
+    std::vector<int> cand;
+    for (int P : {1, 2, 3, 4, 6, 8, 12, 16}) {
+        const int m = (ny % P == 0) ? ny / P : 0;
+        if (m == 4 || m == 8 || m == 16 || m == 32) cand.push_back(P);
+    }
+    if (cand.empty()) return choose_segments(ny);
+    for (int P : cand)
+        if (ncol * P >= (1L << 17)) return P;
+    for (int P : cand)
+        if (ny / P == 8) return P;
+    return cand.back();
+}
+
 // eta segments per column: the largest power of two up to 16 that divides ny
 // and leaves at least 16 rows per segment
 int choose_segments(int ny) {
@@ -1843,7 +1864,8 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
                 // chose these on the benchmark shapes; the staged variant and the
                 // smallest levels are faster with FP32 metrics); the autotuner
                 // times both
-                L.use16 = n == 0 || (L.ncell() >= 512 && yline_mr(L.G.ny / choose_segments(L.G.ny)) != 0);
+                L.use16 = n == 0 || (L.ncell() >= 512 &&
+                                     yline_mr(L.G.ny / fixed_segments(L.G.ny, (long)L.G.nx * L.G.nz)) != 0);
             }
             if (n > 0) {
                 CK(hipStreamSynchronize(h->s));
@@ -1853,7 +1875,7 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
                 }
             }
         }
-        L.P = choose_segments(L.G.ny);
+        L.P = fixed_segments(L.G.ny, (long)L.G.nx * L.G.nz);
         CKS(dalloc(h, (void**)&L.invm, cells * 4));
         CKS(dalloc(h, (void**)&L.cpr, cells * 4));
         if (L.P > 1) {
