@@ -1876,6 +1876,13 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
             }
         }
         L.P = fixed_segments(L.G.ny, (long)L.G.nx * L.G.nz);
+        {  // V-cycle residual, fixed rule: twice the smoother's segments (up to 16,
+           // an even row count each), 64 lanes up to 8 segments, else 32 (the
+           // autotuner's usual choice on the benchmark shapes)
+            const int r2 = std::min(2 * L.P, 16);
+            L.rP = (L.G.ny % r2 == 0 && L.G.ny / r2 >= 2 && (L.G.ny / r2) % 2 == 0) ? r2 : L.P;
+            L.rbx = L.rP <= 8 ? 64 : 32;
+        }
         CKS(dalloc(h, (void**)&L.invm, cells * 4));
         CKS(dalloc(h, (void**)&L.cpr, cells * 4));
         if (L.P > 1) {
