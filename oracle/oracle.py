@@ -55,6 +55,16 @@ def lib():
                                 C.POINTER(_dp), C.c_double, C.c_double, C.c_double, C.c_int,
                                 C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.orc_multi.restype = C.c_int
+        L.orc_multi_ex.argtypes = [C.POINTER(Dims), C.c_int, _dp, _dp, _dp, C.POINTER(_dp), _dp, _dp,
+                                   C.POINTER(_dp), C.c_double, C.c_double, C.c_double, C.c_int,
+                                   C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.orc_multi_ex.restype = C.c_int
+        L.orc_tridag.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_sett.argtypes = [C.POINTER(Dims), C.POINTER(_dp), C.c_double, _dp, _dp, _dp]
+        L.orc_slor_sweep.argtypes = [C.POINTER(Dims), _dp, _dp, C.POINTER(_dp), C.POINTER(_dp),
+                                     C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
+        L.orc_edge_fill.argtypes = [C.POINTER(Dims), _dp]
+        _bind_hier(L)
         L.orc_contra.argtypes = [C.POINTER(Dims), C.POINTER(_dp), _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_diver.argtypes = [C.POINTER(Dims), _dp, _dp, _dp, C.c_double, _dp]
         L.orc_project.argtypes = [C.POINTER(Dims), _dp, C.POINTER(_dp), C.c_double]
@@ -71,6 +81,15 @@ def lib():
             getattr(L, f).restype = C.c_double
         _lib = L
     return _lib
+
+
+def _bind_hier(L):
+    L.orc_hier_create.argtypes = [C.POINTER(Dims), C.c_int, _dp, _dp, _dp, C.POINTER(_dp)]
+    L.orc_hier_create.restype = C.c_void_p
+    L.orc_hier_destroy.argtypes = [C.c_void_p]
+    L.orc_hier_multi.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(_dp), C.c_double, C.c_double, C.c_double,
+                                 C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
+    L.orc_hier_multi.restype = C.c_int
 
 
 def _p(a: np.ndarray):
@@ -194,13 +213,88 @@ def prolong(g: Grid, level_c, ec, pf):
 
 
 def multi(g: Grid, nlevel, fi, rhs, cs=None, dt=1.0, omega=1.6, eps=1e-8, ficycle=1000,
-          kss1=5, kssc=10, nbc=5, g1=None):
+          kss1=5, kssc=10, nbc=5, g1=None, islor=0):
     d = g.dims(0)
     hist = np.zeros(max(ficycle, 1))
     rm = C.c_double(0.0)
-    n = lib().orc_multi(C.byref(d), nlevel, _p(g.x), _p(g.y), _p(g.z), _parr(g1), _p(fi), _p(rhs),
-                        _parr(cs), dt, omega, eps, ficycle, kss1, kssc, nbc, C.byref(rm), _p(hist))
+    n = lib().orc_multi_ex(C.byref(d), nlevel, _p(g.x), _p(g.y), _p(g.z), _parr(g1), _p(fi), _p(rhs),
+                           _parr(cs), dt, omega, eps, ficycle, kss1, kssc, nbc, islor, C.byref(rm), _p(hist))
     return n, rm.value, hist[:n]
+
+
+def tridag(aa, bb, cc, ff):
+    """Solve with the reference's Thomas algorithm; returns the solution
+    (the inputs are copied, the reference overwrites its own)."""
+    a, b, c, f = (np.array(v, dtype=np.float64) for v in (aa, bb, cc, ff))
+    lib().orc_tridag(len(f), _p(a), _p(b), _p(c), _p(f))
+    return f
+
+
+def sett(g: Grid, level, gm):
+    """Line coefficients aa, bb, cc (ghosted layout of the level)."""
+    d = g.dims(level)
+    aa, bb, cc = g.zeros(level), g.zeros(level), g.zeros(level)
+    lib().orc_sett(C.byref(d), _parr(gm), float(1 << level), _p(aa), _p(bb), _p(cc))
+    return aa, bb, cc
+
+
+def slor_sweep(g: Grid, level, p, rh, gm, cs=None, an=0.0, dt=1.0, omega=1.6, nbc=5):
+    d = g.dims(level)
+    lib().orc_slor_sweep(C.byref(d), _p(p), _p(rh), _parr(gm), _parr(cs), an, dt, float(1 << level), omega, nbc)
+
+
+def edge_fill(g: Grid, level, p):
+    lib().orc_edge_fill(C.byref(g.dims(level)), _p(p))
+
+
+class Hier:
+    """orc_hier: the level hierarchy built once, then timed solves.  `path`
+    selects the library (the OpenMP baseline build for timing)."""
+
+    def __init__(self, g: Grid, nlevel, path=None):
+        self.L = lib() if path is None else _load_baseline(path)
+        self.g = g
+        d = g.dims(0)
+        self.h = self.L.orc_hier_create(C.byref(d), nlevel, _p(g.x), _p(g.y), _p(g.z), None)
+
+    def multi(self, fi, rhs, cs=None, dt=1.0, omega=1.6, eps=1e-8, ficycle=1000, kss1=5, kssc=10,
+              nbc=5, islor=0, nslab=1):
+        hist = np.zeros(max(ficycle, 1))
+        rm = C.c_double(0.0)
+        n = self.L.orc_hier_multi(self.h, _p(fi), _p(rhs), _parr(cs), dt, omega, eps, ficycle, kss1, kssc,
+                                  nbc, islor, nslab, C.byref(rm), _p(hist))
+        if n < 0:
+            raise ValueError(f"nslab={nslab} does not divide the planes of every level")
+        return n, rm.value, hist[:n]
+
+    def close(self):
+        if self.h:
+            self.L.orc_hier_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_baselines = {}
+
+
+def build_baseline(out_dir=None) -> str:
+    """Compile the OpenMP baseline build (-O3 -march=native) on this host."""
+    out = out_dir or os.path.join(_HERE, "_build")
+    subprocess.run(["make", "-s", "-C", _HERE, "baseline", f"OUT={out}"], check=True)
+    return os.path.join(out, "libples_oracle_omp.so")
+
+
+def _load_baseline(path):
+    if path not in _baselines:
+        L = C.CDLL(path)
+        _bind_hier(L)
+        _baselines[path] = L
+    return _baselines[path]
 
 
 def gradie(g: Grid, fi, csv, giac, gm, dt=1.0):

@@ -386,6 +386,107 @@ void orc_metrics(const orc_dims* f, const double* x, const double* y, const doub
 #define P(i, j, k) p[GIDX(d, i, j, k)]
 #define G(m, i, j, k) g[m][GIDX(d, i, j, k)]
 
+/* Edge and corner ghosts of mul_boun_sndrcv (fsrc/multigrid.f03:1358-1742,
+ * jp = 1 only): every ghost that is neither a face ghost nor interior is a
+ * copy of one other value -- its periodic image along the periodic
+ * directions, or its neighbour towards the interior along a wall direction.
+ * The MPI exchanges of rank 0 <-> nproc-1 there become copies between the
+ * planes k = 0, 1, nz, nz+1 of the one global array; the edge strips of
+ * fsrc/multigrid.f03:1750-1931 are slab-internal halos and change nothing. */
+void orc_edge_fill(const orc_dims* d, double* p) {
+    const int nx = d->nx, ny = d->ny, nz = d->nz;
+    /* edges and corners: fsrc/multigrid.f03:1358-1742 (jp = 1 only) */
+    if (d->jp == 1) {
+        if (d->kp == 0) {
+            /* 1) corner lines along x */
+            for (int i = 1; i <= nx; ++i) {
+                P(i, ny + 1, 0) = P(i, ny + 1, nz);
+                P(i, 0, 0) = P(i, 0, nz);
+                P(i, ny + 1, nz + 1) = P(i, ny + 1, 1);
+                P(i, 0, nz + 1) = P(i, 0, 1);
+            }
+            /* 2) vertical frame on planes k = 0, nz+1 */
+            if (d->ip == 0) {
+                for (int j = 0; j <= ny + 1; ++j) {
+                    P(0, j, 0) = P(nx, j, nz);
+                    P(nx + 1, j, 0) = P(1, j, nz);
+                    P(0, j, nz + 1) = P(nx, j, 1);
+                    P(nx + 1, j, nz + 1) = P(1, j, 1);
+                }
+            } else {
+                for (int j = 0; j <= ny + 1; ++j) {
+                    P(nx + 1, j, 0) = P(nx + 1, j, nz);
+                    P(0, j, 0) = P(0, j, nz);
+                    P(nx + 1, j, nz + 1) = P(nx + 1, j, 1);
+                    P(0, j, nz + 1) = P(0, j, 1);
+                }
+            }
+            /* 3) horizontal frame */
+            if (d->ip == 0) {
+                for (int k = 1; k <= nz; ++k) {
+                    P(0, ny + 1, k) = P(nx, ny + 1, k);
+                    P(0, 0, k) = P(nx, 0, k);
+                    P(nx + 1, ny + 1, k) = P(1, ny + 1, k);
+                    P(nx + 1, 0, k) = P(1, 0, k);
+                }
+            } else {
+                for (int k = 1; k <= nz; ++k) {
+                    P(0, ny + 1, k) = P(0, ny, k);
+                    P(0, 0, k) = P(0, 1, k);
+                    P(nx + 1, ny + 1, k) = P(nx + 1, ny, k);
+                    P(nx + 1, 0, k) = P(nx + 1, 1, k);
+                }
+            }
+        } else {
+            /* 1) corner lines along x */
+            for (int i = 1; i <= nx; ++i) {
+                P(i, 0, 0) = P(i, 1, 0);
+                P(i, ny + 1, 0) = P(i, ny, 0);
+            }
+            for (int i = 1; i <= nx; ++i) {
+                P(i, 0, nz + 1) = P(i, 1, nz + 1);
+                P(i, ny + 1, nz + 1) = P(i, ny, nz + 1);
+            }
+            /* 2) vertical frame */
+            if (d->ip == 0) {
+                for (int j = 0; j <= ny + 1; ++j) {
+                    P(0, j, 0) = P(nx, j, 0);
+                    P(nx + 1, j, 0) = P(1, j, 0);
+                }
+                for (int j = 0; j <= ny + 1; ++j) {
+                    P(0, j, nz + 1) = P(nx, j, nz + 1);
+                    P(nx + 1, j, nz + 1) = P(1, j, nz + 1);
+                }
+            } else {
+                for (int j = 0; j <= ny + 1; ++j) {
+                    P(0, j, 0) = P(1, j, 0);
+                    P(nx + 1, j, 0) = P(nx, j, 0);
+                }
+                for (int j = 0; j <= ny + 1; ++j) {
+                    P(0, j, nz + 1) = P(1, j, nz + 1);
+                    P(nx + 1, j, nz + 1) = P(nx, j, nz + 1);
+                }
+            }
+            /* 3) horizontal frame */
+            if (d->ip == 0) {
+                for (int k = 1; k <= nz; ++k) {
+                    P(0, ny + 1, k) = P(nx, ny + 1, k);
+                    P(0, 0, k) = P(nx, 0, k);
+                    P(nx + 1, ny + 1, k) = P(1, ny + 1, k);
+                    P(nx + 1, 0, k) = P(1, 0, k);
+                }
+            } else {
+                for (int k = 1; k <= nz; ++k) {
+                    P(0, ny + 1, k) = P(1, ny + 1, k);
+                    P(0, 0, k) = P(1, 0, k);
+                    P(nx + 1, ny + 1, k) = P(nx, ny + 1, k);
+                    P(nx + 1, 0, k) = P(nx, 0, k);
+                }
+            }
+        }
+    }
+}
+
 void orc_ghost_fill(const orc_dims* d, double* p, const double* const g[9],
                     const double* const cs[6], double an, double dt, int npass) {
     const int nx = d->nx, ny = d->ny, nz = d->nz;
@@ -477,96 +578,7 @@ void orc_ghost_fill(const orc_dims* d, double* p, const double* const g[9],
                     P(i, j, nz + 1) = P(i, j, 1);
                 }
         }
-        /* edges and corners: fsrc/multigrid.f03:1358-1742 (jp = 1 only) */
-        if (d->jp == 1) {
-            if (d->kp == 0) {
-                /* 1) corner lines along x */
-                for (int i = 1; i <= nx; ++i) {
-                    P(i, ny + 1, 0) = P(i, ny + 1, nz);
-                    P(i, 0, 0) = P(i, 0, nz);
-                    P(i, ny + 1, nz + 1) = P(i, ny + 1, 1);
-                    P(i, 0, nz + 1) = P(i, 0, 1);
-                }
-                /* 2) vertical frame on planes k = 0, nz+1 */
-                if (d->ip == 0) {
-                    for (int j = 0; j <= ny + 1; ++j) {
-                        P(0, j, 0) = P(nx, j, nz);
-                        P(nx + 1, j, 0) = P(1, j, nz);
-                        P(0, j, nz + 1) = P(nx, j, 1);
-                        P(nx + 1, j, nz + 1) = P(1, j, 1);
-                    }
-                } else {
-                    for (int j = 0; j <= ny + 1; ++j) {
-                        P(nx + 1, j, 0) = P(nx + 1, j, nz);
-                        P(0, j, 0) = P(0, j, nz);
-                        P(nx + 1, j, nz + 1) = P(nx + 1, j, 1);
-                        P(0, j, nz + 1) = P(0, j, 1);
-                    }
-                }
-                /* 3) horizontal frame */
-                if (d->ip == 0) {
-                    for (int k = 1; k <= nz; ++k) {
-                        P(0, ny + 1, k) = P(nx, ny + 1, k);
-                        P(0, 0, k) = P(nx, 0, k);
-                        P(nx + 1, ny + 1, k) = P(1, ny + 1, k);
-                        P(nx + 1, 0, k) = P(1, 0, k);
-                    }
-                } else {
-                    for (int k = 1; k <= nz; ++k) {
-                        P(0, ny + 1, k) = P(0, ny, k);
-                        P(0, 0, k) = P(0, 1, k);
-                        P(nx + 1, ny + 1, k) = P(nx + 1, ny, k);
-                        P(nx + 1, 0, k) = P(nx + 1, 1, k);
-                    }
-                }
-            } else {
-                /* 1) corner lines along x */
-                for (int i = 1; i <= nx; ++i) {
-                    P(i, 0, 0) = P(i, 1, 0);
-                    P(i, ny + 1, 0) = P(i, ny, 0);
-                }
-                for (int i = 1; i <= nx; ++i) {
-                    P(i, 0, nz + 1) = P(i, 1, nz + 1);
-                    P(i, ny + 1, nz + 1) = P(i, ny, nz + 1);
-                }
-                /* 2) vertical frame */
-                if (d->ip == 0) {
-                    for (int j = 0; j <= ny + 1; ++j) {
-                        P(0, j, 0) = P(nx, j, 0);
-                        P(nx + 1, j, 0) = P(1, j, 0);
-                    }
-                    for (int j = 0; j <= ny + 1; ++j) {
-                        P(0, j, nz + 1) = P(nx, j, nz + 1);
-                        P(nx + 1, j, nz + 1) = P(1, j, nz + 1);
-                    }
-                } else {
-                    for (int j = 0; j <= ny + 1; ++j) {
-                        P(0, j, 0) = P(1, j, 0);
-                        P(nx + 1, j, 0) = P(nx, j, 0);
-                    }
-                    for (int j = 0; j <= ny + 1; ++j) {
-                        P(0, j, nz + 1) = P(1, j, nz + 1);
-                        P(nx + 1, j, nz + 1) = P(nx, j, nz + 1);
-                    }
-                }
-                /* 3) horizontal frame */
-                if (d->ip == 0) {
-                    for (int k = 1; k <= nz; ++k) {
-                        P(0, ny + 1, k) = P(nx, ny + 1, k);
-                        P(0, 0, k) = P(nx, 0, k);
-                        P(nx + 1, ny + 1, k) = P(1, ny + 1, k);
-                        P(nx + 1, 0, k) = P(1, 0, k);
-                    }
-                } else {
-                    for (int k = 1; k <= nz; ++k) {
-                        P(0, ny + 1, k) = P(1, ny + 1, k);
-                        P(0, 0, k) = P(1, 0, k);
-                        P(nx + 1, ny + 1, k) = P(nx, ny + 1, k);
-                        P(nx + 1, 0, k) = P(nx, 0, k);
-                    }
-                }
-            }
-        }
+        orc_edge_fill(d, p);
     }
 #undef CS
 }
@@ -655,6 +667,20 @@ void orc_resid(const orc_dims* d, const double* p, const double* rh, const doubl
                     (res_dx - res_sn + res_sop - res_sot + res_av - res_ind) * inv_pot - rh[GIDX(d, i, j, k)];
             }
 }
+
+/* orc_resid with its planes split over nthr OpenMP threads (bitwise the same) */
+static void orc_resid_mt(const orc_dims* d, const double* p, const double* rh, const double* const g[9],
+                         double pot, double* r, int nthr) {
+    const double inv_pot = 1. / pot, coef = .25;
+#pragma omp parallel for num_threads(nthr) if (nthr > 1) schedule(static)
+    for (int k = 1; k <= d->nz; ++k)
+        for (int j = 1; j <= d->ny; ++j)
+            for (int i = 1; i <= d->nx; ++i) {
+                FLUXES
+                r[GIDX(d, i, j, k)] =
+                    (res_dx - res_sn + res_sop - res_sot + res_av - res_ind) * inv_pot - rh[GIDX(d, i, j, k)];
+            }
+}
 #undef FLUXES
 
 void orc_restrict(const orc_dims* dc, const double* rf, double* rc) {
@@ -703,6 +729,167 @@ void orc_prolong(const orc_dims* dc, const double* ec, double* pf) {
                 }
             }
 }
+
+/* ---- line relaxation in eta (islor = 1) ------------------------------ */
+
+void orc_tridag(int n, double* aa, double* bb, double* cc, double* ff) {
+    /* fsrc/tridag.f03:11-30, rows 0..n-1 for the Fortran's 1..n */
+    bb[0] = 1. / bb[0];
+    aa[0] = ff[0] * bb[0];
+    for (int k = 1; k < n; ++k) {
+        const int k1 = k - 1;
+        cc[k1] = cc[k1] * bb[k1];
+        bb[k] = bb[k] - aa[k] * cc[k1];
+        bb[k] = 1. / bb[k];
+        aa[k] = (ff[k] - aa[k] * aa[k1]) * bb[k];
+    }
+    ff[n - 1] = aa[n - 1];
+    for (int k = n - 2; k >= 0; --k) ff[k] = aa[k] - cc[k] * ff[k + 1];
+}
+
+/* wall flags of sett (fsrc/sett.f03:53-108): 0 on a non-periodic boundary
+ * face of the cell, 1 elsewhere.  Global semantics: both zeta walls are
+ * flagged (the reference flags i_in on rank 0 and i_av on rank nproc-1,
+ * fsrc/sett.f03:95-108; a single rank would flag only i_in). */
+typedef struct {
+    int dx, sn, sp, st, av, in;
+} flags_t;
+
+static flags_t cell_flags(const orc_dims* d, int i, int j, int k) {
+    flags_t f = {1, 1, 1, 1, 1, 1};
+    if (i == 1) f.sn = 1 - d->ip;
+    if (i == d->nx) f.dx = 1 - d->ip;
+    if (j == 1) f.st = 1 - d->jp;
+    if (j == d->ny) f.sp = 1 - d->jp;
+    if (k == 1) f.in = 1 - d->kp;
+    if (k == d->nz) f.av = 1 - d->kp;
+    return f;
+}
+
+/* the line coefficients of column (i, k) (rows 0..ny+1) of sett,
+ * fsrc/sett.f03:119-143; d is the level (k, nz global) */
+static void sett_column(const orc_dims* d, const double* const g[9], double pot, int i, int k, int kl,
+                        double* aa, double* bb, double* cc) {
+    const double ppot2 = 1. / pot / pot;
+    const int ny = d->ny;
+    /* metrics of the level are indexed by the local plane kl (slab offset) */
+    const orc_dims dl = {d->nx, d->ny, 1, d->ip, d->jp, d->kp};
+#define GL(m, a, b, c) g[m][GIDX(&dl, a, b, c)]
+    for (int j = 1; j <= ny; ++j) {
+        const flags_t f = cell_flags(d, i, j, k);
+        aa[j] = ppot2 * GL(4, i, j - 1, kl);
+        bb[j] = -ppot2 * (GL(4, i, j, kl) + GL(4, i, j - 1, kl) + GL(0, i, j, kl) * f.dx + GL(0, i - 1, j, kl) * f.sn +
+                          GL(8, i, j, kl) * f.av + GL(8, i, j, kl - 1) * f.in);
+        cc[j] = ppot2 * GL(4, i, j, kl);
+    }
+    /* side bottom (3): aa = 0; side upper (4): cc = 0 */
+    aa[0] = 0.;
+    bb[0] = GL(4, i, 0, kl) * ppot2;
+    cc[0] = -GL(4, i, 0, kl) * ppot2;
+    aa[ny + 1] = -GL(4, i, ny, kl) * ppot2;
+    bb[ny + 1] = GL(4, i, ny, kl) * ppot2;
+    cc[ny + 1] = 0.;
+#undef GL
+}
+
+void orc_sett(const orc_dims* d, const double* const g[9], double pot, double* aa, double* bb, double* cc) {
+    const int ny = d->ny;
+    const size_t n = gsize(d);
+    memset(aa, 0, n * sizeof(double));
+    memset(bb, 0, n * sizeof(double));
+    memset(cc, 0, n * sizeof(double));
+    double* col = (double*)malloc((size_t)(ny + 2) * 3 * sizeof(double));
+    for (int k = 1; k <= d->nz; ++k)
+        for (int i = 1; i <= d->nx; ++i) {
+            sett_column(d, g, pot, i, k, k, col, col + ny + 2, col + 2 * (ny + 2));
+            for (int j = 0; j <= ny + 1; ++j) {
+                const size_t id = GIDX(d, i, j, k);
+                aa[id] = col[j];
+                bb[id] = col[ny + 2 + j];
+                cc[id] = col[2 * (ny + 2) + j];
+            }
+        }
+    free(col);
+}
+
+/* the line solves of one SLOR sweep over the planes of d (a k-slab: global
+ * plane = local + kofs, nzg global planes; p, rh and g offset to the slab) */
+static void slor_planes(const orc_dims* d, int kofs, int nzg, double* p, const double* rh,
+                        const double* const g[9], const double* const cs[6], double an, double dt,
+                        double pot, double omega) {
+    const int nx = d->nx, ny = d->ny, nz = d->nz;
+    const orc_dims dg = {nx, ny, nzg, d->ip, d->jp, d->kp};
+    const double ppot2 = 1. / pot / pot, q = .25;
+    const int have_cs = (cs != NULL) && an != 0.0;
+#define CSV(m, a, b, lda) (have_cs ? cs[m][(size_t)((a)-1) + (size_t)(lda) * (size_t)((b)-1)] : 0.0)
+    double* aa = (double*)malloc((size_t)(ny + 2) * 4 * sizeof(double));
+    double *bb = aa + (ny + 2), *cc = bb + (ny + 2), *ff = cc + (ny + 2);
+    for (int k = 1; k <= nz; ++k) {
+        if (d->ip == 0)
+            for (int j = 1; j <= ny; ++j) {
+                P(0, j, k) = P(nx, j, k);
+                P(nx + 1, j, k) = P(1, j, k);
+            }
+        for (int i = 1; i <= nx; ++i) {
+            for (int j = 1; j <= ny; ++j) { /* 2128-2183 */
+                const flags_t f = cell_flags(&dg, i, j, k + kofs);
+                double s =
+                    -G(0, i, j, k) * P(i + 1, j, k) * f.dx - (1. - f.dx) * an * CSV(1, j, k + kofs, ny) / dt -
+                    G(0, i - 1, j, k) * P(i - 1, j, k) * f.sn + (1. - f.sn) * an * CSV(0, j, k + kofs, ny) / dt -
+                    q * G(1, i, j, k) * f.dx * (P(i + 1, j + 1, k) + P(i, j + 1, k) - P(i + 1, j - 1, k) - P(i, j - 1, k)) -
+                    q * G(2, i, j, k) * f.dx * (P(i + 1, j, k + 1) + P(i, j, k + 1) - P(i + 1, j, k - 1) - P(i, j, k - 1)) +
+                    q * G(1, i - 1, j, k) * f.sn *
+                        (P(i, j + 1, k) + P(i - 1, j + 1, k) - P(i, j - 1, k) - P(i - 1, j - 1, k)) +
+                    q * G(2, i - 1, j, k) * f.sn *
+                        (P(i, j, k + 1) + P(i - 1, j, k + 1) - P(i, j, k - 1) - P(i - 1, j, k - 1)) -
+                    G(8, i, j, k) * P(i, j, k + 1) * f.av - (1. - f.av) * an * CSV(5, i, j, nx) / dt -
+                    G(8, i, j, k - 1) * P(i, j, k - 1) * f.in + (1. - f.in) * an * CSV(4, i, j, nx) / dt -
+                    q * G(6, i, j, k) * f.av * (P(i + 1, j, k + 1) + P(i + 1, j, k) - P(i - 1, j, k + 1) - P(i - 1, j, k)) -
+                    q * G(7, i, j, k) * f.av * (P(i, j + 1, k + 1) + P(i, j + 1, k) - P(i, j - 1, k + 1) - P(i, j - 1, k)) +
+                    q * G(6, i, j, k - 1) * f.in *
+                        (P(i + 1, j, k) + P(i + 1, j, k - 1) - P(i - 1, j, k) - P(i - 1, j, k - 1)) +
+                    q * G(7, i, j, k - 1) * f.in *
+                        (P(i, j + 1, k) + P(i, j + 1, k - 1) - P(i, j - 1, k) - P(i, j - 1, k - 1)) -
+                    q * G(3, i, j, k) * (P(i + 1, j + 1, k) + P(i + 1, j, k) - P(i - 1, j + 1, k) - P(i - 1, j, k)) -
+                    q * G(5, i, j, k) * (P(i, j + 1, k + 1) + P(i, j, k + 1) - P(i, j + 1, k - 1) - P(i, j, k - 1)) +
+                    q * G(3, i, j - 1, k) * (P(i + 1, j, k) + P(i + 1, j - 1, k) - P(i - 1, j, k) - P(i - 1, j - 1, k)) +
+                    q * G(5, i, j - 1, k) * (P(i, j, k + 1) + P(i, j - 1, k + 1) - P(i, j, k - 1) - P(i, j - 1, k - 1));
+                ff[j] = s * ppot2 + rh[GIDX(d, i, j, k)];
+            }
+            /* wall rows of the ghosts, 2185-2205 */
+            ff[0] = (-an * CSV(2, i, k + kofs, nx) / dt +
+                     q * G(3, i, 0, k) * (P(i + 1, 1, k) + P(i + 1, 0, k) - P(i - 1, 1, k) - P(i - 1, 0, k)) +
+                     q * G(5, i, 0, k) * (P(i, 1, k + 1) + P(i, 0, k + 1) - P(i, 1, k - 1) - P(i, 0, k - 1))) *
+                    ppot2;
+            ff[ny + 1] = (an * CSV(3, i, k + kofs, nx) / dt -
+                          q * G(3, i, ny, k) *
+                              (P(i + 1, ny + 1, k) + P(i + 1, ny, k) - P(i - 1, ny + 1, k) - P(i - 1, ny, k)) -
+                          q * G(5, i, ny, k) *
+                              (P(i, ny + 1, k + 1) + P(i, ny, k + 1) - P(i, ny + 1, k - 1) - P(i, ny, k - 1))) *
+                         ppot2;
+            sett_column(&dg, g, pot, i, k + kofs, k, aa, bb, cc);
+            orc_tridag(ny + 2, aa, bb, cc, ff);
+            for (int j = 0; j <= ny + 1; ++j) P(i, j, k) = P(i, j, k) + omega * (ff[j] - P(i, j, k));
+        }
+    }
+    free(aa);
+#undef CSV
+}
+
+void orc_slor_sweep(const orc_dims* d, double* p, const double* rh, const double* const g[9],
+                    const double* const cs[6], double an, double dt, double pot, double omega, int nbc) {
+    /* one pseudo-time iteration (kk) of solut_sndrcv_slor,
+     * fsrc/multigrid.f03:2050-2199 */
+    const int nx = d->nx, ny = d->ny, nz = d->nz;
+    orc_ghost_fill(d, p, g, cs, an, dt, nbc); /* mul_boun_sndrcv (2050-2057) */
+    if (d->kp == 0) {                         /* periodic-k planes (2068-2094) */
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) P(i, j, 0) = P(i, j, nz);
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) P(i, j, nz + 1) = P(i, j, 1);
+    }
+    slor_planes(d, 0, nz, p, rh, g, cs, an, dt, pot, omega);
+}
 #undef P
 #undef G
 
@@ -710,29 +897,137 @@ void orc_prolong(const orc_dims* dc, const double* ec, double* pf) {
 /* V-cycle driver                                                      */
 /* ------------------------------------------------------------------ */
 
-int orc_multi(const orc_dims* d, int nlevel, const double* x, const double* y, const double* z,
-              const double* const g1[9], double* fi, const double* rhs, const double* const cs[6],
-              double dt, double omega, double eps, int ficycle, int kss1, int kssc, int nbc,
-              double* resmax_out, double* res_hist) {
-    int L = nlevel;
-    orc_dims* dl = (orc_dims*)calloc((size_t)L, sizeof(orc_dims));
-    double*** gl = (double***)calloc((size_t)L, sizeof(double**));
-    double** pr = (double**)calloc((size_t)L, sizeof(double*));
-    double** rv = (double**)calloc((size_t)L, sizeof(double*)); /* level rhs (v) */
-    double** rn = (double**)calloc((size_t)L, sizeof(double*)); /* level residual (n) */
+/* Level hierarchy of multi: per-level dims, metrics (metrica / mul_met,
+ * computed once as the reference does at init, fsrc/stratiParticle.f03:767-769)
+ * and the work arrays pr, rhs, residual of fsrc/multigrid.f03:144-154. */
+struct orc_hier {
+    orc_dims d;
+    int L;
+    orc_dims* dl;
+    double*** gl;
+    double **pr, **rv, **rn;
+    double** slab; /* k-slab buffers (nslab > 1) */
+    int nslab_alloc;
+};
+
+orc_hier* orc_hier_create(const orc_dims* d, int nlevel, const double* x, const double* y, const double* z,
+                          const double* const g1[9]) {
+    const int L = nlevel;
+    orc_hier* H = (orc_hier*)calloc(1, sizeof(orc_hier));
+    H->d = *d;
+    H->L = L;
+    H->dl = (orc_dims*)calloc((size_t)L, sizeof(orc_dims));
+    H->gl = (double***)calloc((size_t)L, sizeof(double**));
+    H->pr = (double**)calloc((size_t)L, sizeof(double*));
+    H->rv = (double**)calloc((size_t)L, sizeof(double*));
+    H->rn = (double**)calloc((size_t)L, sizeof(double*));
     for (int n = 0; n < L; ++n) {
-        int st = 1 << n;
-        dl[n] = (orc_dims){d->nx / st, d->ny / st, d->nz / st, d->ip, d->jp, d->kp};
-        size_t sz = gsize(&dl[n]);
-        gl[n] = (double**)calloc(9, sizeof(double*));
-        for (int m = 0; m < 9; ++m) gl[n][m] = (double*)calloc(sz, sizeof(double));
-        pr[n] = (double*)calloc(sz, sizeof(double));
-        rv[n] = (double*)calloc(sz, sizeof(double));
-        rn[n] = (double*)calloc(sz, sizeof(double));
+        const int st = 1 << n;
+        H->dl[n] = (orc_dims){d->nx / st, d->ny / st, d->nz / st, d->ip, d->jp, d->kp};
+        const size_t sz = gsize(&H->dl[n]);
+        H->gl[n] = (double**)calloc(9, sizeof(double*));
+        for (int m = 0; m < 9; ++m) H->gl[n][m] = (double*)calloc(sz, sizeof(double));
+        H->pr[n] = (double*)calloc(sz, sizeof(double));
+        H->rv[n] = (double*)calloc(sz, sizeof(double));
+        H->rn[n] = (double*)calloc(sz, sizeof(double));
         if (n == 0 && g1) {
-            for (int m = 0; m < 9; ++m) memcpy(gl[0][m], g1[m], sz * sizeof(double));
+            for (int m = 0; m < 9; ++m) memcpy(H->gl[0][m], g1[m], sz * sizeof(double));
         } else {
-            orc_metrics(d, x, y, z, st, gl[n], NULL, NULL);
+            orc_metrics(d, x, y, z, st, H->gl[n], NULL, NULL);
+        }
+    }
+    return H;
+}
+
+void orc_hier_destroy(orc_hier* H) {
+    if (!H) return;
+    for (int n = 0; n < H->L; ++n) {
+        for (int m = 0; m < 9; ++m) free(H->gl[n][m]);
+        free(H->gl[n]);
+        free(H->pr[n]);
+        free(H->rv[n]);
+        free(H->rn[n]);
+    }
+    for (int s = 0; s < H->nslab_alloc; ++s) free(H->slab[s]);
+    free(H->slab);
+    free(H->dl);
+    free(H->gl);
+    free(H->pr);
+    free(H->rv);
+    free(H->rn);
+    free(H);
+}
+
+/* One smoother sweep of level d on ns k-slabs, the reference's MPI slab
+ * semantics (fsrc/multigrid.f03:929-966): every slab relaxes its own planes
+ * in the reference order, reading the planes next to it (its halo) as they
+ * were when the sweep started; the halos are exchanged after the sweep.
+ * The slabs run on ns OpenMP threads.  The ghost fill before the sweep is
+ * global.  Results do not depend on the thread count, only on ns. */
+static void sweep_slabs(orc_hier* H, const orc_dims* d, double* p, const double* rh, const double* const g[9],
+                        const double* const cs[6], double an, double dt, double pot, double omega, int islor,
+                        int ns) {
+    const int nz = d->nz, nzs = nz / ns;
+    const size_t sz = (size_t)(d->nx + 2) * (size_t)(d->ny + 2);
+    if (d->kp == 0) { /* periodic-k planes before the sweep (fsrc/multigrid.f03:784-812) */
+        for (int j = 1; j <= d->ny; ++j)
+            for (int i = 1; i <= d->nx; ++i) {
+                p[GIDX(d, i, j, 0)] = p[GIDX(d, i, j, nz)];
+                p[GIDX(d, i, j, nz + 1)] = p[GIDX(d, i, j, 1)];
+            }
+    }
+#pragma omp parallel for num_threads(ns) schedule(static, 1)
+    for (int s = 0; s < ns; ++s)
+        memcpy(H->slab[s], p + sz * (size_t)(s * nzs), (size_t)(nzs + 2) * sz * sizeof(double));
+#pragma omp parallel for num_threads(ns) schedule(static, 1)
+    for (int s = 0; s < ns; ++s) {
+        const size_t off = sz * (size_t)(s * nzs);
+        const orc_dims ds = {d->nx, d->ny, nzs, d->ip, d->jp, 1};
+        const double* gs[9];
+        for (int m = 0; m < 9; ++m) gs[m] = g[m] + off;
+        if (islor)
+            slor_planes(&ds, s * nzs, nz, H->slab[s], rh + off, gs, cs, an, dt, pot, omega);
+        else
+            orc_sor_sweep(&ds, H->slab[s], rh + off, gs, pot, omega);
+    }
+#pragma omp parallel for num_threads(ns) schedule(static, 1)
+    for (int s = 0; s < ns; ++s)
+        memcpy(p + sz * (size_t)(s * nzs + 1), H->slab[s] + sz, (size_t)nzs * sz * sizeof(double));
+}
+
+static void smooth_level(orc_hier* H, int n, const double* rh, const double* const cs[6], double dt,
+                         double omega, int kss, int nbc, int islor, int ns) {
+    const orc_dims* dn = &H->dl[n];
+    const double pot = (double)(1 << n), an = n == 0 ? 1. : 0.;
+    const double* const* g = (const double* const*)H->gl[n];
+    for (int s = 0; s < kss; ++s) {
+        if (ns > 1) {
+            orc_ghost_fill(dn, H->pr[n], g, cs, an, dt, nbc);
+            sweep_slabs(H, dn, H->pr[n], rh, g, cs, an, dt, pot, omega, islor, ns);
+        } else if (islor) {
+            orc_slor_sweep(dn, H->pr[n], rh, g, cs, an, dt, pot, omega, nbc);
+        } else {
+            orc_ghost_fill(dn, H->pr[n], g, cs, an, dt, nbc);
+            orc_sor_sweep(dn, H->pr[n], rh, g, pot, omega);
+        }
+    }
+}
+
+int orc_hier_multi(orc_hier* H, double* fi, const double* rhs, const double* const cs[6], double dt,
+                   double omega, double eps, int ficycle, int kss1, int kssc, int nbc, int islor, int nslab,
+                   double* resmax_out, double* res_hist) {
+    const int L = H->L;
+    const orc_dims* d = &H->d;
+    const int ns = nslab > 1 ? nslab : 1;
+    if (ns > 1) {
+        if (d->nz % (ns << (L - 1)) != 0) return -1;
+        if (H->nslab_alloc < ns) {
+            for (int s = 0; s < H->nslab_alloc; ++s) free(H->slab[s]);
+            free(H->slab);
+            const size_t sz = (size_t)(d->nx + 2) * (size_t)(d->ny + 2);
+            H->slab = (double**)calloc((size_t)ns, sizeof(double*));
+            for (int s = 0; s < ns; ++s) H->slab[s] = (double*)malloc((size_t)(d->nz / ns + 2) * sz * sizeof(double));
+            H->nslab_alloc = ns;
         }
     }
     const size_t sz0 = gsize(d);
@@ -741,52 +1036,50 @@ int orc_multi(const orc_dims* d, int nlevel, const double* x, const double* y, c
     /* fsrc/multigrid.f03:174-637 */
     while (ktime < ficycle && resmax >= eps) {
         ++ktime;
-        memcpy(pr[0], fi, sz0 * sizeof(double));
-        for (int n = 1; n < L; ++n) memset(pr[n], 0, gsize(&dl[n]) * sizeof(double));
+        memcpy(H->pr[0], fi, sz0 * sizeof(double));
+        for (int n = 1; n < L; ++n) memset(H->pr[n], 0, gsize(&H->dl[n]) * sizeof(double));
         /* down leg */
         for (int n = 0; n < L; ++n) {
             const double pot = (double)(1 << n);
-            const double* rh = (n == 0) ? rhs : rv[n];
-            const int kss = (n == 0) ? kss1 : kssc;
-            for (int s = 0; s < kss; ++s) {
-                orc_ghost_fill(&dl[n], pr[n], (const double* const*)gl[n], cs, n == 0 ? 1. : 0., dt, nbc);
-                orc_sor_sweep(&dl[n], pr[n], rh, (const double* const*)gl[n], pot, omega);
-            }
+            const double* rh = (n == 0) ? rhs : H->rv[n];
+            smooth_level(H, n, rh, cs, dt, omega, n == 0 ? kss1 : kssc, nbc, islor, ns);
             if (n == 0 || n < L - 1) {
-                orc_resid(&dl[n], pr[n], rh, (const double* const*)gl[n], pot, rn[n]);
-                if (n < L - 1) orc_restrict(&dl[n + 1], rn[n], rv[n + 1]);
+                orc_resid_mt(&H->dl[n], H->pr[n], rh, (const double* const*)H->gl[n], pot, H->rn[n], ns);
+                if (n < L - 1) orc_restrict(&H->dl[n + 1], H->rn[n], H->rv[n + 1]);
             }
         }
         /* up leg */
         for (int n = L - 2; n >= 0; --n) {
-            const double pot = (double)(1 << n);
-            const double* rh = (n == 0) ? rhs : rv[n];
-            const int kss = (n == 0) ? kss1 : kssc;
-            orc_prolong(&dl[n + 1], pr[n + 1], pr[n]);
-            for (int s = 0; s < kss; ++s) {
-                orc_ghost_fill(&dl[n], pr[n], (const double* const*)gl[n], cs, n == 0 ? 1. : 0., dt, nbc);
-                orc_sor_sweep(&dl[n], pr[n], rh, (const double* const*)gl[n], pot, omega);
-            }
+            const double* rh = (n == 0) ? rhs : H->rv[n];
+            orc_prolong(&H->dl[n + 1], H->pr[n + 1], H->pr[n]);
+            smooth_level(H, n, rh, cs, dt, omega, n == 0 ? kss1 : kssc, nbc, islor, ns);
         }
-        if (L > 1) orc_resid(&dl[0], pr[0], rhs, (const double* const*)gl[0], 1., rn[0]);
-        memcpy(fi, pr[0], sz0 * sizeof(double));
-        resmax = orc_absmax(d, rn[0]);
+        if (L > 1) orc_resid_mt(&H->dl[0], H->pr[0], rhs, (const double* const*)H->gl[0], 1., H->rn[0], ns);
+        memcpy(fi, H->pr[0], sz0 * sizeof(double));
+        resmax = orc_absmax(d, H->rn[0]);
         if (res_hist) res_hist[ktime - 1] = resmax;
     }
     if (resmax_out) *resmax_out = resmax;
-    for (int n = 0; n < L; ++n) {
-        for (int m = 0; m < 9; ++m) free(gl[n][m]);
-        free(gl[n]);
-        free(pr[n]);
-        free(rv[n]);
-        free(rn[n]);
-    }
-    free(dl);
-    free(gl);
-    free(pr);
-    free(rv);
-    free(rn);
     return ktime;
+}
+
+int orc_multi_ex(const orc_dims* d, int nlevel, const double* x, const double* y, const double* z,
+                 const double* const g1[9], double* fi, const double* rhs, const double* const cs[6], double dt,
+                 double omega, double eps, int ficycle, int kss1, int kssc, int nbc, int islor,
+                 double* resmax_out, double* res_hist) {
+    orc_hier* H = orc_hier_create(d, nlevel, x, y, z, g1);
+    const int n = orc_hier_multi(H, fi, rhs, cs, dt, omega, eps, ficycle, kss1, kssc, nbc, islor, 1,
+                                 resmax_out, res_hist);
+    orc_hier_destroy(H);
+    return n;
+}
+
+int orc_multi(const orc_dims* d, int nlevel, const double* x, const double* y, const double* z,
+              const double* const g1[9], double* fi, const double* rhs, const double* const cs[6],
+              double dt, double omega, double eps, int ficycle, int kss1, int kssc, int nbc,
+              double* resmax_out, double* res_hist) {
+    return orc_multi_ex(d, nlevel, x, y, z, g1, fi, rhs, cs, dt, omega, eps, ficycle, kss1, kssc, nbc, 0,
+                        resmax_out, res_hist);
 }
 
 /* ------------------------------------------------------------------ */

@@ -30,14 +30,28 @@
  *   - periodic flags follow the reference (Aboundary.in): ip/jp/kp = 1 means
  *     walls (non-periodic), 0 means periodic.
  *
- * What pins each part (tests/test_oracle.py): metrics exact on affine maps at
- * every level and on quadratic maps; operator exact on quadratics; wall ghosts
- * reproduce an affine pressure (eta walls); restriction/prolongation
- * invariants; orc_multi equal to the exact FFT/tridiagonal channel solution
- * and to a dense solve on a warped grid; case rhs = divergence of the face
- * fluxes; gradie exact on an affine pressure.  Parity unpinned: the
- * edge/corner ghost copy rules with xi/zeta walls
- * (fsrc/multigrid.f03:1358-1742) and the per-cycle SOR residual trajectory.
+ * What pins each part (tests/test_oracle*.py):
+ *   - metrics: exact on affine node maps at every level (constant metric
+ *     tensor with cross terms), and on node maps quadratic in each index with
+ *     walls in every direction at every level, where the central and the
+ *     second-order one-sided wall derivatives are exact at the face centres
+ *     (a first-order wall derivative fails it);
+ *   - operator: exact on quadratic pressures, annihilates affine ones;
+ *   - wall ghosts: an affine pressure with its exact wall flux is reproduced;
+ *   - edge/corner ghosts: a field that satisfies every boundary condition is
+ *     kept, and fields constant along chosen directions fix the direction of
+ *     every copy;
+ *   - transfers: restriction of constants and the sum rule, prolongation of
+ *     constants and linear fields;
+ *   - tridag against a banded solve; sett's rows against the operator on a
+ *     column; SLOR: the first relaxed column solves its equations, the exact
+ *     solution is a fixed point on warped walled grids with fluxes;
+ *   - orc_multi: the exact FFT/tridiagonal channel solution and a dense solve
+ *     on a warped grid; the SLOR V-cycle reaches the SOR V-cycle's solution;
+ *   - case rhs = divergence of the face fluxes; gradie exact on an affine
+ *     pressure; projection-step routines (tests/test_oracle_projection.py).
+ * Parity unpinned: the per-cycle residual trajectory of the reference's
+ * smoothers (only a run of the reference itself would fix it).
  */
 #ifndef PLES_ORACLE_H
 #define PLES_ORACLE_H
@@ -128,6 +142,61 @@ int orc_multi(const orc_dims* d, int nlevel, const double* x, const double* y,
               const double* rhs, const double* const cs[6], double dt, double omega,
               double eps, int ficycle, int kss1, int kssc, int nbc,
               double* resmax_out, double* res_hist);
+
+/* The same with the smoother of the reference's islor switch
+ * (fsrc/multigrid.f03:223-563): islor = 0 the lexicographic point SOR,
+ * islor = 1 the eta-line SOR (orc_slor_sweep). */
+int orc_multi_ex(const orc_dims* d, int nlevel, const double* x, const double* y,
+                 const double* z, const double* const g1[9], double* fi,
+                 const double* rhs, const double* const cs[6], double dt, double omega,
+                 double eps, int ficycle, int kss1, int kssc, int nbc, int islor,
+                 double* resmax_out, double* res_hist);
+
+/* Level hierarchy of multi built once (metrics of every level, as the
+ * reference's init does with metrica / mul_met, and the work arrays), then
+ * any number of solves.  nslab > 1 runs every smoother sweep on nslab
+ * k-slabs with the reference's MPI slab semantics (each slab relaxes its own
+ * planes with its halo planes as they were at the start of the sweep,
+ * fsrc/multigrid.f03:929-966), one OpenMP thread per slab when built with
+ * OpenMP; requires nz % (nslab * 2^(nlevel-1)) == 0 (else returns -1).
+ * nslab <= 1 is orc_multi_ex. */
+typedef struct orc_hier orc_hier;
+orc_hier* orc_hier_create(const orc_dims* d, int nlevel, const double* x, const double* y,
+                          const double* z, const double* const g1[9]);
+void orc_hier_destroy(orc_hier* H);
+int orc_hier_multi(orc_hier* H, double* fi, const double* rhs, const double* const cs[6],
+                   double dt, double omega, double eps, int ficycle, int kss1, int kssc,
+                   int nbc, int islor, int nslab, double* resmax_out, double* res_hist);
+
+/* ---- line relaxation (islor = 1) ------------------------------------ */
+
+/* Thomas algorithm of tridag (fsrc/tridag.f03:11-30) on rows 0..n-1:
+ * aa[r] x[r-1] + bb[r] x[r] + cc[r] x[r+1] = ff[r]; aa, bb, cc are
+ * overwritten as the reference does, the solution replaces ff. */
+void orc_tridag(int n, double* aa, double* bb, double* cc, double* ff);
+
+/* Line coefficients of sett (fsrc/sett.f03:119-143) of a level with 1/s =
+ * pot, ghosted layout, j = 0..ny+1 of every column (i,k): interior rows
+ * aa = s^2 g22(j-1), bb = -s^2 (g22(j) + g22(j-1) + the xi/zeta normal
+ * metrics of the non-wall faces), cc = s^2 g22(j); the wall rows
+ * 0 and ny+1 are the flux relations of the eta-wall ghosts. */
+void orc_sett(const orc_dims* d, const double* const g[9], double pot, double* aa,
+              double* bb, double* cc);
+
+/* One sweep of solut_sndrcv_slor (fsrc/multigrid.f03:2050-2199): the ghost
+ * fill (nbc passes, an, cs, dt as in orc_ghost_fill), the periodic-k plane
+ * exchange, then for every plane k and column i (in that order) the
+ * tridiagonal eta-line system of rows 0..ny+1 with the explicit terms at
+ * their current values, solved by orc_tridag, and p += omega (line solution
+ * - p) on the whole column including its wall ghosts. */
+void orc_slor_sweep(const orc_dims* d, double* p, const double* rh, const double* const g[9],
+                    const double* const cs[6], double an, double dt, double pot, double omega,
+                    int nbc);
+
+/* Edge and corner ghosts only (the last step of every orc_ghost_fill pass,
+ * fsrc/multigrid.f03:1358-1742): each is a copy of its periodic image or of
+ * its neighbour towards the interior across a wall. */
+void orc_edge_fill(const orc_dims* d, double* p);
 
 /* ---- projection step ------------------------------------------------ */
 
