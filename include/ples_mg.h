@@ -164,6 +164,18 @@ int ples_mg_residual(ples_mg_handle* h, const double* fi, int deepl, int deepr,
                      const double* cs3, const double* cs4, const double* cs5,
                      const double* cs6, double dt, double* r);
 
+/* Zero-mean (compatibility) projection of a rhs, in place on host arrays:
+ * rhs -= (sum(rhs) - (sum of the wall fluxes cs, outflow - inflow) / dt) / N,
+ * in FP64 on the device; *removed (may be NULL) receives the value taken off.
+ * The reference has no such step (it only prints the rhs integral,
+ * fsrc/stratiParticle.f03:2847-2866); on a Neumann / periodic problem an
+ * inconsistent rhs has no solution.  ples_mg_solve, ples_mg_solve_resident
+ * and ples_mg_projection_step apply it by themselves (opts.project_rhs).
+ * The projected rhs and cs become the handle's resident ones. */
+int ples_mg_project_rhs(ples_mg_handle* h, double* rhs, const double* cs1, const double* cs2,
+                        const double* cs3, const double* cs4, const double* cs5, const double* cs6,
+                        double dt, double* removed);
+
 /* Level-1 metrics as metrica builds them (fsrc/metrica.f03:85-483), in the
  * reference shapes: g[9] (see above), face vectors csv[9] = csx,csy,csz
  * (0:jx,1:jy,1:jz), etx,ety,etz (1:jx,0:jy,1:jz), ztx,zty,ztz
@@ -175,6 +187,37 @@ int ples_mg_metrica(ples_mg_handle* h, double* const g[9], double* const csv[9],
  * level's counts from ples_mg_level_shape, face index f stored at cell index
  * f.  For parity tests. */
 int ples_mg_level_metrics(ples_mg_handle* h, int level, double* const g[9]);
+
+/* One operation of multigrid level n (1-based) on host arrays in the
+ * ghosted layout of that level ((nx+2)(ny+2)(nz+2) values, i fastest;
+ * counts from ples_mg_level_shape), run by the kernels the V-cycle runs, in
+ * the level's precision (inputs of an FP32 level are rounded to FP32).  For
+ * kernel-level parity tests:
+ *   PLES_MG_OP_SMOOTH: one eta-line smoother sweep, the relaxation of
+ *     solut_sndrcv_sor / solut_sndrcv_slor (fsrc/multigrid.f03:656-1010,
+ *     1939-2290): in0 = iterate (its ghosts as given), in1 = rhs, out = the
+ *     iterate after the sweep (p + omega M_line^-1 (b - A p) in the interior,
+ *     eta-wall ghosts and periodic images rewritten); P = eta segments per
+ *     column (0: the level's own), omega = the damping;
+ *   PLES_MG_OP_RESID: r = A p - b (fsrc/resid_sndrcv.f03:80-151), levels
+ *     below the coarsest: in0 = p, in1 = b, out = r;
+ *   PLES_MG_OP_RESTRICT: rc = -(1/8) sum of the 8 children
+ *     (fsrc/restrict_sndrcv.f03:32-53): in0 = fine residual (level n), out =
+ *     coarse rhs (level n+1);
+ *   PLES_MG_OP_RESID_RESTRICT: the two fused as the V-cycle runs them: in0 =
+ *     p, in1 = b (level n), out = coarse rhs (level n+1);
+ *   PLES_MG_OP_PROLONG: pf += trilinear prolongation of ec over the
+ *     ghost-inclusive range (fsrc/prolong.f03:64-261): in0 = ec (level n+1,
+ *     ghosts included), in1 = pf (level n), out = the updated pf.
+ * The operations overwrite the handle's level work arrays (not the resident
+ * iterate or rhs of level 1). */
+#define PLES_MG_OP_SMOOTH 0
+#define PLES_MG_OP_RESID 1
+#define PLES_MG_OP_RESTRICT 2
+#define PLES_MG_OP_RESID_RESTRICT 3
+#define PLES_MG_OP_PROLONG 4
+int ples_mg_level_op(ples_mg_handle* h, int level, int op, int P, double omega, const double* in0,
+                     const double* in1, double* out);
 
 /* Cell counts of multigrid level n (1-based): jz/2^(n-1), and jx/2^(n-1),
  * jy/2^(n-1), or jx/2^(n-2) (and jy/2^(n-2)) below a first coarse level that
@@ -361,10 +404,16 @@ int ples_mg_restart_read(const char* path, int jx, int jy, int jz, int nscal, do
  * those shapes without timing; save the current table. */
 int ples_mg_tuning_load(const char* path);
 int ples_mg_tuning_save(const char* path);
+/* Empty the table: handles created afterwards take the fixed rule's shapes
+ * (or time theirs with opts.autotune = 1). */
+int ples_mg_tuning_clear(void);
 
-/* Measured copy bandwidth of device `device` (GB/s, read + write bytes) for a
- * 16-byte-per-lane copy of `bytes` bytes repeated reps times. */
+/* HBM bandwidth probes (no counterpart in the reference: the roofline's
+ * measured ceiling).  Copy: GB/s of read + write bytes of a copy of `bytes`
+ * bytes repeated reps times; read: GB/s of a read-only sweep.  Each lane
+ * keeps four independent 16-byte loads in flight. */
 int ples_mg_copy_bandwidth(int device, size_t bytes, int reps, double* gbs);
+int ples_mg_read_bandwidth(int device, size_t bytes, int reps, double* gbs);
 
 /* Launch shape of level `level` (1-based): smoother segments per column P,
  * V-cycle residual segments rP x lanes rbx (outputs may be NULL). */

@@ -812,13 +812,14 @@ void orc_sett(const orc_dims* d, const double* const g[9], double pot, double* a
     free(col);
 }
 
-/* the line solves of one SLOR sweep over the planes of d (a k-slab: global
- * plane = local + kofs, nzg global planes; p, rh and g offset to the slab) */
-static void slor_planes(const orc_dims* d, int kofs, int nzg, double* p, const double* rh,
+/* the line solves of one SLOR sweep over the planes of d (a k-slab of the
+ * level dglob: global plane = local + kofs; p, rh and g offset to the slab;
+ * the wall flags are the level's) */
+static void slor_planes(const orc_dims* d, const orc_dims* dglob, int kofs, double* p, const double* rh,
                         const double* const g[9], const double* const cs[6], double an, double dt,
                         double pot, double omega) {
     const int nx = d->nx, ny = d->ny, nz = d->nz;
-    const orc_dims dg = {nx, ny, nzg, d->ip, d->jp, d->kp};
+    const orc_dims dg = *dglob;
     const double ppot2 = 1. / pot / pot, q = .25;
     const int have_cs = (cs != NULL) && an != 0.0;
 #define CSV(m, a, b, lda) (have_cs ? cs[m][(size_t)((a)-1) + (size_t)(lda) * (size_t)((b)-1)] : 0.0)
@@ -888,7 +889,7 @@ void orc_slor_sweep(const orc_dims* d, double* p, const double* rh, const double
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) P(i, j, nz + 1) = P(i, j, 1);
     }
-    slor_planes(d, 0, nz, p, rh, g, cs, an, dt, pot, omega);
+    slor_planes(d, d, 0, p, rh, g, cs, an, dt, pot, omega);
 }
 #undef P
 #undef G
@@ -986,7 +987,7 @@ static void sweep_slabs(orc_hier* H, const orc_dims* d, double* p, const double*
         const double* gs[9];
         for (int m = 0; m < 9; ++m) gs[m] = g[m] + off;
         if (islor)
-            slor_planes(&ds, s * nzs, nz, H->slab[s], rh + off, gs, cs, an, dt, pot, omega);
+            slor_planes(&ds, d, s * nzs, H->slab[s], rh + off, gs, cs, an, dt, pot, omega);
         else
             orc_sor_sweep(&ds, H->slab[s], rh + off, gs, pot, omega);
     }
@@ -1021,7 +1022,7 @@ int orc_hier_multi(orc_hier* H, double* fi, const double* rhs, const double* con
     const int ns = nslab > 1 ? nslab : 1;
     if (ns > 1) {
         if (d->nz % (ns << (L - 1)) != 0) return -1;
-        if (H->nslab_alloc < ns) {
+        if (H->nslab_alloc != ns) { /* buffers hold nz / ns + 2 planes */
             for (int s = 0; s < H->nslab_alloc; ++s) free(H->slab[s]);
             free(H->slab);
             const size_t sz = (size_t)(d->nx + 2) * (size_t)(d->ny + 2);

@@ -1938,11 +1938,36 @@ __global__ void __launch_bounds__(256) k_add_back(Geo G, double* __restrict__ x,
     x[c] += s * (double)e32[c];
 }
 
-// plain 16-byte copy (bandwidth probe)
+// Bandwidth probes.  Each lane keeps four independent 16-byte loads in
+// flight (four float4 one block-width apart, so every load instruction of a
+// wavefront reads 1 KB contiguous), then stores them (copy) or folds them
+// into a register written once per lane (read).
 __global__ void __launch_bounds__(256) k_copy16(long n, const float4* __restrict__ a, float4* __restrict__ b) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (; t < n; t += stride) b[t] = a[t];
+    const long bw = blockDim.x;
+    const long stride = (long)gridDim.x * 4 * bw;
+    for (long t = (long)blockIdx.x * 4 * bw + threadIdx.x; t < n; t += stride) {
+        float4 v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (t + q * bw < n) v[q] = a[t + q * bw];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (t + q * bw < n) b[t + q * bw] = v[q];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_read16(long n, const float4* __restrict__ a, float* __restrict__ out) {
+    const long bw = blockDim.x;
+    const long stride = (long)gridDim.x * 4 * bw;
+    float acc = 0.f;
+    for (long t = (long)blockIdx.x * 4 * bw + threadIdx.x; t < n; t += stride) {
+        float4 v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = (t + q * bw < n) ? a[t + q * bw] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc += (v[q].x + v[q].y) + (v[q].z + v[q].w);
+    }
+    out[(long)blockIdx.x * bw + threadIdx.x] = acc;
 }
 
 // elementwise helpers over the full ghosted array

@@ -75,6 +75,7 @@ struct ples_mg_handle {
     double* W[2] = {};
     double* cs[6] = {};
     bool have_cs = false;
+    double cs_sum = 0.0;  // sum of the wall fluxes cs (outflow - inflow) of the resident cs
     double dt = 1.0;
     double omega_now = 0.8;               // damping of the sweep being launched
     int halo_passes32 = 6;                // wall-ghost passes of the FP32 inner direction
@@ -1264,10 +1265,14 @@ bool valid_segments(int P) {
     return P == 1 || P == 2 || P == 3 || P == 4 || P == 6 || P == 8 || P == 12 || P == 16;
 }
 
+// a segment count the kernels and the autotuner accept for ny rows: a valid
+// count dividing ny into segments of at least four rows
+bool seg_ok(int ny, int P) { return valid_segments(P) && ny % P == 0 && ny / P >= 4; }
+
 std::vector<int> seg_candidates(int ny) {
     std::vector<int> v;
     for (int P : {1, 2, 3, 4, 6, 8, 12, 16})
-        if (ny % P == 0 && ny / P >= 4) v.push_back(P);
+        if (seg_ok(ny, P)) v.push_back(P);
     return v;
 }
 
@@ -1952,20 +1957,32 @@ void* ples_mg_stream(ples_mg_handle* h) { return h ? (void*)h->s : nullptr; }
 size_t ples_mg_device_bytes(const ples_mg_handle* h) { return h ? h->bytes : 0; }
 
 // ---------------------------------------------------------------- rhs helpers
-static int project_rhs(H* h, double bflux) {
+// FP64 compatibility projection of the resident rhs on the handle's stream:
+// S[S_SUM] = sum(b), then b -= (S[S_SUM] - bflux) / N when on (no host sync)
+static int project_rhs_async(H* h, double bflux, bool on) {
     const Level& L0 = h->lv[0];
     const ColLaunch cl = col_launch(L0.G);
     hipLaunchKernelGGL(k_sum, cl.grid, cl.block, 0, h->s, L0.G, (const double*)h->bvec, h->part, cl.nblocks);
     hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1), 0, h->s, (const double*)h->part, cl.nblocks, 2, h->S);
-    const double ncell = (double)L0.ncell();
-    if (h->o.project_rhs) {
+    if (on) {
         const ColLaunch ce = cell_launch(L0.G);
         hipLaunchKernelGGL(k_sub_mean, ce.grid, ce.block, 0, h->s, L0.G, h->bvec, (const double*)h->S, bflux,
-                           ncell);
+                           (double)L0.ncell());
     }
     CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+// the value project_rhs_async took off, once h->Sh holds S
+static double removed_mean(const H* h, double bflux, bool on) {
+    return on ? (h->Sh[S_SUM] - bflux) / (double)h->lv[0].ncell() : 0.0;
+}
+
+static int project_rhs(H* h, double bflux, bool force = false) {
+    const bool on = force || h->o.project_rhs;
+    CKS(project_rhs_async(h, bflux, on));
     CKS(read_S(h));
-    h->last_removed = h->o.project_rhs ? (h->Sh[S_SUM] - bflux) / ncell : 0.0;
+    h->last_removed = removed_mean(h, bflux, on);
     return PLES_MG_OK;
 }
 
@@ -1993,6 +2010,25 @@ static int upload_cs(H* h, const double* cs[6]) {
         }
     }
     h->have_cs = any;
+    h->cs_sum = boundary_flux(h, cs, 1.0);
+    return PLES_MG_OK;
+}
+
+// cs_sum of the resident (device) wall fluxes, e.g. after contra computed them
+static int cs_sum_device(H* h) {
+    std::vector<double> hcs[6];
+    const double* csp[6] = {};
+    const size_t csz[6] = {(size_t)h->jy * h->jz, (size_t)h->jy * h->jz, (size_t)h->jx * h->jz,
+                           (size_t)h->jx * h->jz, (size_t)h->jx * h->jy, (size_t)h->jx * h->jy};
+    const int wall[3] = {h->ip, h->jp, h->kp};
+    for (int q = 0; q < 6; ++q) {
+        if (!wall[q / 2]) continue;
+        hcs[q].resize(csz[q]);
+        CK(hipMemcpyAsync(hcs[q].data(), h->cs[q], csz[q] * 8, hipMemcpyDeviceToHost, h->s));
+        csp[q] = hcs[q].data();
+    }
+    CK(hipStreamSynchronize(h->s));
+    h->cs_sum = boundary_flux(h, csp, 1.0);
     return PLES_MG_OK;
 }
 
@@ -2022,7 +2058,7 @@ int ples_mg_solve(ples_mg_handle* h, double* fi, int deepl, int deepr, const dou
     CKS(upload_cs(h, cs));
     CKS(upload_fi(h, fi, deepl));
     CKS(pack_box<double>(h, G, 1, G.nx, 1, G.ny, 1, G.nz, rhs, h->bvec, h->W[1]));
-    CKS(project_rhs(h, boundary_flux(h, cs, dt)));
+    CKS(project_rhs(h, h->cs_sum / dt));
     const int st = solve_core(h, eps, ficycle, res);
     if (st != PLES_MG_OK && st != PLES_MG_NOT_CONVERGED && st != PLES_MG_ERR_NONFINITE) return st;
     CKS(download_fi(h, fi, deepl));
@@ -2114,6 +2150,120 @@ int ples_mg_level_metrics(ples_mg_handle* h, int level, double* const g[9]) {
     for (int q = 0; q < 9; ++q) {
         dfree(h, gg[q]);
     }
+    return st;
+}
+
+// host double <-> device level array of element type T (ghosted level layout)
+static int level_upload(H* h, void* dst, bool f32, const double* src, long n) {
+    if (!f32) {
+        CK(hipMemcpyAsync(dst, src, (size_t)n * 8, hipMemcpyHostToDevice, h->s));
+        CK(hipStreamSynchronize(h->s));
+        return PLES_MG_OK;
+    }
+    std::vector<float> t((size_t)n);
+    for (long q = 0; q < n; ++q) t[(size_t)q] = (float)src[q];
+    CK(hipMemcpyAsync(dst, t.data(), (size_t)n * 4, hipMemcpyHostToDevice, h->s));
+    CK(hipStreamSynchronize(h->s));
+    return PLES_MG_OK;
+}
+
+static int level_download(H* h, double* dst, bool f32, const void* src, long n) {
+    if (!f32) {
+        CK(hipMemcpyAsync(dst, src, (size_t)n * 8, hipMemcpyDeviceToHost, h->s));
+        CK(hipStreamSynchronize(h->s));
+        return PLES_MG_OK;
+    }
+    std::vector<float> t((size_t)n);
+    CK(hipMemcpyAsync(t.data(), src, (size_t)n * 4, hipMemcpyDeviceToHost, h->s));
+    CK(hipStreamSynchronize(h->s));
+    for (long q = 0; q < n; ++q) dst[q] = (double)t[(size_t)q];
+    return PLES_MG_OK;
+}
+
+int ples_mg_level_op(ples_mg_handle* h, int level, int op, int P, double omega, const double* in0,
+                     const double* in1, double* out) {
+    if (!h || !in0 || !out || level < 1 || level > h->L) return PLES_MG_ERR_ARG;
+    const int n = level - 1;
+    Level& L = h->lv[n];
+    const bool coarse_op = op == PLES_MG_OP_RESTRICT || op == PLES_MG_OP_RESID_RESTRICT || op == PLES_MG_OP_PROLONG;
+    if (coarse_op && n == h->L - 1) return PLES_MG_ERR_ARG;
+    if ((op == PLES_MG_OP_SMOOTH || op == PLES_MG_OP_RESID || op == PLES_MG_OP_RESID_RESTRICT ||
+         op == PLES_MG_OP_PROLONG) && !in1)
+        return PLES_MG_ERR_ARG;
+    if (op == PLES_MG_OP_RESID && !L.r) return PLES_MG_ERR_ARG;
+    if (op == PLES_MG_OP_SMOOTH && (P < 0 || (P > 0 && !seg_ok(L.G.ny, P)) || !(omega > 0.0 && omega < 2.0)))
+        return PLES_MG_ERR_ARG;
+    if (op < PLES_MG_OP_SMOOTH || op > PLES_MG_OP_PROLONG) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    CK(hipStreamSynchronize(h->s));
+    const long nf = L.G.n;
+    // level-0 rhs of the smoother and residual: the FP64 scratch vector rvec
+    // (the V-cycle reads the fine rhs as r0); coarser levels keep theirs in b
+    const double* r0 = h->rvec;
+    auto upload_rhs = [&]() -> int {
+        if (n == 0) return level_upload(h, h->rvec, false, in1, nf);
+        return level_upload(h, L.b, L.f32, in1, nf);
+    };
+    const float* rhs32 = h->rhs32;
+    h->rhs32 = nullptr;
+    int st = PLES_MG_OK;
+    switch (op) {
+        case PLES_MG_OP_SMOOTH: {
+            CKS(level_upload(h, L.p[L.cur], L.f32, in0, nf));
+            CKS(upload_rhs());
+            const int P0 = L.P;
+            const double om0 = h->o.omega;
+            if (P > 0 && P != L.P) {
+                if (P > 1 && !L.lv_v) CKS(dalloc(h, (void**)&L.lv_v, (size_t)nf * 4));
+                if (P > 1 && !L.lv_w) CKS(dalloc(h, (void**)&L.lv_w, (size_t)nf * 4));
+                L.P = P;
+                CKS(level_factors(h, L));
+            }
+            h->o.omega = omega;  // one sweep: the constant damping
+            st = lv_smooth(h, n, r0, 1, false);
+            h->o.omega = om0;
+            if (st == PLES_MG_OK) st = level_download(h, out, L.f32, L.p[L.cur], nf);
+            if (L.P != P0) {
+                L.P = P0;
+                CKS(level_factors(h, L));
+            }
+            break;
+        }
+        case PLES_MG_OP_RESID:
+            CKS(level_upload(h, L.p[L.cur], L.f32, in0, nf));
+            CKS(upload_rhs());
+            st = lv_resid(h, n, r0);
+            if (st == PLES_MG_OK) st = level_download(h, out, L.f32, L.r, nf);
+            break;
+        case PLES_MG_OP_RESTRICT: {
+            Level& C = h->lv[n + 1];
+            CKS(level_upload(h, L.r, L.f32, in0, nf));
+            CK(hipMemsetAsync(C.b, 0, (size_t)C.G.n * (C.f32 ? 4 : 8), h->s));
+            st = lv_restrict(h, n);
+            if (st == PLES_MG_OK) st = level_download(h, out, C.f32, C.b, C.G.n);
+            break;
+        }
+        case PLES_MG_OP_RESID_RESTRICT: {
+            Level& C = h->lv[n + 1];
+            CKS(level_upload(h, L.p[L.cur], L.f32, in0, nf));
+            CKS(upload_rhs());
+            CK(hipMemsetAsync(C.b, 0, (size_t)C.G.n * (C.f32 ? 4 : 8), h->s));
+            st = lv_resid_restrict(h, n, r0);
+            if (st == PLES_MG_OK) st = level_download(h, out, C.f32, C.b, C.G.n);
+            break;
+        }
+        case PLES_MG_OP_PROLONG: {
+            Level& C = h->lv[n + 1];
+            CKS(level_upload(h, C.p[C.cur], C.f32, in0, C.G.n));
+            CKS(level_upload(h, L.p[L.cur], L.f32, in1, nf));
+            st = lv_prolong(h, n);
+            if (st == PLES_MG_OK) st = level_download(h, out, L.f32, L.p[L.cur], nf);
+            break;
+        }
+        default: st = PLES_MG_ERR_ARG;
+    }
+    h->rhs32 = rhs32;
+    CK(hipStreamSynchronize(h->s));
     return st;
 }
 
@@ -2254,7 +2404,23 @@ int ples_mg_perturb_fi(ples_mg_handle* h, uint64_t seed, double rel) {
 int ples_mg_solve_resident(ples_mg_handle* h, double eps, int maxcycles, ples_mg_result* res) {
     if (!h || !(eps > 0.0) || maxcycles < 0) return PLES_MG_ERR_ARG;
     CK(hipSetDevice(h->dev));
+    // the FP64 compatibility projection of the resident rhs, every solve
+    if (h->o.project_rhs) CKS(project_rhs(h, h->cs_sum / h->dt));
     return solve_core(h, eps, maxcycles, res);
+}
+
+int ples_mg_project_rhs(ples_mg_handle* h, double* rhs, const double* cs1, const double* cs2, const double* cs3,
+                        const double* cs4, const double* cs5, const double* cs6, double dt, double* removed) {
+    if (!h || !rhs || !(dt > 0.0)) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(h->dev));
+    const Geo& G = h->lv[0].G;
+    const double* cs[6] = {cs1, cs2, cs3, cs4, cs5, cs6};
+    h->dt = dt;
+    CKS(upload_cs(h, cs));
+    CKS(pack_box<double>(h, G, 1, G.nx, 1, G.ny, 1, G.nz, rhs, h->bvec, h->W[1]));
+    CKS(project_rhs(h, h->cs_sum / dt, true));
+    if (removed) *removed = h->last_removed;
+    return unpack_box<double>(h, G, 1, G.nx, 1, G.ny, 1, G.nz, h->bvec, rhs, h->W[1]);
 }
 
 int ples_mg_get_fi(ples_mg_handle* h, double* fi) {
@@ -2491,6 +2657,7 @@ int ples_mg_contra(ples_mg_handle* h, const double* u, const double* v, const do
     const double* fcin[3] = {uc, vc, wc};
     CKS(upload_faces(h, h->pj_fc, fcin));
     CKS(launch_contra(h, potenziale));
+    CKS(cs_sum_device(h));
     double* fcout[3] = {uc, vc, wc};
     CKS(download_faces(h, fcout, h->pj_fc));
     return download_cs(h, cs);
@@ -2564,22 +2731,9 @@ int ples_mg_projection_step(ples_mg_handle* h, double* u, double* v, double* w, 
         CK(hipGetLastError());
     }
     // boundary flux of the compatibility projection from the new cs
-    std::vector<double> hcs[6];
-    const double* csp[6] = {};
-    {
-        const size_t csz[6] = {(size_t)h->jy * h->jz, (size_t)h->jy * h->jz, (size_t)h->jx * h->jz,
-                               (size_t)h->jx * h->jz, (size_t)h->jx * h->jy, (size_t)h->jx * h->jy};
-        const int wall[3] = {h->ip, h->jp, h->kp};
-        for (int q = 0; q < 6; ++q) {
-            if (!wall[q / 2]) continue;
-            hcs[q].resize(csz[q]);
-            CK(hipMemcpyAsync(hcs[q].data(), h->cs[q], csz[q] * 8, hipMemcpyDeviceToHost, h->s));
-            csp[q] = hcs[q].data();
-        }
-        CK(hipStreamSynchronize(h->s));
-    }
+    CKS(cs_sum_device(h));
     h->dt = dt;
-    CKS(project_rhs(h, boundary_flux(h, csp, dt)));
+    CKS(project_rhs(h, h->cs_sum / dt));
     // multi (fsrc/stratiParticle.f03:2881-2887)
     const int st = solve_core(h, eps, ficycle, res);
     if (st != PLES_MG_OK && st != PLES_MG_NOT_CONVERGED) return st;
@@ -2814,12 +2968,15 @@ int ples_mg_batch_solve(ples_mg_batch* b, double eps, int maxcycles, int steps_p
     for (int i = 0; i < n; ++i) {
         H* h = b->hs[i];
         CK(hipStreamWaitEvent(h->s, t0, 0));
+        // the FP64 compatibility projection of every rhs, as ples_mg_solve_resident
+        if (h->o.project_rhs) CKS(project_rhs_async(h, h->cs_sum / h->dt, true));
         CKS(outer_residual(h, h->x, h->rvec));
         CK(hipMemcpyAsync(h->Sh, h->S, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, h->s));
     }
     for (int i = 0; i < n; ++i) {
         H* h = b->hs[i];
         CK(hipStreamSynchronize(h->s));
+        if (h->o.project_rhs) h->last_removed = removed_mean(h, h->cs_sum / h->dt, true);
         const bool absmode = h->o.stop_mode == PLES_MG_STOP_ABS_MAX;
         bnorm[i] = std::sqrt(h->Sh[S_BB]) > 0 ? std::sqrt(h->Sh[S_BB]) : 1.0;
         thr[i] = absmode ? eps : (eps * bnorm[i]) * (eps * bnorm[i]);
@@ -3429,13 +3586,17 @@ int ples_mg_server_stats(const ples_mg_server* sv, size_t* used_bytes, int* resi
 // Copy bandwidth of this device: a 16-byte-per-lane copy of `bytes` bytes,
 // reps launches timed with HIP events.  The measured streaming ceiling that
 // the stencil kernels' achieved bandwidth is read against.
-int ples_mg_copy_bandwidth(int device, size_t bytes, int reps, double* gbs) {
+// copy (read + write bytes) or read-only probe of `bytes` bytes, reps launches
+static int probe_bandwidth(int device, size_t bytes, int reps, bool read_only, double* gbs) {
     if (!gbs || bytes < 4096 || reps < 1) return PLES_MG_ERR_ARG;
     CK(hipSetDevice(device));
     const size_t n4 = bytes / 16;
+    const int nb = 8192, bt = 256;
     float4 *a = nullptr, *b = nullptr;
+    float* sink = nullptr;
     CK(hipMalloc((void**)&a, n4 * 16));
-    if (hipMalloc((void**)&b, n4 * 16) != hipSuccess) {
+    const size_t second = read_only ? (size_t)nb * bt * sizeof(float) : n4 * 16;
+    if (hipMalloc(read_only ? (void**)&sink : (void**)&b, second) != hipSuccess) {
         (void)hipFree(a);
         return PLES_MG_ERR_OOM;
     }
@@ -3447,22 +3608,36 @@ int ples_mg_copy_bandwidth(int device, size_t bytes, int reps, double* gbs) {
         st = PLES_MG_ERR_HIP;
     if (st == PLES_MG_OK) {
         (void)hipMemsetAsync(a, 0, n4 * 16, s);
-        const int nb = 32768;
-        hipLaunchKernelGGL(k_copy16, dim3(nb), dim3(256), 0, s, (long)n4, (const float4*)a, b);
+        auto launch = [&]() {
+            if (read_only)
+                hipLaunchKernelGGL(k_read16, dim3(nb), dim3(bt), 0, s, (long)n4, (const float4*)a, sink);
+            else
+                hipLaunchKernelGGL(k_copy16, dim3(nb), dim3(bt), 0, s, (long)n4, (const float4*)a, b);
+        };
+        launch();
         (void)hipEventRecord(e0, s);
-        for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy16, dim3(nb), dim3(256), 0, s, (long)n4, (const float4*)a, b);
+        for (int r = 0; r < reps; ++r) launch();
         (void)hipEventRecord(e1, s);
         if (hipEventSynchronize(e1) != hipSuccess) st = PLES_MG_ERR_HIP;
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        *gbs = 2.0 * (double)(n4 * 16) * reps / (ms * 1e-3) / 1e9;
+        *gbs = (read_only ? 1.0 : 2.0) * (double)(n4 * 16) * reps / (ms * 1e-3) / 1e9;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         (void)hipStreamDestroy(s);
     }
     (void)hipFree(a);
-    (void)hipFree(b);
+    if (b) (void)hipFree(b);
+    if (sink) (void)hipFree(sink);
     return st;
+}
+
+int ples_mg_copy_bandwidth(int device, size_t bytes, int reps, double* gbs) {
+    return probe_bandwidth(device, bytes, reps, false, gbs);
+}
+
+int ples_mg_read_bandwidth(int device, size_t bytes, int reps, double* gbs) {
+    return probe_bandwidth(device, bytes, reps, true, gbs);
 }
 
 
@@ -3491,15 +3666,21 @@ int ples_mg_tuning_load(const char* path) {
             continue;
         }
         if ((v.use16 != 0 && v.use16 != 1) || (k[7] != 0 && k[7] != 1) || (v.use16 && !k[7])) continue;
-        const bool okP = valid_segments(v.P) && k[1] % v.P == 0;
-        const bool okR = valid_segments(v.rP) && k[1] % v.rP == 0 && v.rbx >= 1 && (v.rbx & (v.rbx - 1)) == 0 &&
-                         v.rbx <= 256 && v.rbx * v.rP <= 512;
+        const bool okP = seg_ok(k[1], v.P);
+        const bool okR = seg_ok(k[1], v.rP) && v.rbx >= 1 && (v.rbx & (v.rbx - 1)) == 0 && v.rbx <= 256 &&
+                         v.rbx * v.rP <= 512;
         if (!okP || !okR) continue;
         g_tune[k] = v;
         ++n;
     }
     std::fclose(f);
     return n > 0 ? PLES_MG_OK : PLES_MG_ERR_ARG;
+}
+
+int ples_mg_tuning_clear(void) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune.clear();
+    return PLES_MG_OK;
 }
 
 int ples_mg_tuning_save(const char* path) {

@@ -36,6 +36,8 @@ MIXED = 1
 STOP_REL_L2 = 0
 STOP_ABS_MAX = 1
 
+OP_SMOOTH, OP_RESID, OP_RESTRICT, OP_RESID_RESTRICT, OP_PROLONG = range(5)
+
 K_SMOOTH, K_SMOOTH_FINE, K_RESID, K_RESID_FINE, K_OUTER_APPLY, K_OUTER_UPDATE, K_TRANSFER, K_HALO = range(8)
 KERNEL_FAMILIES = {
     "smooth": K_SMOOTH, "smooth_fine": K_SMOOTH_FINE, "resid": K_RESID, "resid_fine": K_RESID_FINE,
@@ -133,10 +135,14 @@ def lib():
         L.ples_mg_batch_solve.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.POINTER(_Result)]
         L.ples_mg_tiles.argtypes = [C.c_void_p, C.c_int] + [C.POINTER(C.c_int)] * 3
         L.ples_mg_copy_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
+        L.ples_mg_read_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
         L.ples_mg_tuning_load.argtypes = [C.c_char_p]
         L.ples_mg_restart_write.argtypes = [C.c_char_p] + [C.c_int] * 4 + [C.c_double] * 3 + [_dp] * 8
         L.ples_mg_restart_read.argtypes = [C.c_char_p] + [C.c_int] * 4 + [_dp] * 3 + [_dp] * 8
         L.ples_mg_tuning_save.argtypes = [C.c_char_p]
+        L.ples_mg_tuning_clear.argtypes = []
+        L.ples_mg_project_rhs.argtypes = [C.c_void_p, _dp] + [_dp] * 6 + [C.c_double, _dp]
+        L.ples_mg_level_op.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp]
         L.ples_mg_server_default_opts.argtypes = [C.POINTER(_ServerOpts)]
         L.ples_mg_server_create.argtypes = [C.POINTER(_ServerOpts), C.POINTER(_Opts), C.POINTER(C.c_void_p)]
         L.ples_mg_server_destroy.argtypes = [C.c_void_p]
@@ -182,6 +188,15 @@ def _p(a):
     return a.ctypes.data_as(_dp)
 
 
+def _need(a, shape, what):
+    """Raise unless a is a float64 array of exactly this shape: the C ABI
+    reads and writes caller memory by the reference shapes."""
+    if a is None:
+        return
+    if tuple(np.shape(a)) != tuple(shape):
+        raise ValueError(f"{what}: shape {tuple(np.shape(a))}, expected {tuple(shape)}")
+
+
 def _check(st: int, what: str):
     if st != OK:
         raise PlesMGError(st, what)
@@ -217,6 +232,13 @@ def copy_bandwidth(nbytes=1 << 30, reps=20, device=0) -> float:
     """Measured device copy bandwidth in GB/s (read + write)."""
     g = C.c_double()
     _check(lib().ples_mg_copy_bandwidth(device, nbytes, reps, C.byref(g)), "ples_mg_copy_bandwidth")
+    return g.value
+
+
+def read_bandwidth(nbytes=1 << 30, reps=20, device=0) -> float:
+    """Measured device read bandwidth in GB/s (read-only sweep)."""
+    g = C.c_double()
+    _check(lib().ples_mg_read_bandwidth(device, nbytes, reps, C.byref(g)), "ples_mg_read_bandwidth")
     return g.value
 
 
@@ -257,6 +279,11 @@ def tuning_load(path=TUNING_TABLE) -> bool:
     return lib().ples_mg_tuning_load(path.encode()) == OK
 
 
+def tuning_clear():
+    """Empty the process-wide launch-shape table (fixed rule from now on)."""
+    _check(lib().ples_mg_tuning_clear(), "ples_mg_tuning_clear")
+
+
 def tuning_save(path):
     _check(lib().ples_mg_tuning_save(path.encode()), "ples_mg_tuning_save")
 
@@ -287,6 +314,11 @@ class Solver:
         self.jz, self.jy, self.jx = (s - 1 for s in self._x.shape)
         self.ip, self.jp, self.kp, self.nlevel = ip, jp, kp, nlevel
         self._metrics = None if metrics is None else [np.ascontiguousarray(m, np.float64) for m in metrics]
+        if self._metrics is not None:
+            if len(self._metrics) != 9:
+                raise ValueError("metrics: nine arrays g11..g33")
+            for q, m in enumerate(self._metrics):
+                _need(m, self.face_shapes()[q // 3], f"metrics[{q}]")
         g = _grid(self._x, self._y, self._z, ip, jp, kp, nlevel, self._metrics)
         o = _opts(precision, **opts)
         h = C.c_void_p()
@@ -325,7 +357,12 @@ class Solver:
     def _cs(self, cs):
         if cs is None:
             return [None] * 6
-        return [None if c is None else np.ascontiguousarray(c, np.float64) for c in cs]
+        if len(cs) != 6:
+            raise ValueError("cs: six arrays cs1..cs6")
+        out = [None if c is None else np.ascontiguousarray(c, np.float64) for c in cs]
+        for q, c in enumerate(out):
+            _need(c, self.cs_shapes()[q], f"cs{q + 1}")
+        return out
 
     # ---- drop-in
     def multi(self, fi, rhs, cs=None, dt=1.0, eps=1e-8, ficycle=1000, deepl=1, deepr=1) -> Result:
@@ -343,6 +380,8 @@ class Solver:
         return Result(r.status, r.cycles, r.rel_res, r.abs_res, r.rhs_mean_removed, r.seconds)
 
     def residual(self, fi, rhs, cs=None, dt=1.0):
+        _need(fi, self.fi_shape, "fi")
+        _need(rhs, self.rhs_shape, "rhs")
         cs = self._cs(cs)
         out = np.zeros(self.rhs_shape)
         _check(lib().ples_mg_residual(self._h, _p(np.ascontiguousarray(fi)), 1, 1,
@@ -376,13 +415,48 @@ class Solver:
                "ples_mg_level_metrics")
         return g
 
+    def project_rhs(self, rhs, cs=None, dt=1.0):
+        """Zero-mean compatibility projection in place; returns the value removed."""
+        _need(rhs, self.rhs_shape, "rhs")
+        if not (rhs.flags["C_CONTIGUOUS"] and rhs.dtype == np.float64):
+            raise TypeError("rhs: C-contiguous float64, projected in place")
+        cs = self._cs(cs)
+        rem = C.c_double()
+        _check(lib().ples_mg_project_rhs(self._h, _p(rhs), *[_p(c) for c in cs], dt, C.byref(rem)),
+               "ples_mg_project_rhs")
+        return rem.value
+
+    def level_op(self, level, op, in0, in1=None, P=0, omega=1.0):
+        """One kernel-level operation of level `level` (1-based) on ghosted
+        level arrays (see ples_mg_level_op); returns the output array."""
+        nx, ny, nz = self.level_shape(level)
+        sh = (nz + 2, ny + 2, nx + 2)
+        if op in (OP_RESTRICT, OP_RESID_RESTRICT, OP_PROLONG):
+            cx, cy, cz = self.level_shape(level + 1)
+            csh = (cz + 2, cy + 2, cx + 2)
+        else:
+            csh = sh
+        in_sh = csh if op == OP_PROLONG else sh
+        out_sh = csh if op in (OP_RESTRICT, OP_RESID_RESTRICT) else sh
+        _need(in0, in_sh, "in0")
+        if in1 is not None:
+            _need(in1, sh, "in1")
+        a0 = np.ascontiguousarray(in0, np.float64)
+        a1 = None if in1 is None else np.ascontiguousarray(in1, np.float64)
+        out = np.zeros(out_sh)
+        _check(lib().ples_mg_level_op(self._h, level, op, P, omega, _p(a0), _p(a1), _p(out)), "ples_mg_level_op")
+        return out
+
     def diver(self, uc, vc, wc, dt=1.0):
+        for a, s, nm in zip((uc, vc, wc), self.face_shapes(), ("uc", "vc", "wc")):
+            _need(a, s, nm)
         out = np.zeros(self.rhs_shape)
         _check(lib().ples_mg_diver(self._h, _p(np.ascontiguousarray(uc)), _p(np.ascontiguousarray(vc)),
                                    _p(np.ascontiguousarray(wc)), dt, _p(out)), "ples_mg_diver")
         return out
 
     def gradie(self, fi, dt=1.0):
+        _need(fi, self.fi_shape, "fi")
         jx, jy, jz = self.jx, self.jy, self.jz
         outs = [np.zeros((jz, jy, jx)) for _ in range(3)] + \
                [np.zeros((jz, jy, jx + 1)), np.zeros((jz, jy + 1, jx)), np.zeros((jz + 1, jy, jx))]
@@ -398,16 +472,29 @@ class Solver:
     def _cs_out(self):
         return [np.zeros(s) for s in self.cs_shapes()]
 
+    def _check_cells_faces(self, vel, fc):
+        for a in vel:
+            _need(a, self.fi_shape, "vel")
+        for a, s in zip(fc, self.face_shapes()):
+            _need(a, s, "fc")
+            if not (a.flags["C_CONTIGUOUS"] and a.dtype == np.float64):
+                raise TypeError("fc: C-contiguous float64 arrays, updated in place")
+
     def update(self, vel, dvel):
         """In place on vel = [u, v, w] (fi_shape); dvel shaped fi_shape too."""
         for a in vel:
-            assert a.shape == self.fi_shape and a.flags["C_CONTIGUOUS"] and a.dtype == np.float64
+            _need(a, self.fi_shape, "vel")
+            if not (a.flags["C_CONTIGUOUS"] and a.dtype == np.float64):
+                raise TypeError("vel: C-contiguous float64 arrays, updated in place")
+        for a in dvel:
+            _need(a, self.fi_shape, "dvel")
         dv = [np.ascontiguousarray(a, np.float64) for a in dvel]
         _check(lib().ples_mg_update(self._h, *[_p(a) for a in vel], 1, 1, *[_p(a) for a in dv]),
                "ples_mg_update")
 
     def contra(self, vel, fc, potenziale=0):
         """In place on fc = [uc, vc, wc]; returns cs1..cs6 (zeros for periodic directions)."""
+        self._check_cells_faces(vel, fc)
         cs = self._cs_out()
         _check(lib().ples_mg_contra(self._h, *[_p(np.ascontiguousarray(a)) for a in vel], 1, 1,
                                     *[_p(a) for a in fc], (_dp * 6)(*[_p(a) for a in cs]), potenziale),
@@ -415,6 +502,11 @@ class Solver:
         return cs
 
     def vel_up(self, vel, fc, gra, cgra, dt=1.0, potenziale=0):
+        self._check_cells_faces(vel, fc)
+        for a in gra:
+            _need(a, self.rhs_shape, "gra")
+        for a, s in zip(cgra, self.face_shapes()):
+            _need(a, s, "cgra")
         gra = [np.ascontiguousarray(a, np.float64) for a in gra]
         cgra = [np.ascontiguousarray(a, np.float64) for a in cgra]
         _check(lib().ples_mg_vel_up(self._h, *[_p(a) for a in vel], 1, 1, *[_p(a) for a in fc],
@@ -422,6 +514,8 @@ class Solver:
                                     dt, potenziale), "ples_mg_vel_up")
 
     def check_divergence(self, fc):
+        for a, s in zip(fc, self.face_shapes()):
+            _need(a, s, "fc")
         out = np.zeros(4)
         _check(lib().ples_mg_check_divergence(self._h, *[_p(np.ascontiguousarray(a)) for a in fc], _p(out)),
                "ples_mg_check_divergence")
@@ -430,6 +524,10 @@ class Solver:
     def projection_step(self, vel, dvel, fc, fi, dt=1.0, eps=1e-10, ficycle=200, potenziale=0):
         """update -> contra -> diver -> multi -> gradie -> vel_up -> check_divergence,
         in place on vel, fc, fi.  Returns (Result, cs1..cs6, div maxima)."""
+        self._check_cells_faces(vel, fc)
+        for a in dvel:
+            _need(a, self.fi_shape, "dvel")
+        _need(fi, self.fi_shape, "fi")
         cs = self._cs_out()
         dv = [np.ascontiguousarray(a, np.float64) for a in dvel]
         div = np.zeros(4)
@@ -464,6 +562,7 @@ class Solver:
         return a
 
     def set_fi(self, fi):
+        _need(fi, self.fi_shape, "fi")
         _check(lib().ples_mg_set_fi(self._h, _p(np.ascontiguousarray(fi, np.float64))), "ples_mg_set_fi")
 
     def get_rhs(self):
@@ -472,6 +571,7 @@ class Solver:
         return a
 
     def set_rhs(self, rhs):
+        _need(rhs, self.rhs_shape, "rhs")
         _check(lib().ples_mg_set_rhs(self._h, _p(np.ascontiguousarray(rhs, np.float64))), "ples_mg_set_rhs")
 
     @property

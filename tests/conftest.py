@@ -24,3 +24,15 @@ def pytest_collection_modifyitems(config, items):
     for it in items:
         if "gpu" in it.keywords:
             it.add_marker(skip)
+
+
+@pytest.fixture(autouse=True)
+def _fresh_tuning_table(request):
+    """Every GPU test starts from an empty launch-shape table (the fixed
+    rule), whatever earlier tests loaded; tests that want the committed
+    table load it themselves."""
+    if "gpu" in request.keywords and _have_gpu():
+        from ples_amd import mg
+
+        mg.tuning_clear()
+    yield

@@ -53,3 +53,15 @@ def test_tuning_table_parses():
         assert use16 in (0, 1) and half in (0, 1) and use16 <= half and half <= f32
         n += 1
     assert n > 20
+
+
+def test_tuning_loader_rejects_segments_below_four_rows(tmp_path):
+    """A table entry whose smoother or residual segments would be shorter
+    than the four rows the autotuner's candidates keep is not loaded."""
+    bad = tmp_path / "bad.txt"
+    bad.write_text("32 16 32 0 0 1 1 1 16 8 32 1\n32 16 32 0 0 1 1 1 4 16 32 1\n")
+    assert not mg.tuning_load(str(bad))
+    good = tmp_path / "good.txt"
+    good.write_text("32 16 32 0 0 1 1 1 4 2 64 1\n")
+    assert mg.tuning_load(str(good))
+    mg.tuning_clear()

@@ -1,0 +1,246 @@
+"""The benchmark configurations at their real sizes on the GPU, checked
+against the oracle (the CPU restatement of the reference):
+
+  * c1, channel 64x32x64 (tanh 2.0, 4 levels): the pressure equals the exact
+    discrete solution of the channel operator (FFT in x, z, tridiagonal in
+    y) to 1e-9, both precisions;
+  * c2, terrain 256x128x256 (6 levels): the pressure solved to 1e-12 equals
+    the reference V-cycle's (the oracle's multi with the reference's line
+    SOR, L = 6) to 1e-9;
+  * c3, warped 512x256x512 (7 levels), warm start: the returned pressure's
+    residual under the oracle's operator and metrics meets 1.5 x 1e-10;
+  * c4, the 64-problem batch: every pressure's oracle-operator residual
+    meets 1.5 x 1e-8, and the batch gives bitwise the results of solving
+    alone (a subset with the largest problem);
+  * c5, a shortened real-time serving stream (100 Poisson arrivals at
+    200/s over 24 cases with 16 warm slots, so admission evicts): every
+    request meets its tolerance, and the requests of three cases (with the
+    largest) give bitwise the results of replaying each case alone with the
+    same cold/warm history.
+The committed launch-shape table is loaded, as the bench runs.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from ples_amd import cases as PC
+from ples_amd import mg
+from tests import cases
+from tests.test_oracle import _channel_exact
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid(cfg):
+    x, y, z = cfg.nodes()
+    return O.Grid(cfg.nx, cfg.ny, cfg.nz, 0, 1, 0, x, y, z)
+
+
+def _oracle_rel_res(g, fi, rhs_int, gm=None):
+    """||A p - b|| / ||b|| with the oracle's metrics and operator."""
+    if gm is None:
+        gm, _, _ = O.metrics(g, 0)
+    r = O.resid(g, 0, np.ascontiguousarray(fi), cases.ghosted(g, rhs_int), gm)
+    return float(np.linalg.norm(r[1:-1, 1:-1, 1:-1]) / np.linalg.norm(rhs_int))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_c1_channel_matches_exact_solution(precision):
+    cfg = PC.CONFIGS["c1"]
+    assert (cfg.nx, cfg.ny, cfg.nz, cfg.nlevel, cfg.amp) == (64, 32, 64, 4, 0.0)
+    mg.tuning_load()
+    x, y, z = cfg.nodes()
+    with mg.Solver(x, y, z, 0, 1, 0, cfg.nlevel, precision=precision) as s:
+        s.case_rhs(cfg.seed)
+        s.zero_fi()
+        r8 = s.solve_resident(cfg.tol, 400)
+        fi8 = s.get_fi()
+        r = s.solve_resident(1e-12, 400)
+        fi = s.get_fi()
+        rhs = s.get_rhs()
+    assert r8.ok and r8.rel_res < cfg.tol and r.ok
+    exact = _channel_exact(cfg.nx, cfg.ny, cfg.nz, cfg.gamma, rhs)
+    assert cases.rel_l2_meanfree(cases.interior(fi), exact) < 1e-9
+    # at the config's tolerance the pressure is already within 1e-5
+    assert cases.rel_l2_meanfree(cases.interior(fi8), exact) < 1e-5
+
+
+_C2 = {}
+
+
+def _c2_oracle():
+    """The reference V-cycle (L = 6, 5/10 sweeps per visit, 5 ghost passes)
+    with the reference's eta-line SOR (islor = 1, omega = 1: the shipped
+    omega = 1.6 stalls on this grid, and the point SOR contracts by only
+    0.8 per cycle) on the device-generated c2 rhs, to max|r| < 3e-14 max|b|
+    (an L2 relative residual near 1e-14: the pressure error of a solve is
+    about 240 times its relative residual on this grid, whose residual is
+    dominated by smooth modes; once per session)."""
+    if not _C2:
+        cfg = PC.CONFIGS["c2"]
+        g = _grid(cfg)
+        x, y, z = cfg.nodes()
+        with mg.Solver(x, y, z, 0, 1, 0, cfg.nlevel, precision="fp64") as s:
+            s.case_rhs(cfg.seed)
+            rhs = s.get_rhs()
+        H = O.Hier(g, cfg.nlevel)
+        fi = g.zeros()
+        eps = 3e-14 * np.abs(rhs).max()
+        n, rm, _ = H.multi(fi, cases.ghosted(g, rhs), eps=eps, ficycle=60, islor=1, omega=1.0)
+        H.close()
+        _C2.update(g=g, rhs=rhs, fi=fi, n=n, rm=rm, eps=eps)
+    return _C2
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_c2_pressure_matches_reference_vcycle(precision):
+    cfg = PC.CONFIGS["c2"]
+    ref = _c2_oracle()
+    assert ref["rm"] < ref["eps"], (ref["n"], ref["rm"])
+    mg.tuning_load()
+    x, y, z = cfg.nodes()
+    with mg.Solver(x, y, z, 0, 1, 0, cfg.nlevel, precision=precision) as s:
+        s.case_rhs(cfg.seed)
+        assert np.array_equal(s.get_rhs(), ref["rhs"])
+        s.zero_fi()
+        r = s.solve_resident(1e-12, 400)
+        fi = s.get_fi()
+    assert r.ok, r
+    err = cases.rel_l2_meanfree(fi, ref["fi"])
+    assert err < 1e-9, err
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_c3_warm_start_residual_under_the_oracle_operator(precision):
+    cfg = PC.CONFIGS["c3"]
+    mg.tuning_load()
+    x, y, z = cfg.nodes()
+    with mg.Solver(x, y, z, 0, 1, 0, cfg.nlevel, precision=precision) as s:
+        s.case_rhs(cfg.seed)
+        s.zero_fi()
+        assert s.solve_resident(cfg.tol, 400).ok       # p_prev
+        s.perturb_fi(1004, 1e-3)
+        r = s.solve_resident(cfg.tol, 400)
+        fi = s.get_fi()
+        rhs = s.get_rhs()
+    assert r.ok and r.rel_res < cfg.tol, r
+    g = O.Grid(cfg.nx, cfg.ny, cfg.nz, 0, 1, 0, x, y, z)
+    rel = _oracle_rel_res(g, fi, rhs)
+    assert rel < 1.5 * cfg.tol, rel
+
+
+def _alone(pr, tol):
+    with mg.Solver(pr["x"], pr["y"], pr["z"], 0, 1, 0, pr["nlevel"], precision="mixed", fp32_krylov=0) as s:
+        s.case_rhs(pr["seed"])
+        s.zero_fi()
+        r = s.solve_resident(tol, 200)
+        return r, s.get_fi()
+
+
+def test_c4_batch_residuals_and_batch_equals_alone():
+    probs = PC.c4_problems()
+    assert len(probs) == 64
+    tol = 1e-8
+    mg.tuning_load()
+    with mg.Batch(probs, precision="mixed") as b:
+        for p, s in zip(probs, b.solvers):
+            s.case_rhs(p["seed"])
+            s.zero_fi()
+        res = b.solve(tol, 200)
+        fis = [s.get_fi() for s in b.solvers]
+        rhss = [s.get_rhs() for s in b.solvers]
+    for q, (p, r) in enumerate(zip(probs, res)):
+        assert r.ok and r.rel_res < tol, (q, r)
+        nx, ny, nz = p["shape"]
+        g = O.Grid(nx, ny, nz, 0, 1, 0, p["x"], p["y"], p["z"])
+        rel = _oracle_rel_res(g, fis[q], rhss[q])
+        assert rel < 1.5 * tol, (q, p["shape"], rel)
+    sizes = [int(np.prod(p["shape"])) for p in probs]
+    subset = sorted({int(np.argmax(sizes)), int(np.argmin(sizes)), 7})
+    for q in subset:
+        r, fi = _alone(probs[q], tol)
+        assert r.cycles == res[q].cycles and r.rel_res == res[q].rel_res, q
+        assert np.array_equal(fi, fis[q]), q
+
+
+def _run_stream(cases_, reqs, max_active, warm_cases):
+    """Real-time arrivals into a server; returns {ticket order: (Result, warm)}
+    and the final pressure of every resident case."""
+    out = [None] * len(reqs)
+    with mg.Server(max_active=max_active, warm_cases=warm_cases, precision="mixed") as sv:
+        for c in cases_:
+            sv.add_case(c["case_id"], c["x"], c["y"], c["z"], c["nlevel"], c["rhs_seed"])
+        t0 = time.perf_counter()
+        nxt, open_t = 0, {}
+        while nxt < len(reqs) or open_t:
+            now = time.perf_counter() - t0
+            while nxt < len(reqs) and reqs[nxt][0] <= now:
+                _, c, eps, seed = reqs[nxt]
+                open_t[sv.submit(c, seed, 1e-3, eps, 200)] = nxt
+                nxt += 1
+            progressed = sv.step()
+            for tk in list(open_t):
+                r = sv.poll(tk)
+                if r is not None:
+                    out[open_t.pop(tk)] = r
+            if not progressed and open_t and sv.pending():
+                sv.wait()
+            elif not open_t and nxt < len(reqs):
+                time.sleep(max(0.0, min(1e-3, reqs[nxt][0] - (time.perf_counter() - t0))))
+        st = sv.stats()
+    return out, st
+
+
+def _replay_alone(case, reqs, flags):
+    """The case's requests in order on servers holding only it: a new server
+    wherever the stream started the request cold (after an eviction)."""
+    res, sv = [], None
+    try:
+        for (_, _, eps, seed), warm in zip(reqs, flags):
+            if not warm:
+                if sv is not None:
+                    sv.close()
+                sv = mg.Server(max_active=1, warm_cases=1, precision="mixed")
+                sv.add_case(case["case_id"], case["x"], case["y"], case["z"], case["nlevel"], case["rhs_seed"])
+            t = sv.submit(case["case_id"], seed, 1e-3, eps, 200)
+            sv.drain()
+            res.append(sv.poll(t))
+        fi = sv.get_fi(case["case_id"])
+    finally:
+        if sv is not None:
+            sv.close()
+    return res, fi
+
+
+def test_c5_stream_tolerances_eviction_and_alone_equals_mix():
+    mg.tuning_load()
+    n_cases = 24
+    cs = PC.c5_cases(n_cases)
+    reqs = PC.c5_requests(100, n_cases, 200.0)
+    assert max(int(np.prod(c["shape"])) for c in cs) <= 256 * 128 * 256
+    out, st = _run_stream(cs, reqs, max_active=16, warm_cases=16)
+    for (_, c, eps, _), (r, _) in zip(reqs, out):
+        assert r.ok and r.rel_res < eps, (c, r)
+    used = sorted({c for _, c, _, _ in reqs})
+    assert len(used) > 16 and st["evictions"] > 0, st
+    largest = max(used, key=lambda c: int(np.prod(cs[c]["shape"])))
+    picks = [largest] + [c for c in used if c != largest][:2]
+    for c in picks:
+        idx = [q for q, rq in enumerate(reqs) if rq[1] == c]
+        alone, _ = _replay_alone(cs[c], [reqs[q] for q in idx], [out[q][1] for q in idx])
+        for q, (ra, wa) in zip(idx, alone):
+            rm, wm = out[q]
+            assert wa == wm and ra.cycles == rm.cycles and ra.rel_res == rm.rel_res, (c, q, ra, rm)
+
+
+def test_fixed_rule_shapes_at_256x128x256():
+    """Without a table, the fine level of the 256x128x256 solve takes the
+    fixed rule's launch shapes: 4 smoother segments (32 rows) and the fused
+    residual on 8 segments x 64 lanes."""
+    cfg = PC.CONFIGS["c2"]
+    x, y, z = cfg.nodes()
+    for prec in ("mixed", "fp64"):
+        with mg.Solver(x, y, z, 0, 1, 0, cfg.nlevel, precision=prec, autotune=-1) as s:
+            assert s.tiles(1) == (4, 8, 64), (prec, s.tiles(1))

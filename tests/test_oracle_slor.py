@@ -152,3 +152,24 @@ def test_slor_vcycle_converges_to_the_sor_solution(shape, nlevel, flags):
     assert r0 < 1e-13 and r1 < 1e-13
     assert n1 < n0  # the line relaxation is the stronger smoother on stretched cells
     assert cases.rel_l2_meanfree(fi_slor, fi_sor) < 1e-10
+
+
+@pytest.mark.parametrize("islor", [0, 1])
+def test_k_slab_sweeps_reach_the_same_solution(islor):
+    """orc_hier_multi on k-slabs (the reference's MPI slab semantics, used by
+    the CPU baseline) converges to the global-domain solution for any slab
+    count, and one hierarchy serves solves of different slab counts."""
+    g = cases.grid(32, 16, 32, gamma=1.5, amp=0.2)
+    rhs = O.case_rhs(g, 3)
+    H = O.Hier(g, 3)
+    out = {}
+    for ns in (1, 4, 2, 8):
+        fi = g.zeros()
+        n, rm, _ = H.multi(fi, rhs, eps=1e-13, ficycle=3000, islor=islor, omega=1.0 if islor else 1.6, nslab=ns)
+        assert rm < 1e-13, (ns, rm)
+        out[ns] = fi
+    with pytest.raises(ValueError):
+        H.multi(g.zeros(), rhs, eps=1e-8, ficycle=3, nslab=16)   # 32 planes / (16 slabs * 4) < 1
+    H.close()
+    for ns in (2, 4, 8):
+        assert cases.rel_l2_meanfree(out[ns], out[1]) < 1e-9, ns
