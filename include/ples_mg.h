@@ -176,6 +176,22 @@ int ples_mg_project_rhs(ples_mg_handle* h, double* rhs, const double* cs1, const
                         const double* cs3, const double* cs4, const double* cs5, const double* cs6,
                         double dt, double* removed);
 
+/* ples_mg_project_rhs on device arrays, allocating nothing.  No counterpart
+ * in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  rhs: device array (1:jx,1:jy,1:jz),
+ * projected in place; cs1..cs6: device arrays in the shapes above, NULL for
+ * periodic directions (no flux); removed (may be NULL): a device double that
+ * receives the value taken off; stream: hipStream_t (NULL = the null stream).
+ * The work is enqueued on the stream and not waited for.  Scratch: a
+ * file-scope device array of 1032 doubles (8 KB) per GPU, held until the
+ * process ends and shared by every call on that GPU: calls that overlap on
+ * one GPU race.  At most 2^31 - 2^18 cells (and per cs array), else
+ * PLES_MG_ERR_ARG. */
+int ples_mg_project_rhs_noalloc(double* rhs, int jx, int jy, int jz, const double* cs1,
+                                const double* cs2, const double* cs3, const double* cs4,
+                                const double* cs5, const double* cs6, double dt, double* removed,
+                                void* stream);
+
 /* Level-1 metrics as metrica builds them (fsrc/metrica.f03:85-483), in the
  * reference shapes: g[9] (see above), face vectors csv[9] = csx,csy,csz
  * (0:jx,1:jy,1:jz), etx,ety,etz (1:jx,0:jy,1:jz), ztx,zty,ztz

@@ -3632,6 +3632,128 @@ static int probe_bandwidth(int device, size_t bytes, int reps, bool read_only, d
     return st;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------- no-allocation projection
+// The compatibility projection of a device-resident rhs for callers that
+// cannot allocate (graph capture, tight serving loops): the block partial
+// sums and the removed value live in this file-scope device array, one copy
+// per GPU, shared by every call on that GPU (calls that overlap on one GPU
+// race on it).  Host code never passes the array's name: it takes the
+// device address from hipGetSymbolAddress once per device.
+namespace {
+constexpr int kNaBlocks = 1024;  // reduction grid: one partial per block
+constexpr int kNaThreads = 256;
+// 32-bit cell indices in the kernels (a grid-stride index stays below 2^31)
+constexpr long kNaMaxCells = 0x7fffffffL - (long)kNaBlocks * kNaThreads;
+static __device__ __align__(256) double scratch[kNaBlocks + 8];
+
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_ptr;
+
+// device address of `scratch` on the current device (looked up once per device)
+int scratch_address(int dev, double** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) {
+        *out = it->second;
+        return PLES_MG_OK;
+    }
+    double* p = nullptr;
+    if (hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return PLES_MG_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = p;
+    *out = p;
+    return PLES_MG_OK;
+}
+
+struct NaIn {
+    const double* a[7];  // rhs, cs1..cs6 (NULL: absent)
+    int n[7];            // their lengths
+    double w[7];         // weights: 1 for the rhs, -/+ 1/dt for the inflow/outflow fluxes
+};
+
+// part[b] = sum over the arrays of w * (the elements block b visits), fixed order
+__global__ void __launch_bounds__(kNaThreads) k_na_sum(NaIn in, double* __restrict__ part) {
+    double s = 0.0;
+    const int stride = gridDim.x * blockDim.x;
+    for (int q = 0; q < 7; ++q) {
+        if (!in.a[q]) continue;
+        double t = 0.0;
+        for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < in.n[q]; c += stride) t += in.a[q][c];
+        s += in.w[q] * t;
+    }
+    __shared__ double red[kNaThreads / 64];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = 0.0;
+        for (int w = 0; w < kNaThreads / 64; ++w) b += red[w];
+        part[blockIdx.x] = b;
+    }
+}
+
+// mean = sum(part) / ncell into part[kNaBlocks] (and *removed when given)
+__global__ void __launch_bounds__(kNaThreads) k_na_fin(double* __restrict__ part, int nb, double inv_n,
+                                                       double* __restrict__ removed) {
+    double s = 0.0;
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) s += part[q];
+    __shared__ double red[kNaThreads / 64];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = 0.0;
+        for (int w = 0; w < kNaThreads / 64; ++w) b += red[w];
+        part[kNaBlocks] = b * inv_n;
+        if (removed) *removed = b * inv_n;
+    }
+}
+
+__global__ void __launch_bounds__(kNaThreads) k_na_sub(int n, double* __restrict__ a, const double* __restrict__ mean) {
+    const double m = mean[0];
+    const int stride = gridDim.x * blockDim.x;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) a[c] -= m;
+}
+}  // namespace
+
+extern "C" {
+
+int ples_mg_project_rhs_noalloc(double* rhs, int jx, int jy, int jz, const double* cs1, const double* cs2,
+                                const double* cs3, const double* cs4, const double* cs5, const double* cs6,
+                                double dt, double* removed, void* stream) {
+    if (!rhs || jx < 1 || jy < 1 || jz < 1 || !(dt > 0.0)) return PLES_MG_ERR_ARG;
+    const long n = (long)jx * jy * jz;
+    if (n > kNaMaxCells || (long)jy * jz > kNaMaxCells || (long)jx * jz > kNaMaxCells ||
+        (long)jx * jy > kNaMaxCells)
+        return PLES_MG_ERR_ARG;
+    int dev = 0;
+    CK(hipGetDevice(&dev));
+    double* part = nullptr;
+    CKS(scratch_address(dev, &part));
+    const hipStream_t s = (hipStream_t)stream;
+    NaIn in{};
+    const double* cs[6] = {cs1, cs2, cs3, cs4, cs5, cs6};
+    const int len[6] = {jy * jz, jy * jz, jx * jz, jx * jz, jx * jy, jx * jy};
+    in.a[0] = rhs;
+    in.n[0] = (int)n;
+    in.w[0] = 1.0;
+    for (int q = 0; q < 6; ++q) {  // sum(rhs) - sum(outflow - inflow) / dt
+        in.a[q + 1] = cs[q];
+        in.n[q + 1] = len[q];
+        in.w[q + 1] = (q & 1) ? -1.0 / dt : 1.0 / dt;
+    }
+    const int nb = (int)std::min<long>(kNaBlocks, (n + 8L * kNaThreads - 1) / (8L * kNaThreads));
+    hipLaunchKernelGGL(k_na_sum, dim3(nb), dim3(kNaThreads), 0, s, in, part);
+    hipLaunchKernelGGL(k_na_fin, dim3(1), dim3(kNaThreads), 0, s, part, nb, 1.0 / (double)n, removed);
+    hipLaunchKernelGGL(k_na_sub, dim3(nb), dim3(kNaThreads), 0, s, (int)n, rhs, (const double*)(part + kNaBlocks));
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
 int ples_mg_copy_bandwidth(int device, size_t bytes, int reps, double* gbs) {
     return probe_bandwidth(device, bytes, reps, false, gbs);
 }

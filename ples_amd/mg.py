@@ -142,6 +142,8 @@ def lib():
         L.ples_mg_tuning_save.argtypes = [C.c_char_p]
         L.ples_mg_tuning_clear.argtypes = []
         L.ples_mg_project_rhs.argtypes = [C.c_void_p, _dp] + [_dp] * 6 + [C.c_double, _dp]
+        L.ples_mg_project_rhs_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 6 + \
+            [C.c_double, C.c_void_p, C.c_void_p]
         L.ples_mg_level_op.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp]
         L.ples_mg_server_default_opts.argtypes = [C.POINTER(_ServerOpts)]
         L.ples_mg_server_create.argtypes = [C.POINTER(_ServerOpts), C.POINTER(_Opts), C.POINTER(C.c_void_p)]
@@ -233,6 +235,14 @@ def copy_bandwidth(nbytes=1 << 30, reps=20, device=0) -> float:
     g = C.c_double()
     _check(lib().ples_mg_copy_bandwidth(device, nbytes, reps, C.byref(g)), "ples_mg_copy_bandwidth")
     return g.value
+
+
+def project_rhs_noalloc(rhs_dev, jx, jy, jz, cs_dev=None, dt=1.0, removed_dev=None, stream=None):
+    """ples_mg_project_rhs_noalloc on device pointers (ints): the projection of
+    a device rhs with no device allocation (enqueued on `stream`)."""
+    cs = list(cs_dev) if cs_dev is not None else [None] * 6
+    _check(lib().ples_mg_project_rhs_noalloc(rhs_dev, jx, jy, jz, *cs, dt, removed_dev, stream),
+           "ples_mg_project_rhs_noalloc")
 
 
 def read_bandwidth(nbytes=1 << 30, reps=20, device=0) -> float:
