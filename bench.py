@@ -111,8 +111,10 @@ def traffic_per_launch():
 
 
 def bench_c4(steps):
-    """64 uneven problems in one batch (graph launches of the whole batch)
-    against the same 64 solved one after another on their own handles."""
+    """64 uneven problems in one batch (graph launches of the whole batch,
+    grouped: one launch per kernel over each shape family) against the same
+    batch with every problem on its own stream and against the 64 solved
+    one after another on their own handles."""
     import ctypes
 
     from ples_amd import cases as PC
@@ -140,8 +142,20 @@ def bench_c4(steps):
         hip.hipDeviceSynchronize()
         dt = (time.perf_counter() - t) / reps
         assert all(r.ok for r in res), res
-        out.update(batch_solves_per_s=len(probs) / dt, batch_ms=1e3 * dt,
+        out.update(batch_solves_per_s=len(probs) / dt, batch_ms=1e3 * dt, families=b.families(),
                    cycles_max=max(r.cycles for r in res), cycles_mean=float(np.mean([r.cycles for r in res])))
+        # the same batch with every problem's kernels on its own stream
+        b.set_grouped(False)
+        run()
+        hip.hipDeviceSynchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            res_s = run()
+        hip.hipDeviceSynchronize()
+        dt_s = (time.perf_counter() - t) / reps
+        assert all(r.ok for r in res_s), res_s
+        out.update(per_stream_solves_per_s=len(probs) / dt_s, per_stream_ms=1e3 * dt_s)
+        b.set_grouped(True)
         # the same problems one at a time (each handle's own graph-replayed loop)
         hip.hipDeviceSynchronize()
         t = time.perf_counter()

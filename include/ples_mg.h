@@ -327,10 +327,15 @@ int ples_mg_set_rhs(ples_mg_handle* h, const double* rhs);
  * the refinement loops of all of them together: one HIP graph holds
  * steps_per_launch refinement steps (V-cycle + FP64 update) of every problem,
  * forked onto the problems' streams so their kernels overlap on the device,
- * and one launch of it advances the whole batch.  Each problem stops on its
- * own test (device-side; later steps leave it unchanged), the host reads all
- * residuals back with one copy per launch.  There is no reference
- * counterpart: PLES solves one pressure field per time step. */
+ * and one launch of it advances the whole batch.  Grouped (the default):
+ * problems whose refinement steps issue the same launch sequence (same shape
+ * and options) form a family, and each kernel of the family's steps is ONE
+ * launch over all its problems (the problem is the grid's z index, its
+ * arguments come from a device table); families run side by side.  Each
+ * problem stops on its own test (device-side; a converged problem's blocks
+ * exit at once), the host reads all residuals back with one copy per launch.
+ * Results are bitwise those of solving each problem alone.  There is no
+ * reference counterpart: PLES solves one pressure field per time step. */
 typedef struct ples_mg_batch ples_mg_batch;
 int ples_mg_batch_create(int n, const ples_mg_grid* grids, const ples_mg_opts* opts,
                          ples_mg_batch** out);
@@ -345,6 +350,11 @@ size_t ples_mg_batch_device_bytes(const ples_mg_batch* b);
  * Returns PLES_MG_OK when all converged, else the first failing status. */
 int ples_mg_batch_solve(ples_mg_batch* b, double eps, int maxcycles, int steps_per_launch,
                         ples_mg_result* res);
+/* Grouped launches on (1, the default) or off (0: every problem's kernels
+ * launched on its own stream).  The number of families of the last grouped
+ * graph (0 before the first grouped solve) by ples_mg_batch_families. */
+int ples_mg_batch_set_grouped(ples_mg_batch* b, int on);
+int ples_mg_batch_families(const ples_mg_batch* b);
 
 /* ---- serving loop ----
  * A solve server for many cases (independent simulations, each a grid and a

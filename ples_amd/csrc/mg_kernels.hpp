@@ -458,9 +458,9 @@ __device__ inline void yline_column(const Geo& G, const Met<TM>& M, const LineF&
 // in a block's columns), ping-pong p0 -> p1 -> p0 ... with a block barrier
 // between sweeps.  The first sweep starts from zero.
 template <class T, class TM>
-__global__ void __launch_bounds__(1024) k_coarse(Geo G, Met<TM> M, LineF F, T* __restrict__ p0,
-                                                 T* __restrict__ p1, const T* __restrict__ b, T omega,
-                                                 T inv_pot, int nu) {
+__device__ __forceinline__ void d_coarse(Geo G, Met<TM> M, LineF F, T* __restrict__ p0,
+                                         T* __restrict__ p1, const T* __restrict__ b, T omega,
+                                         T inv_pot, int nu) {
     const int ncol = G.nx * G.nz;
     for (int s = 0; s < nu; ++s) {
         const T* pin = (s & 1) ? p1 : p0;
@@ -470,6 +470,13 @@ __global__ void __launch_bounds__(1024) k_coarse(Geo G, Met<TM> M, LineF F, T* _
         __syncthreads();
     }
 }
+template <class T, class TM>
+__global__ void __launch_bounds__(1024) k_coarse(Geo G, Met<TM> M, LineF F, T* __restrict__ p0,
+                                                 T* __restrict__ p1, const T* __restrict__ b, T omega,
+                                                 T inv_pot, int nu) {
+    d_coarse<T, TM>(G, M, F, p0, p1, b, omega, inv_pot, nu);
+}
+
 
 // Coarsest level held in LDS: the iterate pair, rhs, metrics and line
 // factors are staged into the workgroup's LDS once, the nu sweeps run on LDS
@@ -477,8 +484,8 @@ __global__ void __launch_bounds__(1024) k_coarse(Geo G, Met<TM> M, LineF F, T* _
 // the dynamic LDS: 12 double arrays of n = ghosted cells (p0, p1, b, g11..g33),
 // then 2 float arrays (invm, cpr).
 template <class T>
-__global__ void __launch_bounds__(256) k_coarse_lds(Geo G, Met<T> M, LineF F, T* __restrict__ pout_g,
-                                                    const T* __restrict__ b_g, T omega, T inv_pot, int nu) {
+__device__ __forceinline__ void d_coarse_lds(Geo G, Met<T> M, LineF F, T* __restrict__ pout_g,
+                                             const T* __restrict__ b_g, T omega, T inv_pot, int nu) {
     extern __shared__ double smem[];
     const long n = G.n;
     T* base = reinterpret_cast<T*>(smem);
@@ -509,6 +516,12 @@ __global__ void __launch_bounds__(256) k_coarse_lds(Geo G, Met<T> M, LineF F, T*
     const T* res = (nu & 1) ? p1 : p0;
     for (long t = threadIdx.x; t < n; t += blockDim.x) pout_g[t] = res[t];
 }
+template <class T>
+__global__ void __launch_bounds__(256) k_coarse_lds(Geo G, Met<T> M, LineF F, T* __restrict__ pout_g,
+                                                    const T* __restrict__ b_g, T omega, T inv_pot, int nu) {
+    d_coarse_lds<T>(G, M, F, pout_g, b_g, omega, inv_pot, nu);
+}
+
 
 // Segmented smoother with the segment held in registers (M rows per segment
 // known at compile time): forward pass fused with the 19-point residual,
@@ -533,12 +546,14 @@ struct YlineOtf {
 #ifndef PLES_YLINE_WPE_F16
 #define PLES_YLINE_WPE_F16 4
 #endif
+template <class T, class MV>
+struct YlineWpe {
+    static constexpr int value = sizeof(T) == 4 ? (MVHalf<MV>::value ? PLES_YLINE_WPE_F16 : PLES_YLINE_WPE_F32) : 1;
+};
 // MV: the metric view, Met<T>, Met<_Float16> or the packed MetPk (X0 levels).
 template <int P, int MR, class T, class MV, class TR, bool X0 = false>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
-    sizeof(T) == 4 ? (MVHalf<MV>::value ? PLES_YLINE_WPE_F16 : PLES_YLINE_WPE_F32) : 1)))
-k_yline(Geo G, MV M, LineF F, const T* __restrict__ pin, T* __restrict__ pout, const TR* __restrict__ b,
-        T omega, T inv_pot, int zero_in) {
+__device__ __forceinline__ void d_yline(Geo G, MV M, LineF F, const T* __restrict__ pin, T* __restrict__ pout,
+                                        const TR* __restrict__ b, T omega, T inv_pot, int zero_in) {
     constexpr bool PK = std::is_same<MV, MetPk>::value;
     constexpr int BX = SegBX<P>::value;
     constexpr bool OTF = YlineOtf<P, MR, T>::value;
@@ -682,13 +697,20 @@ k_yline(Geo G, MV M, LineF F, const T* __restrict__ pin, T* __restrict__ pout, c
     }
     yline_ghosts(G, M, pin, pout, i, k, q.j0 == 1, q.j1 == G.ny, e[0], pfin, zero_in);
 }
+template <int P, int MR, class T, class MV, class TR, bool X0 = false>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(YlineWpe<T, MV>::value)))
+k_yline(Geo G, MV M, LineF F, const T* __restrict__ pin, T* __restrict__ pout, const TR* __restrict__ b, T omega,
+        T inv_pot, int zero_in) {
+    d_yline<P, MR, T, MV, TR, X0>(G, M, F, pin, pout, b, omega, inv_pot, zero_in);
+}
+
 
 // Segmented smoother for segment lengths without a register variant: the
 // local solve is staged through pout (e, then q), same algorithm.
 template <int P, class T, class TM, class TR>
-__global__ void __launch_bounds__(512) k_yline_staged(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin,
-                                                      T* __restrict__ pout, const TR* __restrict__ b, T omega,
-                                                      T inv_pot, int zero_in) {
+__device__ __forceinline__ void d_yline_staged(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin,
+                                               T* __restrict__ pout, const TR* __restrict__ b, T omega,
+                                               T inv_pot, int zero_in) {
     constexpr int BX = SegBX<P>::value;
     __shared__ T red[8][P][BX];
     Seg q;
@@ -771,6 +793,13 @@ __global__ void __launch_bounds__(512) k_yline_staged(Geo G, Met<TM> M, LineF F,
     }
     yline_ghosts(G, M, pin, pout, i, k, q.j0 == 1, q.j1 == G.ny, pn1, pnn, zero_in);
 }
+template <int P, class T, class TM, class TR>
+__global__ void __launch_bounds__(512) k_yline_staged(Geo G, Met<TM> M, LineF F, const T* __restrict__ pin,
+                                                      T* __restrict__ pout, const TR* __restrict__ b, T omega,
+                                                      T inv_pot, int zero_in) {
+    d_yline_staged<P, T, TM, TR>(G, M, F, pin, pout, b, omega, inv_pot, zero_in);
+}
+
 
 // ------------------------------------------------------------------------
 // Residual family (segmented columns).
@@ -779,11 +808,11 @@ __global__ void __launch_bounds__(512) k_yline_staged(Geo G, Met<TM> M, LineF F,
 //   MODE 2: w = A p, partial sums of w.w, w.wp, r.w, r.wp
 // ------------------------------------------------------------------------
 template <int MODE, class T, class TM, class TR, class TO, class TV = double, bool X0 = false>
-__global__ void __launch_bounds__(512) k_resid(Geo G, Met<TM> M, const T* __restrict__ p,
-                                               const TR* __restrict__ b, TO* __restrict__ out,
-                                               T inv_pot, const TV* __restrict__ rvec,
-                                               const TV* __restrict__ wp, double* partials,
-                                               int nblocks, int P) {
+__device__ __forceinline__ void d_resid(Geo G, Met<TM> M, const T* __restrict__ p,
+                                        const TR* __restrict__ b, TO* __restrict__ out,
+                                        T inv_pot, const TV* __restrict__ rvec,
+                                        const TV* __restrict__ wp, double* partials,
+                                        int nblocks, int P) {
     Seg q;
     const bool act = seg_of(G, P, q);
     double acc[4] = {0, 0, 0, 0};
@@ -834,6 +863,15 @@ __global__ void __launch_bounds__(512) k_resid(Geo G, Met<TM> M, const T* __rest
         block_reduce_store<4>(v, partials, nblocks, mx);
     }
 }
+template <int MODE, class T, class TM, class TR, class TO, class TV = double, bool X0 = false>
+__global__ void __launch_bounds__(512) k_resid(Geo G, Met<TM> M, const T* __restrict__ p,
+                                               const TR* __restrict__ b, TO* __restrict__ out,
+                                               T inv_pot, const TV* __restrict__ rvec,
+                                               const TV* __restrict__ wp, double* partials,
+                                               int nblocks, int P) {
+    d_resid<MODE, T, TM, TR, TO, TV, X0>(G, M, p, b, out, inv_pot, rvec, wp, partials, nblocks, P);
+}
+
 
 // V-cycle residual of a level restricted straight into the next coarser
 // level's rhs, without storing the fine residual: rc(I,J,K) = -(1/8) * the
@@ -846,9 +884,9 @@ __global__ void __launch_bounds__(512) k_resid(Geo G, Met<TM> M, const T* __rest
 // number of rows and start at an odd row.  Block: bx lanes along i x P
 // segments; XCD-aware block order.
 template <class T, class TM, class TR, class TC, bool X0 = false>
-__global__ void __launch_bounds__(512) k_resid_restrict(Geo G, Met<TM> M, const T* __restrict__ p,
-                                                        const TR* __restrict__ b, Geo Gc, TC* __restrict__ rc,
-                                                        T inv_pot, int P, int fx) {
+__device__ __forceinline__ void d_resid_restrict(Geo G, Met<TM> M, const T* __restrict__ p,
+                                                 const TR* __restrict__ b, Geo Gc, TC* __restrict__ rc,
+                                                 T inv_pot, int P, int fx) {
     int bxi, bk;
     xcd_tile(bxi, bk);
     const int bx = blockDim.x, tx = threadIdx.x, s = threadIdx.y;
@@ -890,6 +928,13 @@ __global__ void __launch_bounds__(512) k_resid_restrict(Geo G, Met<TM> M, const 
         if (act && !(tx & 1)) rc[Gc.idx((i + 1) >> 1, (j0 + 1) / 2 + rp, kc)] = (TC)(-v * (T)0.125);
     }
 }
+template <class T, class TM, class TR, class TC, bool X0 = false>
+__global__ void __launch_bounds__(512) k_resid_restrict(Geo G, Met<TM> M, const T* __restrict__ p,
+                                                        const TR* __restrict__ b, Geo Gc, TC* __restrict__ rc,
+                                                        T inv_pot, int P, int fx) {
+    d_resid_restrict<T, TM, TR, TC, X0>(G, M, p, b, Gc, rc, inv_pot, P, fx);
+}
+
 
 // ------------------------------------------------------------------------
 // Outer-loop scalar logic on the device (no host round trip per iteration).
@@ -957,7 +1002,7 @@ __global__ void __launch_bounds__(256) k_fin_resid(const double* __restrict__ pa
 }
 
 // finalize k_resid MODE 2 sums and compute the GCR step (beta, alpha)
-__global__ void __launch_bounds__(256) k_fin_apply(const double* __restrict__ part, int nb, double* S, int krylov) {
+__device__ __forceinline__ void d_fin_apply(const double* __restrict__ part, int nb, double* S, int krylov) {
     const bool mx[4] = {false, false, false, false};
     double o[4];
     fin_reduce<4>(part, nb, mx, o);
@@ -982,15 +1027,19 @@ __global__ void __launch_bounds__(256) k_fin_apply(const double* __restrict__ pa
     S[S_WPWP] = wwn;
     S[S_HAVEP] = 1.0;
 }
+__global__ void __launch_bounds__(256) k_fin_apply(const double* __restrict__ part, int nb, double* S, int krylov) {
+    d_fin_apply(part, nb, S, krylov);
+}
+
 
 // w <- w - beta wp ; z <- z - beta zp ; x += alpha z ; r -= alpha w ;
 // partial sums of r.r and max|r| (interior cells, segmented columns)
 template <class T>
-__global__ void __launch_bounds__(512) k_gcr_update(Geo G, T* __restrict__ x, T* __restrict__ r,
-                                                    T* __restrict__ z, T* __restrict__ w,
-                                                    const T* __restrict__ zp,
-                                                    const T* __restrict__ wp, const double* S,
-                                                    double* partials, int nblocks, int P) {
+__device__ __forceinline__ void d_gcr_update(Geo G, T* __restrict__ x, T* __restrict__ r,
+                                             T* __restrict__ z, T* __restrict__ w,
+                                             const T* __restrict__ zp,
+                                             const T* __restrict__ wp, const double* S,
+                                             double* partials, int nblocks, int P) {
     if (S[S_DONE] != 0.0) return;  // converged: uniform exit, partials unused
     Seg q;
     const bool act = seg_of(G, P, q);
@@ -1018,8 +1067,17 @@ __global__ void __launch_bounds__(512) k_gcr_update(Geo G, T* __restrict__ x, T*
     const bool mx[2] = {false, true};
     block_reduce_store<2>(v, partials, nblocks, mx);
 }
+template <class T>
+__global__ void __launch_bounds__(512) k_gcr_update(Geo G, T* __restrict__ x, T* __restrict__ r,
+                                                    T* __restrict__ z, T* __restrict__ w,
+                                                    const T* __restrict__ zp,
+                                                    const T* __restrict__ wp, const double* S,
+                                                    double* partials, int nblocks, int P) {
+    d_gcr_update<T>(G, x, r, z, w, zp, wp, S, partials, nblocks, P);
+}
 
-__global__ void __launch_bounds__(256) k_fin_update(const double* __restrict__ part, int nb, double* S) {
+
+__device__ __forceinline__ void d_fin_update(const double* __restrict__ part, int nb, double* S) {
     if (S[S_DONE] != 0.0) return;  // uniform: r.r and max|r| stay those of the converged step
     const bool mx[2] = {false, true};
     double o[2];
@@ -1032,6 +1090,10 @@ __global__ void __launch_bounds__(256) k_fin_update(const double* __restrict__ p
         if (m < S[S_THR]) S[S_DONE] = 1.0;
     }
 }
+__global__ void __launch_bounds__(256) k_fin_update(const double* __restrict__ part, int nb, double* S) {
+    d_fin_update(part, nb, S);
+}
+
 
 // copy the scalars of n solves into one buffer (one read-back per batch)
 struct SPtrs {
@@ -1050,11 +1112,12 @@ __global__ void k_gather_S(const double* const* __restrict__ S, int n, double* _
 // a coarse level that keeps the xi (fx = 1) or the xi and eta counts (fx = fy
 // = 1) averages its four or two children the same way
 template <class TF, class TC>
-__global__ void __launch_bounds__(256) k_restrict(Geo Gc, Geo Gf, const TF* __restrict__ rf,
-                                                  TC* __restrict__ rc, int fx, int fy) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+__device__ __forceinline__ void d_restrict(Geo Gc, Geo Gf, const TF* __restrict__ rf,
+                                           TC* __restrict__ rc, int fx, int fy, int gx) {
+    // grid x = gx blocks per coarse row x planes (the plane index folded into x)
+    const int i = (blockIdx.x % gx) * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+    const int k = blockIdx.x / gx + 1;
     if (i > Gc.nx || j > Gc.ny) return;
     const long sy = Gf.sy, sz = Gf.sz;
     if (fx == 1 && fy == 1) {
@@ -1073,6 +1136,12 @@ __global__ void __launch_bounds__(256) k_restrict(Geo Gc, Geo Gf, const TF* __re
            rf[f - sy - sz] + rf[f - 1 - sy - sz];
     rc[Gc.idx(i, j, k)] = (TC)(-s * (TF)0.125);
 }
+template <class TF, class TC>
+__global__ void __launch_bounds__(256) k_restrict(Geo Gc, Geo Gf, const TF* __restrict__ rf,
+                                                  TC* __restrict__ rc, int fx, int fy, int gx) {
+    d_restrict<TF, TC>(Gc, Gf, rf, rc, fx, fy, gx);
+}
+
 
 // pf += trilinear cell-centred interpolation of ec over the ghost-inclusive
 // fine range (fsrc/prolong.f03:64-261): weight 3 per axis for the coarse cell
@@ -1082,11 +1151,12 @@ __global__ void __launch_bounds__(256) k_restrict(Geo Gc, Geo Gf, const TF* __re
 // 2x2x2 coarse block once and updates its 8 fine children, two consecutive i
 // at a time (the fine strides are even, so the pair is aligned).
 template <class TC, class TF>
-__global__ void __launch_bounds__(256) k_prolong_blk(Geo Gc, Geo Gf, const TC* __restrict__ ec,
-                                                     TF* __restrict__ pf) {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void d_prolong_blk(Geo Gc, Geo Gf, const TC* __restrict__ ec,
+                                              TF* __restrict__ pf, int gx) {
+    // grid x = gx blocks per coarse row x planes (the plane index folded into x)
+    const int I = (blockIdx.x % gx) * blockDim.x + threadIdx.x;
     const int J = blockIdx.y * blockDim.y + threadIdx.y;
-    const int Kc = blockIdx.z;
+    const int Kc = blockIdx.x / gx;
     if (I > Gc.nx || J > Gc.ny) return;
     const long c = Gc.idx(I, J, Kc);
     const long sy = Gc.sy, sz = Gc.sz;
@@ -1117,6 +1187,12 @@ __global__ void __launch_bounds__(256) k_prolong_blk(Geo Gc, Geo Gf, const TC* _
             *f = v;
         }
 }
+template <class TC, class TF>
+__global__ void __launch_bounds__(256) k_prolong_blk(Geo Gc, Geo Gf, const TC* __restrict__ ec,
+                                                     TF* __restrict__ pf, int gx) {
+    d_prolong_blk<TC, TF>(Gc, Gf, ec, pf, gx);
+}
+
 
 // pf += bilinear (eta, zeta) interpolation of ec for a coarse level with the
 // fine xi count: the prolongation above without its xi factor (weights 3 and
@@ -1124,11 +1200,12 @@ __global__ void __launch_bounds__(256) k_prolong_blk(Geo Gc, Geo Gf, const TC* _
 // thread per coarse (i, J, K), i in 0..nx+1, J in 0..ncy, K in 0..ncz (ghosts
 // included), updating its four fine children.
 template <class TC, class TF>
-__global__ void __launch_bounds__(256) k_prolong_yz(Geo Gc, Geo Gf, const TC* __restrict__ ec,
-                                                    TF* __restrict__ pf) {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void d_prolong_yz(Geo Gc, Geo Gf, const TC* __restrict__ ec,
+                                             TF* __restrict__ pf, int gx) {
+    // grid x = gx blocks per coarse row x planes (the plane index folded into x)
+    const int I = (blockIdx.x % gx) * blockDim.x + threadIdx.x;
     const int J = blockIdx.y * blockDim.y + threadIdx.y;
-    const int Kc = blockIdx.z;
+    const int Kc = blockIdx.x / gx;
     if (I > Gc.nx + 1 || J > Gc.ny) return;
     const long c = Gc.idx(I, J, Kc);
     const long sy = Gc.sy, sz = Gc.sz;
@@ -1143,23 +1220,36 @@ __global__ void __launch_bounds__(256) k_prolong_yz(Geo Gc, Geo Gf, const TC* __
                 (wk0 * (wj0 * e00 + wj1 * e01) + wk1 * (wj0 * e10 + wj1 * e11)) * (TF)(1.0 / 16.0);
         }
 }
+template <class TC, class TF>
+__global__ void __launch_bounds__(256) k_prolong_yz(Geo Gc, Geo Gf, const TC* __restrict__ ec,
+                                                    TF* __restrict__ pf, int gx) {
+    d_prolong_yz<TC, TF>(Gc, Gf, ec, pf, gx);
+}
+
 
 // pf += linear zeta interpolation of ec for a coarse level with the fine xi
 // and eta counts (weights 3 and 1 over 4).  One thread per coarse (i, j, K),
 // i in 0..nx+1, j in 0..ny+1, K in 0..ncz (ghosts included), updating its two
 // fine children (i, j, 2K) and (i, j, 2K+1).
 template <class TC, class TF>
-__global__ void __launch_bounds__(256) k_prolong_z(Geo Gc, Geo Gf, const TC* __restrict__ ec,
-                                                   TF* __restrict__ pf) {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void d_prolong_z(Geo Gc, Geo Gf, const TC* __restrict__ ec,
+                                            TF* __restrict__ pf, int gx) {
+    // grid x = gx blocks per coarse row x planes (the plane index folded into x)
+    const int I = (blockIdx.x % gx) * blockDim.x + threadIdx.x;
     const int J = blockIdx.y * blockDim.y + threadIdx.y;
-    const int Kc = blockIdx.z;
+    const int Kc = blockIdx.x / gx;
     if (I > Gc.nx + 1 || J > Gc.ny + 1) return;
     const long c = Gc.idx(I, J, Kc);
     const TF e0 = (TF)ec[c], e1 = (TF)ec[c + Gc.sz];
     pf[Gf.idx(I, J, 2 * Kc)] += ((TF)3 * e0 + e1) * (TF)0.25;
     pf[Gf.idx(I, J, 2 * Kc + 1)] += (e0 + (TF)3 * e1) * (TF)0.25;
 }
+template <class TC, class TF>
+__global__ void __launch_bounds__(256) k_prolong_z(Geo Gc, Geo Gf, const TC* __restrict__ ec,
+                                                   TF* __restrict__ pf, int gx) {
+    d_prolong_z<TC, TF>(Gc, Gf, ec, pf, gx);
+}
+
 
 // ------------------------------------------------------------------------
 // Exact ghost fill (outer FP64 operator, walls in xi/zeta): face relations
@@ -1385,7 +1475,7 @@ __host__ __device__ inline void halo_edge_point(const Geo& G, T* __restrict__ p,
 // the starting guess of k_eta_ghosts, taken before the launch so no block
 // reads a ghost that another block of that launch writes.
 template <class T>
-__global__ void __launch_bounds__(256) k_ghost_rows(Geo G, const T* __restrict__ p, T* __restrict__ snap) {
+__device__ __forceinline__ void d_ghost_rows(Geo G, const T* __restrict__ p, T* __restrict__ snap) {
     const long NX = G.nx + 2, NZ = G.nz + 2;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 2 * NX * NZ) return;
@@ -1393,10 +1483,15 @@ __global__ void __launch_bounds__(256) k_ghost_rows(Geo G, const T* __restrict__
     const long r = t % (NX * NZ);
     snap[t] = p[G.idx((int)(r % NX), side ? G.ny + 1 : 0, (int)(r / NX))];
 }
+template <class T>
+__global__ void __launch_bounds__(256) k_ghost_rows(Geo G, const T* __restrict__ p, T* __restrict__ snap) {
+    d_ghost_rows<T>(G, p, snap);
+}
+
 
 template <class T, class TM, int TX, int TK>
-__global__ void __launch_bounds__(1024) k_eta_ghosts(Geo G, Met<TM> M, T* __restrict__ p, const T* __restrict__ snap,
-                                                     BCs bc, int passes) {
+__device__ __forceinline__ void d_eta_ghosts(Geo G, Met<TM> M, T* __restrict__ p, const T* __restrict__ snap,
+                                             BCs bc, int passes) {
     extern __shared__ __align__(16) unsigned char eta_lds[];
     const int Hw = 2 * passes, WX = TX + 2 * Hw, WK = TK + 2 * Hw, NW = WX * WK;
     T* sf = (T*)eta_lds;
@@ -1450,23 +1545,39 @@ __global__ void __launch_bounds__(1024) k_eta_ghosts(Geo G, Met<TM> M, T* __rest
         write_images(G, p, i, jg, k, v);
     }
 }
+template <class T, class TM, int TX, int TK>
+__global__ void __launch_bounds__(1024) k_eta_ghosts(Geo G, Met<TM> M, T* __restrict__ p, const T* __restrict__ snap,
+                                                     BCs bc, int passes) {
+    d_eta_ghosts<T, TM, TX, TK>(G, M, p, snap, bc, passes);
+}
+
 
 template <class T, class TM>
-__global__ void k_halo_face(Geo G, Met<TM> M, T* __restrict__ p, int color, BCs bc, int set, int images) {
+__device__ __forceinline__ void d_halo_face(Geo G, Met<TM> M, T* __restrict__ p, int color, BCs bc, int set, int images) {
     halo_face_point(G, M, p, (long)blockIdx.x * blockDim.x + threadIdx.x, color, bc, set, images);
 }
+template <class T, class TM>
+__global__ void k_halo_face(Geo G, Met<TM> M, T* __restrict__ p, int color, BCs bc, int set, int images) {
+    d_halo_face<T, TM>(G, M, p, color, bc, set, images);
+}
+
 
 template <class T>
-__global__ void k_halo_edges(Geo G, T* __restrict__ p) {
+__device__ __forceinline__ void d_halo_edges(Geo G, T* __restrict__ p) {
     halo_edge_point(G, p, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
+template <class T>
+__global__ void k_halo_edges(Geo G, T* __restrict__ p) {
+    d_halo_edges<T>(G, p);
+}
+
 
 // Exact coarsest-level solve: p (every ghosted position) = K b, K the dense
 // map built at set-up (rows: ghosted positions, columns: interior cells, i
 // fastest), b the interior of the ghosted rhs.  One wavefront per row, lanes
 // stride the columns, fixed shuffle tree: deterministic.
-__global__ void __launch_bounds__(256) k_coarse_dense(Geo G, const double* __restrict__ K, int N,
-                                                      const double* __restrict__ b, double* __restrict__ p) {
+__device__ __forceinline__ void d_coarse_dense(Geo G, const double* __restrict__ K, int N,
+                                               const double* __restrict__ b, double* __restrict__ p) {
     const int lane = threadIdx.x & 63;
     const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= G.n) return;  // whole wavefronts leave together
@@ -1479,6 +1590,11 @@ __global__ void __launch_bounds__(256) k_coarse_dense(Geo G, const double* __res
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
     if (lane == 0) p[row] = acc;
 }
+__global__ void __launch_bounds__(256) k_coarse_dense(Geo G, const double* __restrict__ K, int N,
+                                                      const double* __restrict__ b, double* __restrict__ p) {
+    d_coarse_dense(G, K, N, b, p);
+}
+
 
 
 // ------------------------------------------------------------------------
@@ -1972,11 +2088,16 @@ __global__ void __launch_bounds__(256) k_read16(long n, const float4* __restrict
 
 // elementwise helpers over the full ghosted array
 template <class TI, class TO>
-__global__ void k_convert(long n, const TI* __restrict__ a, TO* __restrict__ b) {
+__device__ __forceinline__ void d_convert(long n, const TI* __restrict__ a, TO* __restrict__ b) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long stride = (long)gridDim.x * blockDim.x;
     for (; t < n; t += stride) b[t] = (TO)a[t];
 }
+template <class TI, class TO>
+__global__ void k_convert(long n, const TI* __restrict__ a, TO* __restrict__ b) {
+    d_convert<TI, TO>(n, a, b);
+}
+
 
 // b = 16-bit copy of a * scale (round to nearest)
 __global__ void k_to_half(long n, const float* __restrict__ a, _Float16* __restrict__ b, float scale) {

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/ples_mg.h"
+#include "group_launch.hpp"
 #include "mg_kernels.hpp"
 #include "proj_kernels.hpp"
 
@@ -59,6 +60,16 @@ struct Rec {
     hipEvent_t a, b;
 };
 
+// one recorded kernel launch of a problem: the grouped launcher of its device
+// body, the launch shape and the problem's table entry (stop flag and
+// arguments, bytes of a GEntry)
+struct KRec {
+    hipError_t (*go)(const void* tab, int n, dim3 g, dim3 b, size_t shm, hipStream_t s);
+    dim3 g, b;
+    size_t shm;
+    std::vector<unsigned char> e;
+};
+
 }  // namespace
 
 struct ples_mg_handle {
@@ -90,6 +101,7 @@ struct ples_mg_handle {
     std::vector<std::pair<void*, size_t>> allocs;
     unsigned timing = 0;  // bit f: time kernel family f
     std::vector<Rec> recs;
+    std::vector<KRec>* rec = nullptr;  // set: kernel launches are recorded, not enqueued
     hipEvent_t t0 = nullptr, t1 = nullptr;
     hipEvent_t sev[4] = {};          // read-back ring of the refinement loop
     hipGraphExec_t gexec[2] = {};    // captured refinement step per (Z, W) pair
@@ -289,6 +301,28 @@ struct KT {
     }
 };
 
+// Launch of device body F (kernel kern, its single-problem wrapper) with
+// arguments a: enqueued on the handle's stream, or recorded for a grouped
+// launch while h->rec is set (2-D grids only: blockIdx.z selects the problem)
+template <auto F, int LB, int WPE, class K, class... A>
+void klaunch(H* h, K kern, dim3 g, dim3 b, size_t shm, A... a) {
+    if (h->rec) {
+        const GEntry<A...> e{h->S + S_DONE, make_pack(a...)};
+        KRec r{&launch_group<F, LB, WPE, A...>, g, b, shm, {}};
+        r.e.assign((const unsigned char*)&e, (const unsigned char*)&e + sizeof e);
+        if (g.z != 1) r.go = nullptr;  // refused when the batch groups the records
+        h->rec->push_back(std::move(r));
+        return;
+    }
+    hipLaunchKernelGGL(kern, g, b, shm, h->s, a...);
+}
+// KLAUNCH(h, name, TPL(template args), LB, WPE, grid, block, shm, args...):
+// d_name / k_name of the kernel headers
+#define TPL(...) <__VA_ARGS__>
+#define KLAUNCH(h, NAME, TA, LB, WPE, g, b, shm, ...) \
+    klaunch<&d_##NAME TA, LB, WPE>(h, &k_##NAME TA, g, b, shm, __VA_ARGS__)
+#define KLAUNCH0(h, NAME, LB, WPE, g, b, shm, ...) klaunch<&d_##NAME, LB, WPE>(h, &k_##NAME, g, b, shm, __VA_ARGS__)
+
 // metric arrays the level's stencil kernels read
 double nmet(const Level& L) { return L.x0 ? 7.0 : 9.0; }
 
@@ -325,11 +359,11 @@ bool eta_ghosts(H* h, const Geo& G, const Met<TM>& m, T* p, const BCs& bc, int p
     const size_t lds = (size_t)(TX + 2 * hw) * (TK + 2 * hw) * 4 * sizeof(T);
     const long nrow = 2L * (G.nx + 2) * (G.nz + 2);
     if (lds > 65536 || (size_t)nrow * sizeof(T) > h->snap_bytes) return false;
-    hipLaunchKernelGGL((k_ghost_rows<T>), dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, h->s, G,
-                       (const T*)p, (T*)h->snap);
+    KLAUNCH(h, ghost_rows, TPL(T), 256, 1, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, G, (const T*)p,
+            (T*)h->snap);
     const int nb = 2 * ((G.nx + TX - 1) / TX) * ((G.nz + TK - 1) / TK);
-    hipLaunchKernelGGL((k_eta_ghosts<T, TM, TX, TK>), dim3(nb), dim3(1024), lds, h->s, G, m, p, (const T*)h->snap,
-                       bc, passes);
+    KLAUNCH(h, eta_ghosts, TPL(T, TM, TX, TK), 1024, 1, dim3(nb), dim3(1024), lds, G, m, p, (const T*)h->snap, bc,
+            passes);
     return true;
 }
 
@@ -347,7 +381,7 @@ int halo(H* h, const Level& L, T* p, double an_over_dt, bool with_cs, int passes
         KT kt(h, PLES_MG_K_HALO, (double)(2L * G.nx * G.nz * 8 + (interior_images ? 0 : ne * 2)) * sizeof(T));
         if (eta_ghosts<T, T>(h, G, met<T>(L), p, bc, passes)) {
             if (!interior_images)
-                hipLaunchKernelGGL((k_halo_edges<T>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, h->s, G, p);
+                KLAUNCH(h, halo_edges, TPL(T), 1024, 1, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, G, p);
             CK(hipGetLastError());
             return PLES_MG_OK;
         }
@@ -365,10 +399,10 @@ int halo(H* h, const Level& L, T* p, double an_over_dt, bool with_cs, int passes
             const long nset = face_set_size(G, set);
             if (nset == 0) continue;
             for (int color = 0; color < 2; ++color)
-                hipLaunchKernelGGL((k_halo_face<T, T>), dim3((unsigned)((nset + 255) / 256)), dim3(256), 0, h->s,
-                                   G, met<T>(L), p, color, bc, set, 0);
+                KLAUNCH(h, halo_face, TPL(T, T), 1024, 1, dim3((unsigned)((nset + 255) / 256)), dim3(256), 0, G,
+                        met<T>(L), p, color, bc, set, 0);
         }
-        hipLaunchKernelGGL((k_halo_edges<T>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, h->s, G, p);
+        KLAUNCH(h, halo_edges, TPL(T), 1024, 1, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, G, p);
     }
     CK(hipGetLastError());
     return PLES_MG_OK;
@@ -398,8 +432,7 @@ int halo64(H* h, double* p, bool with_bc, bool fast = false, int passes_override
         KT kt(h, PLES_MG_K_HALO, (double)(2L * G.nx * G.nz * 8 + (fast ? 0 : ne * 2)) * 8.0);
         if (eta_ghosts<double, double>(h, G, m, p, bc, passes)) {
             if (!fast)
-                hipLaunchKernelGGL((k_halo_edges<double>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, h->s, G,
-                                   p);
+                KLAUNCH(h, halo_edges, TPL(double), 1024, 1, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, G, p);
             CK(hipGetLastError());
             return PLES_MG_OK;
         }
@@ -410,12 +443,11 @@ int halo64(H* h, double* p, bool with_bc, bool fast = false, int passes_override
             const long nset = face_set_size(G, set);
             if (nset == 0) continue;
             for (int color = 0; color < 2; ++color)
-                hipLaunchKernelGGL((k_halo_face<double, double>), dim3((unsigned)((nset + 255) / 256)), dim3(256),
-                                   0, h->s, G, m, p, color, bc, set, fast ? 1 : 0);
+                KLAUNCH(h, halo_face, TPL(double, double), 1024, 1, dim3((unsigned)((nset + 255) / 256)), dim3(256),
+                        0, G, m, p, color, bc, set, fast ? 1 : 0);
         }
         if (!fast)
-            hipLaunchKernelGGL((k_halo_edges<double>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, h->s, G,
-                               p);
+            KLAUNCH(h, halo_edges, TPL(double), 1024, 1, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, G, p);
     }
     CK(hipGetLastError());
     return PLES_MG_OK;
@@ -433,9 +465,11 @@ template <int P, int MR, class T, class TM, class TR>
 void yline_go(H* h, Level& L, const Met<TM>& m, double inv_pot, const TR* rhs, int zin) {
     const ColLaunch cl = seg_launch(L.G, P);
     if constexpr (MR > 0) {
-        auto go = [&](auto kern) {
-            hipLaunchKernelGGL(kern, cl.grid, cl.block, 0, h->s, L.G, m, line_f(L), (const T*)L.p[L.cur],
-                               (T*)L.p[1 - L.cur], rhs, (T)h->omega_now, (T)inv_pot, zin);
+        auto go = [&](auto x0) {
+            constexpr bool X0 = decltype(x0)::value;
+            KLAUNCH(h, yline, TPL(P, MR, T, Met<TM>, TR, X0), 512, (YlineWpe<T, Met<TM>>::value), cl.grid, cl.block,
+                    0, L.G, m, line_f(L), (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->omega_now,
+                    (T)inv_pot, zin);
         };
         // the variant without the g13, g31 loads: measured faster for the 16-bit-metric
         // and FP64 smoothers, slower for the FP32-metric one (kept with the loads); the
@@ -443,23 +477,23 @@ void yline_go(H* h, Level& L, const Met<TM>& m, double inv_pot, const TR* rhs, i
         if constexpr (sizeof(TM) == 2) {
             if (L.x0 && L.pk_a && (P == 1 || L.vw16)) {
                 const MetPk mp{L.pk_a, L.pk_z, L.pk_b, L.g16[4]};
-                hipLaunchKernelGGL((k_yline<P, MR, T, MetPk, TR, true>), cl.grid, cl.block, 0, h->s, L.G, mp,
-                                   line_f(L), (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->omega_now,
-                                   (T)inv_pot, zin);
+                KLAUNCH(h, yline, TPL(P, MR, T, MetPk, TR, true), 512, (YlineWpe<T, MetPk>::value), cl.grid,
+                        cl.block, 0, L.G, mp, line_f(L), (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs,
+                        (T)h->omega_now, (T)inv_pot, zin);
             } else if (L.x0) {
-                go(k_yline<P, MR, T, Met<TM>, TR, true>);
+                go(std::true_type{});
             } else {
-                go(k_yline<P, MR, T, Met<TM>, TR, false>);
+                go(std::false_type{});
             }
         } else if constexpr (sizeof(T) == 8) {
-            if (L.x0) go(k_yline<P, MR, T, Met<TM>, TR, true>);
-            else go(k_yline<P, MR, T, Met<TM>, TR, false>);
+            if (L.x0) go(std::true_type{});
+            else go(std::false_type{});
         } else {
-            go(k_yline<P, MR, T, Met<TM>, TR, false>);
+            go(std::false_type{});
         }
     } else
-        hipLaunchKernelGGL((k_yline_staged<P, T, TM, TR>), cl.grid, cl.block, 0, h->s, L.G, m, line_f(L),
-                           (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->omega_now, (T)inv_pot, zin);
+        KLAUNCH(h, yline_staged, TPL(P, T, TM, TR), 512, 1, cl.grid, cl.block, 0, L.G, m, line_f(L),
+                (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->omega_now, (T)inv_pot, zin);
 }
 
 template <int P, int MR, class T, class TR>
@@ -547,12 +581,14 @@ int vresid(H* h, Level& L, const TR* rhs, bool fine) {
     }
     const double N = (double)L.ncell();
     KT kt(h, fine ? PLES_MG_K_RESID_FINE : PLES_MG_K_RESID, N * (nmet(L) * sizeof(T) + 2 * sizeof(T) + sizeof(TR)));
-    auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, cl.grid, cl.block, 0, h->s, L.G, met<T>(L), (const T*)L.p[L.cur], rhs, (T*)L.r,
-                           (T)(1.0 / L.pot), (const double*)nullptr, (const double*)nullptr, (double*)nullptr, 0, P);
+    auto go = [&](auto x0) {
+        constexpr bool X0 = decltype(x0)::value;
+        KLAUNCH(h, resid, TPL(0, T, T, TR, T, double, X0), 512, 1, cl.grid, cl.block, 0, L.G, met<T>(L),
+                (const T*)L.p[L.cur], rhs, (T*)L.r, (T)(1.0 / L.pot), (const double*)nullptr, (const double*)nullptr,
+                (double*)nullptr, 0, P);
     };
-    if (L.x0) go(k_resid<0, T, T, TR, T, double, true>);
-    else go(k_resid<0, T, T, TR, T, double, false>);
+    if (L.x0) go(std::true_type{});
+    else go(std::false_type{});
     CK(hipGetLastError());
     return PLES_MG_OK;
 }
@@ -560,10 +596,11 @@ int vresid(H* h, Level& L, const TR* rhs, bool fine) {
 template <class TF, class TC>
 int restrict_to(H* h, Level& F, Level& Cc) {
     dim3 block(std::min(64, pow2ceil(Cc.G.nx)), 4);
-    dim3 grid((Cc.G.nx + block.x - 1) / block.x, (Cc.G.ny + 3) / 4, Cc.G.nz);
+    const int gx = (Cc.G.nx + block.x - 1) / block.x;
+    dim3 grid(gx * Cc.G.nz, (Cc.G.ny + 3) / 4);
     KT kt(h, PLES_MG_K_TRANSFER, (double)F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC));
-    hipLaunchKernelGGL((k_restrict<TF, TC>), grid, block, 0, h->s, Cc.G, F.G, (const TF*)F.r, (TC*)Cc.b,
-                       F.G.nx / Cc.G.nx, F.G.ny / Cc.G.ny);
+    KLAUNCH(h, restrict, TPL(TF, TC), 256, 1, grid, block, 0, Cc.G, F.G, (const TF*)F.r, (TC*)Cc.b,
+            F.G.nx / Cc.G.nx, F.G.ny / Cc.G.ny, gx);
     CK(hipGetLastError());
     return PLES_MG_OK;
 }
@@ -585,12 +622,13 @@ int resid_restrict(H* h, Level& F, Level& Cc, const TR* rhs, bool fine) {
     const double N = (double)F.ncell();
     KT kt(h, fine ? PLES_MG_K_RESID_FINE : PLES_MG_K_RESID,
           N * (nmet(F) * sizeof(T) + sizeof(T) + sizeof(TR)) + (double)Cc.ncell() * sizeof(TC));
-    auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, grid, block, 0, h->s, F.G, met<T>(F), (const T*)F.p[F.cur], rhs, Cc.G, (TC*)Cc.b,
-                           (T)(1.0 / F.pot), P, fx);
+    auto go = [&](auto x0) {
+        constexpr bool X0 = decltype(x0)::value;
+        KLAUNCH(h, resid_restrict, TPL(T, T, TR, TC, X0), 512, 1, grid, block, 0, F.G, met<T>(F),
+                (const T*)F.p[F.cur], rhs, Cc.G, (TC*)Cc.b, (T)(1.0 / F.pot), P, fx);
     };
-    if (F.x0) go(k_resid_restrict<T, T, TR, TC, true>);
-    else go(k_resid_restrict<T, T, TR, TC, false>);
+    if (F.x0) go(std::true_type{});
+    else go(std::false_type{});
     CK(hipGetLastError());
     return PLES_MG_OK;
 }
@@ -599,28 +637,31 @@ template <class TC, class TF>
 int prolong_to(H* h, Level& Cc, Level& F) {
     if (Cc.G.ny == F.G.ny) {  // coarsened in zeta only: 2 fine children per coarse cell
         dim3 block(std::min(64, pow2ceil(Cc.G.nx + 2)), 4);
-        dim3 grid((Cc.G.nx + 2 + block.x - 1) / block.x, (Cc.G.ny + 2 + 3) / 4, Cc.G.nz + 1);
+        const int gx = (Cc.G.nx + 2 + block.x - 1) / block.x;
+        dim3 grid(gx * (Cc.G.nz + 1), (Cc.G.ny + 2 + 3) / 4);
         KT kt(h, PLES_MG_K_TRANSFER, 2.0 * F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC));
-        hipLaunchKernelGGL((k_prolong_z<TC, TF>), grid, block, 0, h->s, Cc.G, F.G, (const TC*)Cc.p[Cc.cur],
-                           (TF*)F.p[F.cur]);
+        KLAUNCH(h, prolong_z, TPL(TC, TF), 256, 1, grid, block, 0, Cc.G, F.G, (const TC*)Cc.p[Cc.cur],
+                (TF*)F.p[F.cur], gx);
         CK(hipGetLastError());
         return PLES_MG_OK;
     }
     if (Cc.G.nx == F.G.nx) {  // coarsened in eta and zeta only: 4 fine children per coarse cell
         dim3 block(std::min(64, pow2ceil(Cc.G.nx + 2)), 4);
-        dim3 grid((Cc.G.nx + 2 + block.x - 1) / block.x, (Cc.G.ny + 1 + 3) / 4, Cc.G.nz + 1);
+        const int gx = (Cc.G.nx + 2 + block.x - 1) / block.x;
+        dim3 grid(gx * (Cc.G.nz + 1), (Cc.G.ny + 1 + 3) / 4);
         KT kt(h, PLES_MG_K_TRANSFER, 2.0 * F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC));
-        hipLaunchKernelGGL((k_prolong_yz<TC, TF>), grid, block, 0, h->s, Cc.G, F.G, (const TC*)Cc.p[Cc.cur],
-                           (TF*)F.p[F.cur]);
+        KLAUNCH(h, prolong_yz, TPL(TC, TF), 256, 1, grid, block, 0, Cc.G, F.G, (const TC*)Cc.p[Cc.cur],
+                (TF*)F.p[F.cur], gx);
         CK(hipGetLastError());
         return PLES_MG_OK;
     }
     // one thread per coarse block (I, J, K) in 0..nc (ghosts included): 8 fine children each
     dim3 block(std::min(64, pow2ceil(Cc.G.nx + 1)), 4);
-    dim3 grid((Cc.G.nx + 1 + block.x - 1) / block.x, (Cc.G.ny + 1 + 3) / 4, Cc.G.nz + 1);
+    const int gx = (Cc.G.nx + 1 + block.x - 1) / block.x;
+    dim3 grid(gx * (Cc.G.nz + 1), (Cc.G.ny + 1 + 3) / 4);
     KT kt(h, PLES_MG_K_TRANSFER, 2.0 * F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC));
-    hipLaunchKernelGGL((k_prolong_blk<TC, TF>), grid, block, 0, h->s, Cc.G, F.G, (const TC*)Cc.p[Cc.cur],
-                       (TF*)F.p[F.cur]);
+    KLAUNCH(h, prolong_blk, TPL(TC, TF), 256, 1, grid, block, 0, Cc.G, F.G, (const TC*)Cc.p[Cc.cur],
+            (TF*)F.p[F.cur], gx);
     CK(hipGetLastError());
     return PLES_MG_OK;
 }
@@ -707,8 +748,8 @@ size_t coarse_lds_bytes(const Level& L) { return (size_t)L.G.n * (12 * sizeof(do
 int coarse_solve(H* h, Level& L, const double* rhs, int nu) {
     if (L.kd) {  // exact: one matrix-vector product gives the ghosted solution
         KT kt(h, PLES_MG_K_SMOOTH, (double)L.G.n * L.ncell() * 8.0);
-        hipLaunchKernelGGL(k_coarse_dense, dim3((unsigned)((L.G.n + 3) / 4)), dim3(256), 0, h->s, L.G,
-                           (const double*)L.kd, (int)L.ncell(), rhs, (double*)L.p[1 - L.cur]);
+        KLAUNCH0(h, coarse_dense, 256, 1, dim3((unsigned)((L.G.n + 3) / 4)), dim3(256), 0, L.G,
+                 (const double*)L.kd, (int)L.ncell(), rhs, (double*)L.p[1 - L.cur]);
         CK(hipGetLastError());
         L.cur ^= 1;
         return PLES_MG_OK;
@@ -718,15 +759,15 @@ int coarse_solve(H* h, Level& L, const double* rhs, int nu) {
     const size_t lds = coarse_lds_bytes(L);
     if (lds <= (size_t)h->lds_max) {
         // result lands in p[1 - cur]
-        hipLaunchKernelGGL((k_coarse_lds<double>), dim3(1), dim3(256), lds, h->s, L.G, met<double>(L), line_f(L),
-                           (double*)L.p[1 - L.cur], rhs, (double)h->o.omega, 1.0 / L.pot, nu);
+        KLAUNCH(h, coarse_lds, TPL(double), 256, 1, dim3(1), dim3(256), lds, L.G, met<double>(L), line_f(L),
+                (double*)L.p[1 - L.cur], rhs, (double)h->o.omega, 1.0 / L.pot, nu);
         CK(hipGetLastError());
         L.cur ^= 1;
         return PLES_MG_OK;
     }
     const int threads = std::min(1024, ((ncol + 63) / 64) * 64);
-    hipLaunchKernelGGL((k_coarse<double, double>), dim3(1), dim3(threads), 0, h->s, L.G, met<double>(L), line_f(L),
-                       (double*)L.p[L.cur], (double*)L.p[1 - L.cur], rhs, (double)h->o.omega, 1.0 / L.pot, nu);
+    KLAUNCH(h, coarse, TPL(double, double), 1024, 1, dim3(1), dim3(threads), 0, L.G, met<double>(L), line_f(L),
+            (double*)L.p[L.cur], (double*)L.p[1 - L.cur], rhs, (double)h->o.omega, 1.0 / L.pot, nu);
     CK(hipGetLastError());
     if (nu & 1) L.cur ^= 1;
     return PLES_MG_OK;
@@ -805,32 +846,32 @@ int step_body(H* h, int q) {
         const long n = L0.G.n;
         const int nb = (int)std::min<long>((n + 255) / 256, 8192);
         if (L0.f32)
-            hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(256), 0, h->s, n,
-                               (const float*)L0.p[L0.cur], Zq);
+            KLAUNCH(h, convert, TPL(float, double), 1024, 1, dim3(nb), dim3(256), 0, n, (const float*)L0.p[L0.cur],
+                    Zq);
         else
-            hipLaunchKernelGGL((k_convert<double, double>), dim3(nb), dim3(256), 0, h->s, n,
-                               (const double*)L0.p[L0.cur], Zq);
+            KLAUNCH(h, convert, TPL(double, double), 1024, 1, dim3(nb), dim3(256), 0, n,
+                    (const double*)L0.p[L0.cur], Zq);
     }
     CKS(halo64(h, Zq, false, true));
     const bool kry = h->o.krylov != 0;
     {
         KT kt(h, PLES_MG_K_OUTER_APPLY, N0 * (8 * nmet(L0) + 24 + (kry ? 8 : 0)));
-        auto go = [&](auto kern) {
-            hipLaunchKernelGGL(kern, cl.grid, cl.block, 0, h->s, L0.G, met<double>(L0), (const double*)Zq,
-                               (const double*)nullptr, Wq, 1.0, (const double*)h->rvec,
-                               (const double*)(kry ? Wp : nullptr), h->part, cl.nblocks, L0.P);
+        auto go = [&](auto x0) {
+            constexpr bool X0 = decltype(x0)::value;
+            KLAUNCH(h, resid, TPL(2, double, double, double, double, double, X0), 512, 1, cl.grid, cl.block, 0,
+                    L0.G, met<double>(L0), (const double*)Zq, (const double*)nullptr, Wq, 1.0,
+                    (const double*)h->rvec, (const double*)(kry ? Wp : nullptr), h->part, cl.nblocks, L0.P);
         };
-        if (L0.x0) go(k_resid<2, double, double, double, double, double, true>);
-        else go(k_resid<2, double, double, double, double, double, false>);
+        if (L0.x0) go(std::true_type{});
+        else go(std::false_type{});
     }
-    hipLaunchKernelGGL(k_fin_apply, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S,
-                       h->o.krylov);
+    KLAUNCH0(h, fin_apply, 256, 1, dim3(1), dim3(256), 0, (const double*)h->part, cl.nblocks, h->S, h->o.krylov);
     {
         KT kt(h, PLES_MG_K_OUTER_UPDATE, N0 * (48 + (kry ? 32 + 16 : 0)));
-        hipLaunchKernelGGL(k_gcr_update<double>, cl.grid, cl.block, 0, h->s, L0.G, h->x, h->rvec, Zq, Wq,
-                           (const double*)Zp, (const double*)Wp, (const double*)h->S, h->part, cl.nblocks, L0.P);
+        KLAUNCH(h, gcr_update, TPL(double), 512, 1, cl.grid, cl.block, 0, L0.G, h->x, h->rvec, Zq, Wq,
+                (const double*)Zp, (const double*)Wp, (const double*)h->S, h->part, cl.nblocks, L0.P);
     }
-    hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S);
+    KLAUNCH0(h, fin_update, 256, 1, dim3(1), dim3(256), 0, (const double*)h->part, cl.nblocks, h->S);
     CK(hipGetLastError());
     return PLES_MG_OK;
 }
@@ -2771,19 +2812,135 @@ struct ples_mg_batch {
     double* ring = nullptr;                 // pinned: 2 slots x n x S_COUNT
     hipEvent_t rev[2] = {};
     size_t bytes = 0;
+    // grouped launches: the problems of one family (identical launch
+    // sequences: same shape and options) share one launch per kernel of the
+    // K steps; tables of per-problem arguments per graph slot
+    bool grouped = true;
+    struct Fam {
+        std::vector<int> mem;       // problem indices
+        std::vector<KRec> recs;     // launch shapes (entries unused)
+        std::vector<size_t> off;    // table offset of each launch
+    };
+    std::vector<Fam> fam[2];
+    void* gtab[2] = {};
 };
 
 namespace {
 
 using B = ples_mg_batch;
 
+std::vector<std::vector<int>> batch_state(const B* b);
+void set_batch_state(B* b, const std::vector<std::vector<int>>& v);
+
 void batch_drop_graph(B* b) {
+    if (b->ms && (b->gtab[0] || b->gtab[1])) (void)hipStreamSynchronize(b->ms);  // tables in use by a replay
     for (int g = 0; g < 2; ++g) {
         if (b->gexec[g]) (void)hipGraphExecDestroy(b->gexec[g]);
         b->gexec[g] = nullptr;
         b->gin[g].clear();
         b->gout[g].clear();
+        b->fam[g].clear();
+        if (b->gtab[g]) (void)hipFree(b->gtab[g]);
+        b->gtab[g] = nullptr;
     }
+}
+
+bool same_launch(const KRec& a, const KRec& c) {
+    return a.go == c.go && a.g.x == c.g.x && a.g.y == c.g.y && a.b.x == c.b.x && a.b.y == c.b.y &&
+           a.b.z == c.b.z && a.shm == c.shm && a.e.size() == c.e.size();
+}
+
+// Record K steps of every problem (the launches they would enqueue), group
+// the problems whose launch sequences match into families, and upload one
+// table per (family, launch) of the problems' entries into slot g.  Every
+// problem's host state advances by K steps, as by a capture.  Returns 1
+// (nothing uploaded, states as on entry) where a launch cannot be grouped.
+int batch_group_prepare(B* b, int slot) {
+    const int n = (int)b->hs.size();
+    const auto before = batch_state(b);
+    std::vector<std::vector<KRec>> rec(n);
+    int st = PLES_MG_OK;
+    for (int i = 0; i < n && st == PLES_MG_OK; ++i) {
+        H* h = b->hs[i];
+        h->rec = &rec[i];
+        for (int k = 0; k < b->K && st == PLES_MG_OK; ++k) {
+            st = step_body(h, b->q[i]);
+            b->q[i] ^= 1;
+        }
+        h->rec = nullptr;
+    }
+    bool ok = st == PLES_MG_OK;
+    for (int i = 0; i < n && ok; ++i)
+        for (const KRec& r : rec[i]) ok = ok && r.go != nullptr;
+    if (!ok) {
+        set_batch_state(b, before);
+        return st != PLES_MG_OK ? st : 1;
+    }
+    std::vector<B::Fam>& fams = b->fam[slot];
+    fams.clear();
+    for (int i = 0; i < n; ++i) {
+        B::Fam* f = nullptr;
+        for (B::Fam& c : fams) {
+            const auto& r0 = rec[c.mem[0]];
+            if (r0.size() != rec[i].size()) continue;
+            bool eq = true;
+            for (size_t r = 0; r < r0.size() && eq; ++r) eq = same_launch(r0[r], rec[i][r]);
+            if (eq) {
+                f = &c;
+                break;
+            }
+        }
+        if (!f) {
+            fams.push_back(B::Fam{});
+            f = &fams.back();
+        }
+        f->mem.push_back(i);
+    }
+    size_t tot = 0;
+    for (B::Fam& f : fams) {
+        f.recs = rec[f.mem[0]];
+        f.off.resize(f.recs.size());
+        for (size_t r = 0; r < f.recs.size(); ++r) {
+            f.off[r] = tot;
+            tot += (f.recs[r].e.size() * f.mem.size() + 255) / 256 * 256;
+        }
+    }
+    std::vector<unsigned char> host(std::max<size_t>(tot, 256));
+    for (const B::Fam& f : fams)
+        for (size_t r = 0; r < f.recs.size(); ++r) {
+            const size_t es = f.recs[r].e.size();
+            for (size_t m = 0; m < f.mem.size(); ++m)
+                std::memcpy(host.data() + f.off[r] + m * es, rec[f.mem[m]][r].e.data(), es);
+        }
+    if (b->gtab[slot]) (void)hipFree(b->gtab[slot]);
+    b->gtab[slot] = nullptr;
+    CK(hipMalloc(&b->gtab[slot], host.size()));
+    CK(hipMemcpy(b->gtab[slot], host.data(), host.size(), hipMemcpyHostToDevice));
+    return PLES_MG_OK;
+}
+
+// the grouped K steps of slot g: each family on the stream of its first
+// problem (forked from and joined back into the batch stream), one launch
+// per recorded kernel over the family's problems, then the gather of all S
+int batch_group_body(B* b, int slot) {
+    const int n = (int)b->hs.size();
+    CK(hipEventRecord(b->fork, b->ms));
+    for (const B::Fam& f : b->fam[slot]) {
+        const int lead = f.mem[0];
+        hipStream_t s = b->hs[lead]->s;
+        CK(hipStreamWaitEvent(s, b->fork, 0));
+        for (size_t r = 0; r < f.recs.size(); ++r) {
+            const KRec& k = f.recs[r];
+            CK(k.go((const unsigned char*)b->gtab[slot] + f.off[r], (int)f.mem.size(), k.g, k.b, k.shm, s));
+        }
+        CK(hipEventRecord(b->join[lead], s));
+        CK(hipStreamWaitEvent(b->ms, b->join[lead], 0));
+    }
+    const int tot = n * S_COUNT;
+    hipLaunchKernelGGL(k_gather_S, dim3((tot + 255) / 256), dim3(256), 0, b->ms, (const double* const*)b->dS, n,
+                       b->Sall);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
 }
 
 // K steps of every problem on its own stream, forked from and joined back
@@ -2840,8 +2997,15 @@ int batch_enqueue_chunk(B* b, bool use_graph) {
             batch_drop_graph(b);
             g = 0;
         }
+        // grouped launches: recorded and tabled before the capture (which
+        // then leaves the states where the recording advanced them)
+        int grp = 1;
+        if (b->grouped) {
+            grp = batch_group_prepare(b, g);
+            if (grp != 1) CKS(grp);
+        }
         CK(hipStreamBeginCapture(b->ms, hipStreamCaptureModeGlobal));
-        const int st = batch_chunk_body(b);
+        const int st = grp == 0 ? batch_group_body(b, g) : batch_chunk_body(b);
         hipGraph_t gr = nullptr;
         const hipError_t e = hipStreamEndCapture(b->ms, &gr);
         if (st != PLES_MG_OK) {
@@ -2932,6 +3096,23 @@ int ples_mg_batch_destroy(ples_mg_batch* b) {
 }
 
 int ples_mg_batch_size(const ples_mg_batch* b) { return b ? (int)b->hs.size() : 0; }
+
+int ples_mg_batch_set_grouped(ples_mg_batch* b, int on) {
+    if (!b || (on != 0 && on != 1)) return PLES_MG_ERR_ARG;
+    if (b->grouped != (on == 1)) {
+        (void)hipSetDevice(b->dev);
+        batch_drop_graph(b);
+        b->grouped = on == 1;
+    }
+    return PLES_MG_OK;
+}
+
+int ples_mg_batch_families(const ples_mg_batch* b) {
+    if (!b) return PLES_MG_ERR_ARG;
+    for (int g = 0; g < 2; ++g)
+        if (!b->fam[g].empty()) return (int)b->fam[g].size();
+    return 0;
+}
 
 ples_mg_handle* ples_mg_batch_handle(ples_mg_batch* b, int i) {
     if (!b || i < 0 || i >= (int)b->hs.size()) return nullptr;

@@ -133,6 +133,8 @@ def lib():
         L.ples_mg_batch_device_bytes.argtypes = [C.c_void_p]
         L.ples_mg_batch_device_bytes.restype = C.c_size_t
         L.ples_mg_batch_solve.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.POINTER(_Result)]
+        L.ples_mg_batch_set_grouped.argtypes = [C.c_void_p, C.c_int]
+        L.ples_mg_batch_families.argtypes = [C.c_void_p]
         L.ples_mg_tiles.argtypes = [C.c_void_p, C.c_int] + [C.POINTER(C.c_int)] * 3
         L.ples_mg_copy_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
         L.ples_mg_read_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
@@ -621,9 +623,11 @@ class Solver:
 
 class Batch:
     """n independent problems solved together (ples_mg_batch_*).  problems:
-    list of dicts with x, y, z (node arrays), nlevel and optional ip, kp."""
+    list of dicts with x, y, z (node arrays), nlevel and optional ip, kp.
+    grouped: one launch per kernel over each family of same-shape problems
+    (the default), else every problem's kernels on its own stream."""
 
-    def __init__(self, problems, precision="mixed", **opts):
+    def __init__(self, problems, precision="mixed", grouped=True, **opts):
         self._probs = []
         grids = (_Grid * len(problems))()
         for q, pr in enumerate(problems):
@@ -637,6 +641,14 @@ class Batch:
         self._b = b
         self.solvers = [Solver._borrow(lib().ples_mg_batch_handle(b, q), *self._probs[q])
                         for q in range(len(problems))]
+        self.set_grouped(grouped)
+
+    def set_grouped(self, on: bool):
+        _check(lib().ples_mg_batch_set_grouped(self._b, 1 if on else 0), "ples_mg_batch_set_grouped")
+
+    def families(self) -> int:
+        """Families of the last grouped graph (0 before a grouped solve)."""
+        return int(lib().ples_mg_batch_families(self._b))
 
     def __len__(self):
         return len(self._probs)

@@ -1,8 +1,9 @@
 """Batched solves (ples_mg_batch_*): every problem of a batch of uneven
 sizes converges and its pressure is bitwise the one the same handle options
 give when the problem is solved alone, whatever the number of refinement
-steps per graph launch.  (The single solve itself is checked against the
-oracle in test_gpu_parity.)"""
+steps per graph launch, with grouped launches (one launch per kernel over the
+problems of one shape) and with every problem on its own stream.  (The single
+solve itself is checked against the oracle in test_gpu_parity.)"""
 import numpy as np
 import pytest
 
@@ -14,8 +15,8 @@ pytestmark = pytest.mark.gpu
 SIZES = [(32, 16, 32, 3), (48, 24, 48, 4), (64, 32, 64, 4), (32, 32, 32, 5), (16, 8, 16, 2), (64, 16, 32, 3)]
 
 
-def _problems():
-    return PC.c4_problems(n=6, seed0=2100, sizes=SIZES)
+def _problems(n=6):
+    return PC.c4_problems(n=n, seed0=2100, sizes=SIZES)
 
 
 def _alone(pr, precision, tol):
@@ -27,12 +28,14 @@ def _alone(pr, precision, tol):
         return r, s.get_fi()
 
 
-@pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_batch_equals_alone(precision):
-    probs = _problems()
+@pytest.mark.parametrize("precision,grouped", [("mixed", True), ("fp64", True), ("mixed", False)])
+def test_batch_equals_alone(precision, grouped):
+    probs = _problems(10)   # 10 problems over 6 shapes: some families hold several
+    shapes = {(p["shape"], p["nlevel"]) for p in probs}
+    assert len(shapes) < len(probs)
     tol = 1e-8
     ref = [_alone(pr, precision, tol) for pr in probs]
-    with mg.Batch(probs, precision=precision) as b:
+    with mg.Batch(probs, precision=precision, grouped=grouped) as b:
         for steps in (2, 1, 4):
             for pr, s in zip(probs, b.solvers):
                 s.case_rhs(pr["seed"])
@@ -42,6 +45,7 @@ def test_batch_equals_alone(precision):
                 assert r.ok and r.rel_res < tol, (steps, q, r)
                 assert r.cycles == ref[q][0].cycles, (steps, q, r.cycles, ref[q][0].cycles)
                 assert np.array_equal(s.get_fi(), ref[q][1]), (steps, q)
+            assert b.families() == (len(shapes) if grouped else 0), (steps, b.families())
 
 
 def test_batch_warm_restart_and_mixed_tolerances():

@@ -9,9 +9,10 @@ against the oracle (the CPU restatement of the reference):
     SOR, L = 6) to 1e-9;
   * c3, warped 512x256x512 (7 levels), warm start: the returned pressure's
     residual under the oracle's operator and metrics meets 1.5 x 1e-10;
-  * c4, the 64-problem batch: every pressure's oracle-operator residual
-    meets 1.5 x 1e-8, and the batch gives bitwise the results of solving
-    alone (a subset with the largest problem);
+  * c4, the 64-problem batch (grouped launches: one per kernel over each of
+    the five shapes): every pressure's oracle-operator residual meets
+    1.5 x 1e-8, and the batch gives bitwise the results of solving alone (a
+    subset with the largest problem);
   * c5, a shortened real-time serving stream (100 Poisson arrivals at
     200/s over 24 cases with 16 warm slots, so admission evicts): every
     request meets its tolerance, and the requests of three cases (with the
@@ -151,6 +152,7 @@ def test_c4_batch_residuals_and_batch_equals_alone():
         res = b.solve(tol, 200)
         fis = [s.get_fi() for s in b.solvers]
         rhss = [s.get_rhs() for s in b.solvers]
+        assert b.families() == len({(p["shape"], p["nlevel"]) for p in probs}) == 5
     for q, (p, r) in enumerate(zip(probs, res)):
         assert r.ok and r.rel_res < tol, (q, r)
         nx, ny, nz = p["shape"]
