@@ -425,7 +425,9 @@ def main():
     tuned = mg.tuning_load()  # committed launch-shape table of this GPU (else tuned live)
     copy_gbs = mg.copy_bandwidth(1 << 30, 20)
     read_gbs = mg.read_bandwidth(1 << 30, 20)
-    print(f"[bench] measured copy bandwidth {copy_gbs:.0f} GB/s, read {read_gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    streams_gbs = mg.streams_bandwidth(1 << 30, 20)
+    print(f"[bench] measured copy bandwidth {copy_gbs:.0f} GB/s, read {read_gbs:.0f} GB/s, "
+          f"12 streams {streams_gbs:.0f} GB/s", file=sys.stderr, flush=True)
 
     def from_zero(s):
         s.zero_fi()
@@ -569,6 +571,8 @@ def main():
             "copy_measured_GBs": copy_gbs,
             "read_measured_GBs": read_gbs,
             "frac_of_copy": achieved / copy_gbs if achieved else None,
+            "streams12_measured_GBs": streams_gbs,
+            "frac_of_streams12": achieved / streams_gbs if achieved else None,
             "launches": n,
             "avg_us": 1e3 * avg_ms,
             "bytes_per_launch": by / max(n, 1),

@@ -440,6 +440,10 @@ int ples_mg_tuning_clear(void);
  * keeps four independent 16-byte loads in flight. */
 int ples_mg_copy_bandwidth(int device, size_t bytes, int reps, double* gbs);
 int ples_mg_read_bandwidth(int device, size_t bytes, int reps, double* gbs);
+/* Twelve FP32 streams, eleven read and one written, 4-byte accesses (the
+ * array count and width of a fine-smoother row): GB/s of the `bytes` read +
+ * written per launch. */
+int ples_mg_streams_bandwidth(int device, size_t bytes, int reps, double* gbs);
 
 /* Launch shape of level `level` (1-based): smoother segments per column P,
  * V-cycle residual segments rP x lanes rbx (outputs may be NULL). */

@@ -546,9 +546,14 @@ struct YlineOtf {
 #ifndef PLES_YLINE_WPE_F16
 #define PLES_YLINE_WPE_F16 4
 #endif
-template <class T, class MV>
+// minimum waves per SIMD of a smoother variant: the FP32 ones as above; the
+// FP64 one with 16 segments of 16 rows (512-lane blocks, the 512x256x512
+// fine level) 4, which holds it at 128 VGPRs (at 130 it ran at 3 waves per
+// SIMD and 20% slower); the other FP64 ones unconstrained
+template <class T, class MV, int P = 1, int MR = 0>
 struct YlineWpe {
-    static constexpr int value = sizeof(T) == 4 ? (MVHalf<MV>::value ? PLES_YLINE_WPE_F16 : PLES_YLINE_WPE_F32) : 1;
+    static constexpr int value = sizeof(T) == 4 ? (MVHalf<MV>::value ? PLES_YLINE_WPE_F16 : PLES_YLINE_WPE_F32)
+                                                : ((P == 16 && MR == 16) ? 4 : 1);
 };
 // MV: the metric view, Met<T>, Met<_Float16> or the packed MetPk (X0 levels).
 template <int P, int MR, class T, class MV, class TR, bool X0 = false>
@@ -698,7 +703,7 @@ __device__ __forceinline__ void d_yline(Geo G, MV M, LineF F, const T* __restric
     yline_ghosts(G, M, pin, pout, i, k, q.j0 == 1, q.j1 == G.ny, e[0], pfin, zero_in);
 }
 template <int P, int MR, class T, class MV, class TR, bool X0 = false>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(YlineWpe<T, MV>::value)))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(YlineWpe<T, MV, P, MR>::value)))
 k_yline(Geo G, MV M, LineF F, const T* __restrict__ pin, T* __restrict__ pout, const TR* __restrict__ b, T omega,
         T inv_pot, int zero_in) {
     d_yline<P, MR, T, MV, TR, X0>(G, M, F, pin, pout, b, omega, inv_pot, zero_in);
@@ -2084,6 +2089,26 @@ __global__ void __launch_bounds__(256) k_read16(long n, const float4* __restrict
         for (int q = 0; q < 4; ++q) acc += (v[q].x + v[q].y) + (v[q].z + v[q].w);
     }
     out[(long)blockIdx.x * bw + threadIdx.x] = acc;
+}
+
+// Eleven FP32 streams read and one written with 4-byte accesses, one element
+// of each per lane and trip: the array count and access width of a row of the
+// fine smoother (iterate at three zeta planes, rhs, packed metrics, spikes,
+// output), without its arithmetic.  Bandwidth probe of that access pattern.
+struct Streams11 {
+    const float* a[11];
+};
+__global__ void __launch_bounds__(256) k_streams12(long n, Streams11 S, float* __restrict__ out) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+        float v[11];
+#pragma unroll
+        for (int q = 0; q < 11; ++q) v[q] = S.a[q][t];
+        float acc = 0.f;
+#pragma unroll
+        for (int q = 0; q < 11; ++q) acc += v[q];
+        out[t] = acc;
+    }
 }
 
 // elementwise helpers over the full ghosted array

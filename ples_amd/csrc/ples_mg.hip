@@ -467,7 +467,7 @@ void yline_go(H* h, Level& L, const Met<TM>& m, double inv_pot, const TR* rhs, i
     if constexpr (MR > 0) {
         auto go = [&](auto x0) {
             constexpr bool X0 = decltype(x0)::value;
-            KLAUNCH(h, yline, TPL(P, MR, T, Met<TM>, TR, X0), 512, (YlineWpe<T, Met<TM>>::value), cl.grid, cl.block,
+            KLAUNCH(h, yline, TPL(P, MR, T, Met<TM>, TR, X0), 512, (YlineWpe<T, Met<TM>, P, MR>::value), cl.grid, cl.block,
                     0, L.G, m, line_f(L), (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->omega_now,
                     (T)inv_pot, zin);
         };
@@ -477,7 +477,7 @@ void yline_go(H* h, Level& L, const Met<TM>& m, double inv_pot, const TR* rhs, i
         if constexpr (sizeof(TM) == 2) {
             if (L.x0 && L.pk_a && (P == 1 || L.vw16)) {
                 const MetPk mp{L.pk_a, L.pk_z, L.pk_b, L.g16[4]};
-                KLAUNCH(h, yline, TPL(P, MR, T, MetPk, TR, true), 512, (YlineWpe<T, MetPk>::value), cl.grid,
+                KLAUNCH(h, yline, TPL(P, MR, T, MetPk, TR, true), 512, (YlineWpe<T, MetPk, P, MR>::value), cl.grid,
                         cl.block, 0, L.G, mp, line_f(L), (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs,
                         (T)h->omega_now, (T)inv_pot, zin);
             } else if (L.x0) {
@@ -3768,6 +3768,42 @@ int ples_mg_server_stats(const ples_mg_server* sv, size_t* used_bytes, int* resi
 // reps launches timed with HIP events.  The measured streaming ceiling that
 // the stencil kernels' achieved bandwidth is read against.
 // copy (read + write bytes) or read-only probe of `bytes` bytes, reps launches
+// twelve-stream probe: bytes = read + written bytes of one launch
+static int probe_streams12(int device, size_t bytes, int reps, double* gbs) {
+    if (!gbs || bytes < 48 * 4096 || reps < 1) return PLES_MG_ERR_ARG;
+    CK(hipSetDevice(device));
+    const size_t n = bytes / 48;
+    float* buf = nullptr;
+    CK(hipMalloc((void**)&buf, 12 * n * sizeof(float)));
+    Streams11 S;
+    for (int q = 0; q < 11; ++q) S.a[q] = buf + (size_t)q * n;
+    float* out = buf + 11 * n;
+    hipStream_t s;
+    hipEvent_t e0, e1;
+    int st = PLES_MG_OK;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) st = PLES_MG_ERR_HIP;
+    if (st == PLES_MG_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        st = PLES_MG_ERR_HIP;
+    if (st == PLES_MG_OK) {
+        (void)hipMemsetAsync(buf, 0, 12 * n * sizeof(float), s);
+        const int nb = 8192, bt = 256;
+        auto launch = [&]() { hipLaunchKernelGGL(k_streams12, dim3(nb), dim3(bt), 0, s, (long)n, S, out); };
+        launch();
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps; ++r) launch();
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) st = PLES_MG_ERR_HIP;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        *gbs = 48.0 * (double)n * reps / (ms * 1e-3) / 1e9;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+    }
+    (void)hipFree(buf);
+    return st;
+}
+
 static int probe_bandwidth(int device, size_t bytes, int reps, bool read_only, double* gbs) {
     if (!gbs || bytes < 4096 || reps < 1) return PLES_MG_ERR_ARG;
     CK(hipSetDevice(device));
@@ -3937,6 +3973,10 @@ int ples_mg_project_rhs_noalloc(double* rhs, int jx, int jy, int jz, const doubl
 
 int ples_mg_copy_bandwidth(int device, size_t bytes, int reps, double* gbs) {
     return probe_bandwidth(device, bytes, reps, false, gbs);
+}
+
+int ples_mg_streams_bandwidth(int device, size_t bytes, int reps, double* gbs) {
+    return probe_streams12(device, bytes, reps, gbs);
 }
 
 int ples_mg_read_bandwidth(int device, size_t bytes, int reps, double* gbs) {

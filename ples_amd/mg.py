@@ -138,6 +138,7 @@ def lib():
         L.ples_mg_tiles.argtypes = [C.c_void_p, C.c_int] + [C.POINTER(C.c_int)] * 3
         L.ples_mg_copy_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
         L.ples_mg_read_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
+        L.ples_mg_streams_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
         L.ples_mg_tuning_load.argtypes = [C.c_char_p]
         L.ples_mg_restart_write.argtypes = [C.c_char_p] + [C.c_int] * 4 + [C.c_double] * 3 + [_dp] * 8
         L.ples_mg_restart_read.argtypes = [C.c_char_p] + [C.c_int] * 4 + [_dp] * 3 + [_dp] * 8
@@ -251,6 +252,13 @@ def read_bandwidth(nbytes=1 << 30, reps=20, device=0) -> float:
     """Measured device read bandwidth in GB/s (read-only sweep)."""
     g = C.c_double()
     _check(lib().ples_mg_read_bandwidth(device, nbytes, reps, C.byref(g)), "ples_mg_read_bandwidth")
+    return g.value
+
+
+def streams_bandwidth(nbytes=1 << 30, reps=20, device=0) -> float:
+    """Measured GB/s of eleven 4-byte FP32 read streams and one written stream."""
+    g = C.c_double()
+    _check(lib().ples_mg_streams_bandwidth(device, nbytes, reps, C.byref(g)), "ples_mg_streams_bandwidth")
     return g.value
 
 

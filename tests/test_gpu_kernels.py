@@ -188,3 +188,16 @@ def test_fp32_level_ops_match_oracle():
         ref = pf.copy()
         O.prolong(g, 1, ec, ref)
         assert _rel(out, ref) < 1e-6
+
+
+def test_bandwidth_probes():
+    """The roofline's measured ceilings: copy, read-only and twelve-stream
+    probes return rates below the HBM peak, the twelve-stream one (4-byte
+    accesses) at most the read-only one (16-byte accesses) plus noise."""
+    nbytes = 256 << 20
+    copy = mg.copy_bandwidth(nbytes, 5)
+    read = mg.read_bandwidth(nbytes, 5)
+    streams = mg.streams_bandwidth(nbytes, 5)
+    for v in (copy, read, streams):
+        assert 100.0 < v < 8000.0 * 1.05, (copy, read, streams)
+    assert streams < 1.1 * read, (streams, read)
