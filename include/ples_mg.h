@@ -368,6 +368,13 @@ int ples_mg_batch_families(const ples_mg_batch* b);
  * arrive while others run join at the next ples_mg_server_step; every running
  * request advances on its own stream (graph-replayed refinement steps), so
  * their kernels overlap on the device, and none waits for another to finish.
+ * A cold case's handle is created (and its rhs generated) on the server's
+ * worker thread, its bytes and slot reserved at once; the request is admitted
+ * at the first ples_mg_server_step after the creation lands, and the requests
+ * behind it are admitted meanwhile.  An evicted handle is destroyed on the
+ * same thread: its bytes count as free at eviction, so device memory in use
+ * can exceed hbm_budget by the evictions not yet freed.  ples_mg_server_wait
+ * also returns when a creation lands.
  * No reference counterpart: PLES solves one field per time step in-line. */
 typedef struct ples_mg_server ples_mg_server;
 typedef struct ples_mg_server_opts {

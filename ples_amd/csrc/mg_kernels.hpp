@@ -547,13 +547,13 @@ struct YlineOtf {
 #define PLES_YLINE_WPE_F16 4
 #endif
 // minimum waves per SIMD of a smoother variant: the FP32 ones as above; the
-// FP64 one with 16 segments of 16 rows (512-lane blocks, the 512x256x512
-// fine level) 4, which holds it at 128 VGPRs (at 130 it ran at 3 waves per
-// SIMD and 20% slower); the other FP64 ones unconstrained
-template <class T, class MV, int P = 1, int MR = 0>
+// FP64 X0 one with 16 segments of 16 rows (the 512x256x512 fine level) 4,
+// which holds it at 128 VGPRs (at 130 it ran at 3 waves per SIMD and 20%
+// slower); the other FP64 ones unconstrained (pinning them to 4 spills)
+template <class T, class MV, int P = 1, int MR = 0, bool X0 = false>
 struct YlineWpe {
     static constexpr int value = sizeof(T) == 4 ? (MVHalf<MV>::value ? PLES_YLINE_WPE_F16 : PLES_YLINE_WPE_F32)
-                                                : ((P == 16 && MR == 16) ? 4 : 1);
+                                                : ((P == 16 && MR == 16 && X0) ? 4 : 1);
 };
 // MV: the metric view, Met<T>, Met<_Float16> or the packed MetPk (X0 levels).
 template <int P, int MR, class T, class MV, class TR, bool X0 = false>
@@ -703,7 +703,7 @@ __device__ __forceinline__ void d_yline(Geo G, MV M, LineF F, const T* __restric
     yline_ghosts(G, M, pin, pout, i, k, q.j0 == 1, q.j1 == G.ny, e[0], pfin, zero_in);
 }
 template <int P, int MR, class T, class MV, class TR, bool X0 = false>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(YlineWpe<T, MV, P, MR>::value)))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(YlineWpe<T, MV, P, MR, X0>::value)))
 k_yline(Geo G, MV M, LineF F, const T* __restrict__ pin, T* __restrict__ pout, const TR* __restrict__ b, T omega,
         T inv_pot, int zero_in) {
     d_yline<P, MR, T, MV, TR, X0>(G, M, F, pin, pout, b, omega, inv_pot, zero_in);

@@ -5,7 +5,12 @@
 #include <algorithm>
 #include <array>
 #include <map>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <memory>
 #include <mutex>
+#include <thread>
 #include <chrono>
 #include <deque>
 #include <unordered_map>
@@ -467,7 +472,7 @@ void yline_go(H* h, Level& L, const Met<TM>& m, double inv_pot, const TR* rhs, i
     if constexpr (MR > 0) {
         auto go = [&](auto x0) {
             constexpr bool X0 = decltype(x0)::value;
-            KLAUNCH(h, yline, TPL(P, MR, T, Met<TM>, TR, X0), 512, (YlineWpe<T, Met<TM>, P, MR>::value), cl.grid, cl.block,
+            KLAUNCH(h, yline, TPL(P, MR, T, Met<TM>, TR, X0), 512, (YlineWpe<T, Met<TM>, P, MR, X0>::value), cl.grid, cl.block,
                     0, L.G, m, line_f(L), (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs, (T)h->omega_now,
                     (T)inv_pot, zin);
         };
@@ -477,7 +482,7 @@ void yline_go(H* h, Level& L, const Met<TM>& m, double inv_pot, const TR* rhs, i
         if constexpr (sizeof(TM) == 2) {
             if (L.x0 && L.pk_a && (P == 1 || L.vw16)) {
                 const MetPk mp{L.pk_a, L.pk_z, L.pk_b, L.g16[4]};
-                KLAUNCH(h, yline, TPL(P, MR, T, MetPk, TR, true), 512, (YlineWpe<T, MetPk, P, MR>::value), cl.grid,
+                KLAUNCH(h, yline, TPL(P, MR, T, MetPk, TR, true), 512, (YlineWpe<T, MetPk, P, MR, true>::value), cl.grid,
                         cl.block, 0, L.G, mp, line_f(L), (const T*)L.p[L.cur], (T*)L.p[1 - L.cur], rhs,
                         (T)h->omega_now, (T)inv_pot, zin);
             } else if (L.x0) {
@@ -1449,6 +1454,47 @@ int check_grid(const ples_mg_grid* g) {
 // solution of A p = b - mean(b).  Stored: E K, E the map from the interior to
 // the ghosted vector (ghosts by the same fixed point), so one matrix-vector
 // product gives the whole ghosted solution.
+// Process-wide cache of dense coarsest-level maps: the map depends only on
+// the level's shape, node stride and metrics, so a handle re-created for the
+// same grid (a served case admitted again after its eviction, a batch of
+// equal problems) takes the stored map instead of rebuilding it on the host
+// (tens of milliseconds).  Keyed by the shape and stride; a hit also compares
+// the metric bytes, so the map used is always the one these metrics give.
+struct DenseEntry {
+    int nx, ny, nz, ip, jp, kp;
+    double pot;
+    std::vector<double> met;  // the nine metric arrays, concatenated
+    std::vector<double> map;
+};
+std::mutex g_dense_mu;
+std::deque<DenseEntry> g_dense;  // oldest first
+size_t g_dense_bytes = 0;
+constexpr size_t kDenseCacheMax = size_t(1) << 30;
+
+bool dense_lookup(const Geo& G, double pot, const std::vector<double>& met, std::vector<double>& out) {
+    std::lock_guard<std::mutex> lk(g_dense_mu);
+    for (const DenseEntry& e : g_dense)
+        if (e.nx == G.nx && e.ny == G.ny && e.nz == G.nz && e.ip == G.ip && e.jp == G.jp && e.kp == G.kp &&
+            e.pot == pot && e.met.size() == met.size() &&
+            std::memcmp(e.met.data(), met.data(), met.size() * sizeof(double)) == 0) {
+            out = e.map;
+            return true;
+        }
+    return false;
+}
+
+void dense_store(const Geo& G, double pot, std::vector<double> met, const std::vector<double>& map) {
+    const size_t b = (met.size() + map.size()) * sizeof(double);
+    if (b > kDenseCacheMax) return;
+    std::lock_guard<std::mutex> lk(g_dense_mu);
+    while (!g_dense.empty() && g_dense_bytes + b > kDenseCacheMax) {
+        g_dense_bytes -= (g_dense.front().met.size() + g_dense.front().map.size()) * sizeof(double);
+        g_dense.pop_front();
+    }
+    g_dense.push_back(DenseEntry{G.nx, G.ny, G.nz, G.ip, G.jp, G.kp, pot, std::move(met), map});
+    g_dense_bytes += b;
+}
+
 int build_coarse_dense(H* h, Level& L) {
     const Geo& G = L.G;
     const long N = L.ncell(), ng = G.n;
@@ -1456,6 +1502,18 @@ int build_coarse_dense(H* h, Level& L) {
     for (int q = 0; q < 9; ++q)
         CK(hipMemcpyAsync(gm[q].data(), L.g64[q], (size_t)ng * 8, hipMemcpyDeviceToHost, h->s));
     CK(hipStreamSynchronize(h->s));
+    std::vector<double> metcat;
+    metcat.reserve((size_t)(9 * ng));
+    for (int q = 0; q < 9; ++q) metcat.insert(metcat.end(), gm[q].begin(), gm[q].end());
+    {
+        std::vector<double> cached;
+        if (dense_lookup(G, L.pot, metcat, cached)) {
+            CKS(dalloc(h, (void**)&L.kd, (size_t)(ng * N) * 8));
+            CK(hipMemcpyAsync(L.kd, cached.data(), (size_t)(ng * N) * 8, hipMemcpyHostToDevice, h->s));
+            CK(hipStreamSynchronize(h->s));
+            return PLES_MG_OK;
+        }
+    }
     Met<double> M;
     for (int q = 0; q < 9; ++q) M.g[q] = gm[q].data();
     BCs bc0;
@@ -1541,6 +1599,7 @@ int build_coarse_dense(H* h, Level& L) {
     }
     for (double x : KF)
         if (!std::isfinite(x)) return PLES_MG_ERR_NONFINITE;
+    dense_store(G, L.pot, std::move(metcat), KF);
     CKS(dalloc(h, (void**)&L.kd, (size_t)(ng * N) * 8));
     CK(hipMemcpyAsync(L.kd, KF.data(), (size_t)(ng * N) * 8, hipMemcpyHostToDevice, h->s));
     CK(hipStreamSynchronize(h->s));
@@ -1792,9 +1851,9 @@ int ples_mg_destroy(ples_mg_handle* h) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
     }
-    for (auto& a : h->allocs) (void)hipFree(a.first);
-    drop_graphs(h);
+    drop_graphs(h);  // before the buffers they reference
     drop_graphs32(h);
+    for (auto& a : h->allocs) (void)hipFree(a.first);
     for (int q = 0; q < 4; ++q)
         if (h->sev[q]) (void)hipEventDestroy(h->sev[q]);
     if (h->Sh) (void)hipHostFree(h->Sh);
@@ -3340,6 +3399,13 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// a handle being created for a cold start on the server's worker thread
+struct Creation {
+    std::atomic<bool> done{false};
+    ples_mg_handle* h = nullptr;
+    int status = PLES_MG_OK;
+};
+
 struct SCase {
     uint64_t id = 0;
     int jx = 0, jy = 0, jz = 0, ip = 0, kp = 0, nlevel = 1;
@@ -3347,6 +3413,7 @@ struct SCase {
     uint64_t rhs_seed = 0;
     double dt = 1.0;
     H* h = nullptr;               // resident handle (nullptr = cold)
+    std::shared_ptr<Creation> creating;  // cold start in flight (its est already counted in used)
     size_t est = 0;               // device bytes when resident
     bool busy = false;
     bool solved = false;          // the resident iterate holds a previous solution
@@ -3384,6 +3451,17 @@ struct ples_mg_server {
     uint64_t clock = 0;
     size_t used = 0;
     long completed = 0, evictions = 0, cold_starts = 0;
+    // worker thread: creates the handles of cold starts and destroys evicted
+    // ones, so neither the create (tens of ms) nor the frees of an eviction
+    // (hipFree of the large buffers: up to 0.7 ms each) stall the requests
+    // the serving loop keeps running
+    std::thread worker;
+    std::mutex mu;
+    std::condition_variable cv;       // jobs queued / stop
+    std::condition_variable landed;   // a job finished
+    std::deque<std::function<void()>> jobs;   // creations: run first
+    std::deque<std::function<void()>> frees;  // destructions of evicted handles
+    bool stop = false;
 };
 
 namespace {
@@ -3399,10 +3477,48 @@ ples_mg_grid case_grid(const SCase& c) {
     return g;
 }
 
+void post(SV* sv, std::function<void()> job, bool is_free = false) {
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        (is_free ? sv->frees : sv->jobs).push_back(std::move(job));
+    }
+    sv->cv.notify_one();
+}
+
+void worker_loop(SV* sv) {
+    for (;;) {
+        std::function<void()> job;
+        {
+            std::unique_lock<std::mutex> lk(sv->mu);
+            sv->cv.wait(lk, [&] { return sv->stop || !sv->jobs.empty() || !sv->frees.empty(); });
+            // a cold start waiting is served before any queued destruction
+            std::deque<std::function<void()>>& q = !sv->jobs.empty() ? sv->jobs : sv->frees;
+            if (q.empty()) return;  // stop requested and nothing left
+            job = std::move(q.front());
+            q.pop_front();
+        }
+        job();
+        {  // a waiter checks the jobs' flags under mu: take it once so it cannot miss this notification
+            std::lock_guard<std::mutex> lk(sv->mu);
+        }
+        sv->landed.notify_all();
+    }
+}
+
+// an idle case's handle leaves: its bytes are free for admission at once,
+// the handle is destroyed on the worker thread (its stream is idle)
 void evict(SV* sv, SCase& c) {
     if (!c.h) return;
     sv->used -= c.est;
-    ples_mg_destroy(c.h);
+    H* h = c.h;
+    const int dev = sv->so.device;
+    post(
+        sv,
+        [h, dev] {
+            (void)hipSetDevice(dev);
+            ples_mg_destroy(h);
+        },
+        true);
     c.h = nullptr;
     c.solved = false;
     ++sv->evictions;
@@ -3410,7 +3526,7 @@ void evict(SV* sv, SCase& c) {
 
 int resident_count(const SV* sv) {
     int n = 0;
-    for (const auto& kv : sv->cases) n += kv.second.h != nullptr;
+    for (const auto& kv : sv->cases) n += kv.second.h != nullptr || kv.second.creating != nullptr;
     return n;
 }
 
@@ -3437,19 +3553,47 @@ int try_admit(SV* sv, SReq& r) {
     if (c.busy) return 0;
     if ((int)sv->active.size() >= sv->so.max_active) return 0;
     if (!c.h) {
-        if (!make_room(sv, c.est, true)) return 0;
-        ples_mg_grid g = case_grid(c);
-        ples_mg_handle* h = nullptr;
-        ples_mg_opts o = sv->o;
-        o.device = sv->so.device;
-        const int st = ples_mg_create(&g, &o, &h);
-        if (st == PLES_MG_ERR_OOM) return 0;
-        if (st != PLES_MG_OK) return st;
-        c.h = h;
-        c.est = std::max(c.est, h->bytes);
-        sv->used += c.est;
-        CKS(ples_mg_case_rhs(h, c.rhs_seed, c.dt));
-        ++sv->cold_starts;
+        if (c.creating) {  // cold start in flight on the worker
+            if (!c.creating->done.load(std::memory_order_acquire)) return 0;
+            const std::shared_ptr<Creation> job = std::move(c.creating);
+            if (job->status != PLES_MG_OK) {
+                sv->used -= c.est;
+                return job->status == PLES_MG_ERR_OOM ? 0 : job->status;  // out of memory: retried later
+            }
+            c.h = job->h;
+            if (c.h->bytes > c.est) {
+                sv->used += c.h->bytes - c.est;
+                c.est = c.h->bytes;
+            }
+            ++sv->cold_starts;
+        } else {
+            // reserve the bytes and the slot, create (and generate the rhs) on the worker
+            if (!make_room(sv, c.est, true)) return 0;
+            sv->used += c.est;
+            auto job = std::make_shared<Creation>();
+            c.creating = job;
+            ples_mg_opts o = sv->o;
+            o.device = sv->so.device;
+            const SCase* cp = &c;  // cases are never removed: the node arrays outlive the job
+            post(sv, [job, cp, o] {
+                (void)hipSetDevice(o.device);
+                ples_mg_grid g = case_grid(*cp);
+                ples_mg_handle* h = nullptr;
+                int st = ples_mg_create(&g, &o, &h);
+                if (st == PLES_MG_OK) {
+                    st = ples_mg_case_rhs(h, cp->rhs_seed, cp->dt);
+                    if (st == PLES_MG_OK && hipStreamSynchronize(h->s) != hipSuccess) st = PLES_MG_ERR_HIP;
+                    if (st != PLES_MG_OK) {
+                        ples_mg_destroy(h);
+                        h = nullptr;
+                    }
+                }
+                job->h = h;
+                job->status = st;
+                job->done.store(true, std::memory_order_release);
+            });
+            return 0;
+        }
     }
     if (!c.seed_fi.empty()) {  // pressure given by the caller (e.g. read from a restart file)
         CKS(ples_mg_set_fi(c.h, c.seed_fi.data()));
@@ -3619,14 +3763,24 @@ int ples_mg_server_create(const ples_mg_server_opts* so, const ples_mg_opts* opt
         }
         sv->so.hbm_budget = (size_t)(0.8 * (double)fr);
     }
+    sv->worker = std::thread(worker_loop, sv);
     *out = sv;
     return PLES_MG_OK;
 }
 
 int ples_mg_server_destroy(ples_mg_server* sv) {
     if (!sv) return PLES_MG_OK;
-    for (auto& kv : sv->cases)
+    {  // finish the queued creations and destructions, then stop the worker
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv.notify_all();
+    if (sv->worker.joinable()) sv->worker.join();
+    (void)hipSetDevice(sv->so.device);
+    for (auto& kv : sv->cases) {
         if (kv.second.h) ples_mg_destroy(kv.second.h);
+        if (kv.second.creating && kv.second.creating->h) ples_mg_destroy(kv.second.creating->h);
+    }
     for (auto& kv : sv->reqs)
         if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
     delete sv;
@@ -3702,7 +3856,9 @@ int ples_mg_server_step(ples_mg_server* sv, int* n_done) {
 
 int ples_mg_server_wait(ples_mg_server* sv) {
     if (!sv) return PLES_MG_ERR_ARG;
-    // block on the oldest outstanding read-back of any running request
+    CK(hipSetDevice(sv->so.device));
+    // block on the oldest outstanding read-back of any running request, else
+    // on the next cold start the worker finishes
     for (int64_t t : sv->active) {
         SReq& r = sv->reqs.at(t);
         H* h = sv->cases.at(r.case_id).h;
@@ -3715,6 +3871,17 @@ int ples_mg_server_wait(ples_mg_server* sv) {
             return PLES_MG_OK;
         }
     }
+    auto creation_landed = [&] {
+        bool any = false;
+        for (const auto& kv : sv->cases)
+            if (kv.second.creating) {
+                any = true;
+                if (kv.second.creating->done.load(std::memory_order_acquire)) return true;
+            }
+        return !any;
+    };
+    std::unique_lock<std::mutex> lk(sv->mu);
+    sv->landed.wait(lk, creation_landed);
     return PLES_MG_OK;
 }
 
