@@ -135,18 +135,21 @@ __host__ __device__ inline void win_init(Win<T>& w, const P* __restrict__ p, lon
 }
 
 // load row j+1 of the five face columns and row j of the four diagonals;
-// c = idx(i,j,k)
-template <class T, class P>
+// c = idx(i,j,k).  X0: the diagonals only enter through g13 and g31, which
+// vanish on the level, so they are not loaded.
+template <bool X0 = false, class T, class P>
 __host__ __device__ inline void win_load(Win<T>& w, const P* __restrict__ p, long c, long sy, long sz) {
     w.pp = (T)p[c + sy];
     w.xmp = (T)p[c - 1 + sy];
     w.xpp = (T)p[c + 1 + sy];
     w.zmp = (T)p[c - sz + sy];
     w.zpp = (T)p[c + sz + sy];
-    w.dmm = (T)p[c - 1 - sz];
-    w.dpm = (T)p[c + 1 - sz];
-    w.dmp = (T)p[c - 1 + sz];
-    w.dpp = (T)p[c + 1 + sz];
+    if constexpr (!X0) {
+        w.dmm = (T)p[c - 1 - sz];
+        w.dpm = (T)p[c + 1 - sz];
+        w.dmp = (T)p[c - 1 + sz];
+        w.dpp = (T)p[c + 1 + sz];
+    }
 }
 
 template <class T>
@@ -169,34 +172,73 @@ struct G2 {
 // xi-zeta cross metrics g13, g31 are identically zero (orthogonal xi and
 // zeta grid lines, as on terrain-following grids over x-z separable meshes),
 // so they are not loaded.
+// Face fluxes of cell (i,j,k) from the window and the metrics of its six
+// faces (g: the cell's own high faces, h: the low xi and zeta faces, stored
+// at c-1 and c-sz).  X0 drops the g13/g31 terms (and their diagonal values).
+template <bool X0, class T>
+__host__ __device__ inline T flux_xi_hi(const Win<T>& w, T g11, T g12, T g13, T inv_pot) {
+    const T q = (T)0.25;
+    if constexpr (X0) return (g11 * (w.xp0 - w.p0) + q * (g12 * (w.xpp + w.pp - w.xpm - w.pm))) * inv_pot;
+    else
+        return (g11 * (w.xp0 - w.p0) +
+                q * (g12 * (w.xpp + w.pp - w.xpm - w.pm) + g13 * (w.dpp + w.zp0 - w.dpm - w.zm0))) *
+               inv_pot;
+}
+template <bool X0, class T>
+__host__ __device__ inline T flux_xi_lo(const Win<T>& w, T h11, T h12, T h13, T inv_pot) {
+    const T q = (T)0.25;
+    if constexpr (X0) return (h11 * (w.p0 - w.xm0) + q * (h12 * (w.pp + w.xmp - w.pm - w.xmm))) * inv_pot;
+    else
+        return (h11 * (w.p0 - w.xm0) +
+                q * (h12 * (w.pp + w.xmp - w.pm - w.xmm) + h13 * (w.zp0 + w.dmp - w.zm0 - w.dmm))) *
+               inv_pot;
+}
+template <class T>
+__host__ __device__ inline T flux_eta_hi(const Win<T>& w, T g21, T g22, T g23, T inv_pot) {
+    const T q = (T)0.25;
+    return (g22 * (w.pp - w.p0) +
+            q * (g21 * (w.xpp + w.xp0 - w.xmp - w.xm0) + g23 * (w.zpp + w.zp0 - w.zmp - w.zm0))) *
+           inv_pot;
+}
+template <class T>
+__host__ __device__ inline T flux_eta_lo(const Win<T>& w, T g21, T g22, T g23, T inv_pot) {
+    const T q = (T)0.25;
+    return (g22 * (w.p0 - w.pm) +
+            q * (g21 * (w.xp0 + w.xpm - w.xm0 - w.xmm) + g23 * (w.zp0 + w.zpm - w.zm0 - w.zmm))) *
+           inv_pot;
+}
+template <bool X0, class T>
+__host__ __device__ inline T flux_zeta_hi(const Win<T>& w, T g31, T g32, T g33, T inv_pot) {
+    const T q = (T)0.25;
+    if constexpr (X0) return (g33 * (w.zp0 - w.p0) + q * (g32 * (w.zpp + w.pp - w.zpm - w.pm))) * inv_pot;
+    else
+        return (g33 * (w.zp0 - w.p0) +
+                q * (g31 * (w.dpp + w.xp0 - w.dmp - w.xm0) + g32 * (w.zpp + w.pp - w.zpm - w.pm))) *
+               inv_pot;
+}
+template <bool X0, class T>
+__host__ __device__ inline T flux_zeta_lo(const Win<T>& w, T h31, T h32, T h33, T inv_pot) {
+    const T q = (T)0.25;
+    if constexpr (X0) return (h33 * (w.p0 - w.zm0) + q * (h32 * (w.pp + w.zmp - w.pm - w.zmm))) * inv_pot;
+    else
+        return (h33 * (w.p0 - w.zm0) +
+                q * (h31 * (w.xp0 + w.dpm - w.xm0 - w.dmm) + h32 * (w.pp + w.zmp - w.pm - w.zmm))) *
+               inv_pot;
+}
+
 template <bool X0 = false, class T, class MV>
 __host__ __device__ inline T stencil(const Win<T>& w, const MV& M, long c, long sz, const G2<T>& gm,
                             const G2<T>& g0, T inv_pot) {
-    const T q = (T)0.25;
     const T g11 = (T)mg(M, 0, c), g12 = (T)mg(M, 1, c), g13 = X0 ? (T)0 : (T)mg(M, 2, c);
     const T h11 = (T)mg(M, 0, c - 1), h12 = (T)mg(M, 1, c - 1), h13 = X0 ? (T)0 : (T)mg(M, 2, c - 1);
     const T g31 = X0 ? (T)0 : (T)mg(M, 6, c), g32 = (T)mg(M, 7, c), g33 = (T)mg(M, 8, c);
     const T h31 = X0 ? (T)0 : (T)mg(M, 6, c - sz), h32 = (T)mg(M, 7, c - sz), h33 = (T)mg(M, 8, c - sz);
-    T res_dx = (g11 * (w.xp0 - w.p0) +
-                q * (g12 * (w.xpp + w.pp - w.xpm - w.pm) + g13 * (w.dpp + w.zp0 - w.dpm - w.zm0))) *
-               inv_pot;
-    T res_sn = (h11 * (w.p0 - w.xm0) +
-                q * (h12 * (w.pp + w.xmp - w.pm - w.xmm) + h13 * (w.zp0 + w.dmp - w.zm0 - w.dmm))) *
-               inv_pot;
-    T res_sop = (g0.g22 * (w.pp - w.p0) +
-                 q * (g0.g21 * (w.xpp + w.xp0 - w.xmp - w.xm0) +
-                      g0.g23 * (w.zpp + w.zp0 - w.zmp - w.zm0))) *
-                inv_pot;
-    T res_sot = (gm.g22 * (w.p0 - w.pm) +
-                 q * (gm.g21 * (w.xp0 + w.xpm - w.xm0 - w.xmm) +
-                      gm.g23 * (w.zp0 + w.zpm - w.zm0 - w.zmm))) *
-                inv_pot;
-    T res_av = (g33 * (w.zp0 - w.p0) +
-                q * (g31 * (w.dpp + w.xp0 - w.dmp - w.xm0) + g32 * (w.zpp + w.pp - w.zpm - w.pm))) *
-               inv_pot;
-    T res_ind = (h33 * (w.p0 - w.zm0) +
-                 q * (h31 * (w.xp0 + w.dpm - w.xm0 - w.dmm) + h32 * (w.pp + w.zmp - w.pm - w.zmm))) *
-                inv_pot;
+    const T res_dx = flux_xi_hi<X0>(w, g11, g12, g13, inv_pot);
+    const T res_sn = flux_xi_lo<X0>(w, h11, h12, h13, inv_pot);
+    const T res_sop = flux_eta_hi(w, g0.g21, g0.g22, g0.g23, inv_pot);
+    const T res_sot = flux_eta_lo(w, gm.g21, gm.g22, gm.g23, inv_pot);
+    const T res_av = flux_zeta_hi<X0>(w, g31, g32, g33, inv_pot);
+    const T res_ind = flux_zeta_lo<X0>(w, h31, h32, h33, inv_pot);
     return (res_dx - res_sn + res_sop - res_sot + res_av - res_ind) * inv_pot;
 }
 
@@ -318,6 +360,7 @@ __device__ inline void reduced_solve(T (&red)[8][P][BX], int tx, int s, T& uLm, 
     // red[0..1][t] (Y) -> G_t -> x_t, red[2..3][t] (vF, vL) and red[6..7][t]
     // -> D_t^{-1}; red[4..5][t] keep (wF, wL).
     if (s == 0) {
+#pragma unroll 1
         for (int t = 0; t < P; ++t) {
             const T yF = red[0][t][tx], yL = red[1][t][tx];
             const T vF = red[2][t][tx], vL = red[3][t][tx];
@@ -341,6 +384,7 @@ __device__ inline void reduced_solve(T (&red)[8][P][BX], int tx, int s, T& uLm, 
             red[7][t][tx] = d00 / det;   // Dinv11
         }
         T xF = 0;
+#pragma unroll 1
         for (int t = P - 1; t >= 0; --t) {
             T g0 = red[0][t][tx], g1 = red[1][t][tx];
             if (t < P - 1) {
@@ -555,13 +599,139 @@ struct YlineWpe {
     static constexpr int value = sizeof(T) == 4 ? (MVHalf<MV>::value ? PLES_YLINE_WPE_F16 : PLES_YLINE_WPE_F32)
                                                 : ((P == 16 && MR == 16 && X0) ? 4 : 1);
 };
+// Metrics of one cell row as the smoother reads them: the cell's high faces
+// (g), its low xi and zeta faces (h: stored at c-1, k: stored at c-sz), kept
+// in their storage type until used.  One row is loaded ahead of its use.
+template <class MV, bool X0>
+struct MRow;
+template <class TM, bool X0>
+struct MRow<Met<TM>, X0> {
+    TM g11, g12, g13, h11, h12, h13, g21, g22, g23, g31, g32, g33, k31, k32, k33;
+    __device__ void load(const Met<TM>& M, long c, long sz) {
+        g11 = M.g[0][c]; g12 = M.g[1][c]; h11 = M.g[0][c - 1]; h12 = M.g[1][c - 1];
+        g21 = M.g[3][c]; g22 = M.g[4][c]; g23 = M.g[5][c];
+        g32 = M.g[7][c]; g33 = M.g[8][c]; k32 = M.g[7][c - sz]; k33 = M.g[8][c - sz];
+        if constexpr (!X0) {
+            g13 = M.g[2][c]; h13 = M.g[2][c - 1]; g31 = M.g[6][c]; k31 = M.g[6][c - sz];
+        } else {
+            g13 = h13 = g31 = k31 = (TM)0;
+        }
+    }
+};
+template <bool X0>
+struct MRow<MetPk, X0> {
+    H2 a, am, z, zm, b;
+    _Float16 g22v;
+    __device__ void load(const MetPk& M, long c, long sz) {
+        a = M.a[c]; am = M.a[c - 1]; z = M.z[c]; zm = M.z[c - sz]; b = M.b[c]; g22v = M.g22[c];
+    }
+};
+template <class T, class TM, bool X0>
+__device__ inline void mrow_get(const MRow<Met<TM>, X0>& r, T (&o)[15]) {
+    o[0] = (T)r.g11; o[1] = (T)r.g12; o[2] = (T)r.g13; o[3] = (T)r.h11; o[4] = (T)r.h12; o[5] = (T)r.h13;
+    o[6] = (T)r.g21; o[7] = (T)r.g22; o[8] = (T)r.g23; o[9] = (T)r.g31; o[10] = (T)r.g32; o[11] = (T)r.g33;
+    o[12] = (T)r.k31; o[13] = (T)r.k32; o[14] = (T)r.k33;
+}
+template <class T, bool X0>
+__device__ inline void mrow_get(const MRow<MetPk, X0>& r, T (&o)[15]) {
+    o[0] = (T)r.a.x; o[1] = (T)r.a.y; o[2] = (T)0; o[3] = (T)r.am.x; o[4] = (T)r.am.y; o[5] = (T)0;
+    o[6] = (T)r.b.x; o[7] = (T)r.g22v; o[8] = (T)r.b.y; o[9] = (T)0; o[10] = (T)r.z.x; o[11] = (T)r.z.y;
+    o[12] = (T)0; o[13] = (T)r.zm.x; o[14] = (T)r.zm.y;
+}
+
+// everything the forward pass reads for one row: iterate row j+1 of the five
+// face columns (and row j of the diagonals without X0), rhs, metrics and, for
+// the variants with stored factors, invm and c'
+template <class T, class MV, class TR, bool X0, bool OTF>
+struct RowIn {
+    T pp, xmp, xpp, zmp, zpp, dmm, dpm, dmp, dpp;
+    TR rhs;
+    float invm;
+    MRow<MV, X0> m;
+    __device__ void load(const T* __restrict__ p, const TR* __restrict__ b, const MV& M, const LineF& F, long c,
+                         long sy, long sz) {
+        pp = p[c + sy]; xmp = p[c - 1 + sy]; xpp = p[c + 1 + sy]; zmp = p[c - sz + sy]; zpp = p[c + sz + sy];
+        if constexpr (!X0) {
+            dmm = p[c - 1 - sz]; dpm = p[c + 1 - sz]; dmp = p[c - 1 + sz]; dpp = p[c + 1 + sz];
+        }
+        rhs = b[c];
+        if constexpr (!OTF) invm = F.invm[c];
+        m.load(M, c, sz);
+    }
+};
+
+// s * the face-flux differences of the next row of a column (the stencil
+// of fsrc/resid_sndrcv.f03:85-147 without the "- rh"): the window is advanced
+// to it with the loaded row, Flo holds the flux through its lower eta face
+// (the upper-face flux of the row before) and is advanced; m gets the row's
+// metrics.
+template <bool X0, class T, class MV, class TR, bool OTF>
+__device__ inline T row_apply(Win<T>& w, const RowIn<T, MV, TR, X0, OTF>& cur, T& Flo, T inv_pot, T (&m)[15]) {
+    w.pp = cur.pp; w.xmp = cur.xmp; w.xpp = cur.xpp; w.zmp = cur.zmp; w.zpp = cur.zpp;
+    if constexpr (!X0) {
+        w.dmm = cur.dmm; w.dpm = cur.dpm; w.dmp = cur.dmp; w.dpp = cur.dpp;
+    }
+    mrow_get<T>(cur.m, m);
+    const T Fhi = flux_eta_hi(w, m[6], m[7], m[8], inv_pot);
+    const T ap = (flux_xi_hi<X0>(w, m[0], m[1], m[2], inv_pot) - flux_xi_lo<X0>(w, m[3], m[4], m[5], inv_pot) +
+                  Fhi - Flo + flux_zeta_hi<X0>(w, m[9], m[10], m[11], inv_pot) -
+                  flux_zeta_lo<X0>(w, m[12], m[13], m[14], inv_pot)) *
+                 inv_pot;
+    Flo = Fhi;
+    return ap;
+}
+
+// reciprocal of a line pivot: the hardware reciprocal in FP32 (1 ulp: the
+// smoother is a preconditioner), a division in FP64
+template <class T>
+__device__ inline T pivot_inv(T x) {
+    if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(x);
+    else return (T)1 / x;
+}
+
+// spike pair (v_j, w_j) of row c in the type the variant reads
+template <class MV>
+struct SpikeRow {
+    float v, w;
+    __device__ void load(const LineF& F, long c) { v = F.v[c]; w = F.w[c]; }
+    __device__ float vv() const { return v; }
+    __device__ float ww() const { return w; }
+};
+template <>
+struct SpikeRow<Met<_Float16>> {
+    _Float16 v, w;
+    __device__ void load(const LineF& F, long c) { v = F.v16[c]; w = F.w16[c]; }
+    __device__ float vv() const { return (float)v; }
+    __device__ float ww() const { return (float)w; }
+};
+template <>
+struct SpikeRow<MetPk> {
+    H2 vw;
+    __device__ void load(const LineF& F, long c) { vw = F.vw16[c]; }
+    __device__ float vv() const { return (float)vw.x; }
+    __device__ float ww() const { return (float)vw.y; }
+};
+
+// loads of the next row issued before a row is computed: where the two rows'
+// registers fit the variant's budget without spilling (FP32 with packed
+// 16-bit metrics, up to 8 segments)
+template <int P, int MR, class T, class MV>
+struct YlinePf {
+    static constexpr bool value = sizeof(T) == 4 && std::is_same<MV, MetPk>::value && P <= 8;
+};
+
 // MV: the metric view, Met<T>, Met<_Float16> or the packed MetPk (X0 levels).
+// The forward pass keeps the loads of the next row in flight while a row is
+// computed, carries the eta flux through a row's upper face into the next
+// row (it is that row's lower-face flux), and the last pass loads all the
+// segment's spikes before its first store, so no load of that pass waits on
+// a store.
 template <int P, int MR, class T, class MV, class TR, bool X0 = false>
 __device__ __forceinline__ void d_yline(Geo G, MV M, LineF F, const T* __restrict__ pin, T* __restrict__ pout,
                                         const TR* __restrict__ b, T omega, T inv_pot, int zero_in) {
-    constexpr bool PK = std::is_same<MV, MetPk>::value;
     constexpr int BX = SegBX<P>::value;
     constexpr bool OTF = YlineOtf<P, MR, T>::value;
+    constexpr bool PF = YlinePf<P, MR, T, MV>::value;
     __shared__ T red[8][P][BX];
     __shared__ T cps[OTF ? MR : 1][P][BX];
     Seg q;
@@ -581,13 +751,12 @@ __device__ __forceinline__ void d_yline(Geo G, MV M, LineF F, const T* __restric
     if (act) {
         T dprev = 0, pfirst = 0, plast = 0, cprev = 0;
         // factor of row jj: 1/m and c'; g22m = g22(j-1), g22p = g22(j)
-        auto factor = [&](int jj, long c, T g22m, T g22p, T& inv) {
-            const T den = s2 * (mxp * (T)mg(M, 0, c) + mxm * (T)mg(M, 0, c - 1) + mzp * (T)mg(M, 8, c) +
-                                mzm * (T)mg(M, 8, c - sz) + ((jj == mr - 1) ? myl : (T)1) * g22p +
-                                ((jj == 0) ? myf : (T)1) * g22m);
+        auto factor = [&](int jj, T g11, T h11, T g33, T h33, T g22m, T g22p, T& inv) {
+            const T den = s2 * (mxp * g11 + mxm * h11 + mzp * g33 + mzm * h33 +
+                                ((jj == mr - 1) ? myl : (T)1) * g22p + ((jj == 0) ? myf : (T)1) * g22m);
             const T a = (jj > 0) ? -s2 * g22m : (T)0;
             const T cc = (jj < mr - 1) ? -s2 * g22p : (T)0;
-            inv = (T)1 / (den - a * cprev);
+            inv = pivot_inv(den - a * cprev);
             const T cp = cc * inv;
             cps[jj][s][tx] = cp;
             cprev = cp;
@@ -601,7 +770,8 @@ __device__ __forceinline__ void d_yline(Geo G, MV M, LineF F, const T* __restric
                 const T g22p = (T)mg(M, 4, c);
                 T a, inv;
                 if constexpr (OTF) {
-                    a = factor(jj, c, g22m, g22p, inv);
+                    a = factor(jj, (T)mg(M, 0, c), (T)mg(M, 0, c - 1), (T)mg(M, 8, c), (T)mg(M, 8, c - sz), g22m,
+                               g22p, inv);
                 } else {
                     a = (jj > 0) ? -s2 * g22m : (T)0;
                     inv = (T)F.invm[c];
@@ -614,28 +784,38 @@ __device__ __forceinline__ void d_yline(Geo G, MV M, LineF F, const T* __restric
         } else {
             Win<T> w;
             win_init(w, pin, cb, sy, sz);
-            G2<T> gm = load_g2<T>(M, cb);
             pfirst = w.p0;
+            // eta flux through the segment's lower face (row j0-1's eta-face metrics)
+            T g22m = (T)mg(M, 4, cb);
+            T Flo = flux_eta_lo(w, (T)mg(M, 3, cb), g22m, (T)mg(M, 5, cb), inv_pot);
+            RowIn<T, MV, TR, X0, OTF> nx;
+            if constexpr (PF) nx.load(pin, b, M, F, cb + sy, sy, sz);
 #pragma unroll
             for (int jj = 0; jj < MR; ++jj) {
                 const long c = cb + sy * (jj + 1);
-                win_load(w, pin, c, sy, sz);
-                const G2<T> g0 = load_g2<T>(M, c);
-                const T r = stencil<X0>(w, M, c, sz, gm, g0, inv_pot) - (T)b[c];
+                RowIn<T, MV, TR, X0, OTF> cur;
+                if constexpr (PF) {
+                    cur = nx;
+                    if (jj + 1 < MR) nx.load(pin, b, M, F, c + sy, sy, sz);
+                } else {
+                    cur.load(pin, b, M, F, c, sy, sz);
+                }
+                T m[15];
+                const T r = row_apply(w, cur, Flo, inv_pot, m) - (T)cur.rhs;
                 T a, inv, cp;
                 if constexpr (OTF) {
-                    a = factor(jj, c, gm.g22, g0.g22, inv);
+                    a = factor(jj, m[0], m[3], m[11], m[14], g22m, m[7], inv);
                     cp = cprev;
                 } else {
-                    a = (jj > 0) ? -s2 * gm.g22 : (T)0;
-                    inv = (T)F.invm[c];
+                    a = (jj > 0) ? -s2 * g22m : (T)0;
+                    inv = (T)cur.invm;
                     cp = (T)F.cpr[c];
                 }
                 const T d = (r - a * dprev) * inv;
                 e[jj] = w.p0 + omega * d + cp * w.pp;
                 dprev = d;
                 plast = w.p0;
-                gm = g0;
+                g22m = m[7];
                 win_shift(w);
             }
         }
@@ -652,19 +832,13 @@ __device__ __forceinline__ void d_yline(Geo G, MV M, LineF F, const T* __restric
         yF = e[0] - pfirst;
         yL = elast - plast;
         if (P > 1) {
-            const long cF = cb + sy, cL = cb + sy * mr;
-            if constexpr (PK) {
-                const H2 sF = F.vw16[cF], sL = F.vw16[cL];
-                vF = (T)sF.x;
-                wF = (T)sF.y;
-                vL = (T)sL.x;
-                wL = (T)sL.y;
-            } else {
-                vF = (T)spike<MVHalf<MV>::value>(F.v, F.v16, cF);
-                vL = (T)spike<MVHalf<MV>::value>(F.v, F.v16, cL);
-                wF = (T)spike<MVHalf<MV>::value>(F.w, F.w16, cF);
-                wL = (T)spike<MVHalf<MV>::value>(F.w, F.w16, cL);
-            }
+            SpikeRow<MV> sF, sL;
+            sF.load(F, cb + sy);
+            sL.load(F, cb + sy * mr);
+            vF = (T)sF.vv();
+            wF = (T)sF.ww();
+            vL = (T)sL.vv();
+            wL = (T)sL.ww();
         }
     }
     T uLm = 0, uFp = 0;
@@ -680,24 +854,26 @@ __device__ __forceinline__ void d_yline(Geo G, MV M, LineF F, const T* __restric
     }
     if (!act) return;
     const int i = q.i, k = q.k;
+    SpikeRow<MV> sv[MR];
+    if (P > 1) {
+#pragma unroll
+        for (int jj = 0; jj < MR; ++jj) sv[jj].load(F, cb + sy * (jj + 1));
+    }
+    // periodic images of the column: (ia, k), (i, ka), (ia, ka) on the same row
+    const int ia = (G.ip == 0) ? ((i == G.nx) ? 0 : (i == 1 ? G.nx + 1 : -1)) : -1;
+    const int ka = (G.kp == 0) ? ((k == G.nz) ? 0 : (k == 1 ? G.nz + 1 : -1)) : -1;
+    const long ox = (long)(ia - i), oz = (long)(ka - k) * sz;
     T pfin = 0;
 #pragma unroll
     for (int jj = 0; jj < MR; ++jj) {
         const long c = cb + sy * (jj + 1);
         T pn = e[jj];
-        if constexpr (PK) {
-            if (P > 1) {
-                const H2 sv = F.vw16[c];
-                pn -= uLm * (T)sv.x + uFp * (T)sv.y;
-            }
-        } else {
-            if (P > 1)
-                pn -= uLm * (T)spike<MVHalf<MV>::value>(F.v, F.v16, c) +
-                      uFp * (T)spike<MVHalf<MV>::value>(F.w, F.w16, c);
-        }
+        if (P > 1) pn -= uLm * (T)sv[jj].vv() + uFp * (T)sv[jj].ww();
         e[jj] = pn;
         pout[c] = pn;
-        write_images(G, pout, i, q.j0 + jj, k, pn);
+        if (ia >= 0) pout[c + ox] = pn;
+        if (ka >= 0) pout[c + oz] = pn;
+        if (ia >= 0 && ka >= 0) pout[c + ox + oz] = pn;
         if (jj == mr - 1) pfin = pn;
     }
     yline_ghosts(G, M, pin, pout, i, k, q.j0 == 1, q.j1 == G.ny, e[0], pfin, zero_in);
@@ -829,7 +1005,7 @@ __device__ __forceinline__ void d_resid(Geo G, Met<TM> M, const T* __restrict__ 
         G2<T> gm = load_g2<T>(M, cb);
         for (int j = q.j0; j <= q.j1; ++j) {
             const long c = cb + sy * (j - q.j0 + 1);
-            win_load(w, p, c, sy, sz);
+            win_load<X0>(w, p, c, sy, sz);
             const G2<T> g0 = load_g2<T>(M, c);
             const T ap = stencil<X0>(w, M, c, sz, gm, g0, inv_pot);
             if (MODE == 0) {
@@ -900,6 +1076,50 @@ __device__ __forceinline__ void d_resid_restrict(Geo G, Met<TM> M, const T* __re
     const bool act = i <= G.nx;
     const long sy = G.sy, sz = G.sz;
     const long cb0 = act ? G.idx(i, j0 - 1, 2 * kc - 1) : 0, cb1 = cb0 + sz;
+    if constexpr (sizeof(T) == 8) {
+        // FP64: each plane's next row is loaded while a row is computed, the
+        // eta flux carried between rows (the loads-at-use form below is
+        // scheduled by the compiler with one load in flight in FP64: 1.7x slower)
+        using RI = RowIn<T, Met<TM>, TR, X0, true>;
+        const LineF F{};
+        Win<T> w0, w1;
+        T F0 = 0, F1 = 0;
+        RI n0, n1;
+        if (act) {
+            win_init(w0, p, cb0, sy, sz);
+            win_init(w1, p, cb1, sy, sz);
+            F0 = flux_eta_lo(w0, (T)M.g[3][cb0], (T)M.g[4][cb0], (T)M.g[5][cb0], inv_pot);
+            F1 = flux_eta_lo(w1, (T)M.g[3][cb1], (T)M.g[4][cb1], (T)M.g[5][cb1], inv_pot);
+            n0.load(p, b, M, F, cb0 + sy, sy, sz);
+            n1.load(p, b, M, F, cb1 + sy, sy, sz);
+        }
+        for (int rp = 0; rp < m / 2; ++rp) {
+            T v = 0;
+            if (act) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int row = 2 * rp + h;
+                    const RI c0 = n0, c1 = n1;
+                    if (row + 1 < m) {
+                        n0.load(p, b, M, F, cb0 + sy * (row + 2), sy, sz);
+                        n1.load(p, b, M, F, cb1 + sy * (row + 2), sy, sz);
+                    }
+                    T mm[15];
+                    v += row_apply(w0, c0, F0, inv_pot, mm) - (T)c0.rhs;
+                    v += row_apply(w1, c1, F1, inv_pot, mm) - (T)c1.rhs;
+                    win_shift(w0);
+                    win_shift(w1);
+                }
+            }
+            if (fx == 1) {
+                if (act) rc[Gc.idx(i, (j0 + 1) / 2 + rp, kc)] = (TC)(-v * (T)0.25);
+                continue;
+            }
+            v += __shfl_down(v, 1, 64);
+            if (act && !(tx & 1)) rc[Gc.idx((i + 1) >> 1, (j0 + 1) / 2 + rp, kc)] = (TC)(-v * (T)0.125);
+        }
+        return;
+    }
     Win<T> w0, w1;
     G2<T> gm0{}, gm1{};
     if (act) {
@@ -914,8 +1134,8 @@ __device__ __forceinline__ void d_resid_restrict(Geo G, Met<TM> M, const T* __re
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const long c0 = cb0 + sy * (2 * rp + h + 1), c1 = c0 + sz;
-                win_load(w0, p, c0, sy, sz);
-                win_load(w1, p, c1, sy, sz);
+                win_load<X0>(w0, p, c0, sy, sz);
+                win_load<X0>(w1, p, c1, sy, sz);
                 const G2<T> g00 = load_g2<T>(M, c0), g01 = load_g2<T>(M, c1);
                 v += stencil<X0>(w0, M, c0, sz, gm0, g00, inv_pot) - (T)b[c0];
                 v += stencil<X0>(w1, M, c1, sz, gm1, g01, inv_pot) - (T)b[c1];

@@ -1,6 +1,7 @@
 // ples_mg.hip -- host side of the MI355X pressure multigrid: level
 // hierarchy, V-cycle, FP64 refinement loop (GCR-accelerated), C ABI.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <array>
@@ -107,6 +108,10 @@ struct ples_mg_handle {
     unsigned timing = 0;  // bit f: time kernel family f
     std::vector<Rec> recs;
     std::vector<KRec>* rec = nullptr;  // set: kernel launches are recorded, not enqueued
+    // timing of a single-dispatch scope: the next launch records these events
+    // at its own start and end (per-dispatch timestamps, no gap of their own)
+    hipEvent_t kt_a = nullptr, kt_b = nullptr;
+    bool kt_fired = false;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     hipEvent_t sev[4] = {};          // read-back ring of the refinement loop
     hipGraphExec_t gexec[2] = {};    // captured refinement step per (Z, W) pair
@@ -246,7 +251,7 @@ ColLaunch seg_launch(const Geo& G, int P) {
 
 // eta segments per column, fixed rule: among the counts P with a register
 // smoother variant (ny / P in {4, 8, 16, 32}, P in 1..16 dividing ny), the
-// fewest that still give the level ncol * P >= 2^17 threads (long register
+// fewest that still give the level ncol * P >= 2^16 threads (long register
 // rows where the grid fills the device), else 8 rows per segment where
 // possible (the small, latency-bound levels); the autotuner's choice on the
 // benchmark shapes.  Row counts without a register variant: choose_segments.
@@ -259,7 +264,7 @@ int fixed_segments(int ny, long ncol) {
     }
     if (cand.empty()) return choose_segments(ny);
     for (int P : cand)
-        if (ncol * P >= (1L << 17)) return P;
+        if (ncol * P >= (1L << 16)) return P;
     for (int P : cand)
         if (ny / P == 8) return P;
     return cand.back();
@@ -285,24 +290,43 @@ int choose_segments(int ny) {
 LineF line_f(const Level& L) { return LineF{L.invm, L.cpr, L.lv_v, L.lv_w, L.v16, L.w16, L.vw16}; }
 
 // per-launch event timing
+// Kernel time of a scope: events recorded on the stream before and after it,
+// or, for a scope of one kernel launch (one = true), the launch's own start
+// and end timestamps (the kernel's duration, without the gaps that event
+// records between launches add).
 struct KT {
     H* h;
     int fam;
     double bytes;
     hipEvent_t a = nullptr, b = nullptr;
-    bool on;
-    KT(H* h_, int fam_, double bytes_) : h(h_), fam(fam_), bytes(bytes_), on(((h_->timing >> fam_) & 1u) != 0) {
+    bool on, one;
+    KT(H* h_, int fam_, double bytes_, bool one_ = false)
+        : h(h_), fam(fam_), bytes(bytes_), on(((h_->timing >> fam_) & 1u) != 0), one(one_) {
         if (on) {
             (void)hipEventCreate(&a);
             (void)hipEventCreate(&b);
-            (void)hipEventRecord(a, h->s);
+            if (one) {
+                h->kt_a = a;
+                h->kt_b = b;
+                h->kt_fired = false;
+            } else {
+                (void)hipEventRecord(a, h->s);
+            }
         }
     }
     ~KT() {
-        if (on) {
+        if (!on) return;
+        if (one) {
+            h->kt_a = h->kt_b = nullptr;
+            if (!h->kt_fired) {  // nothing was launched (recorded for a grouped launch instead)
+                (void)hipEventDestroy(a);
+                (void)hipEventDestroy(b);
+                return;
+            }
+        } else {
             (void)hipEventRecord(b, h->s);
-            h->recs.push_back(Rec{fam, bytes, a, b});
         }
+        h->recs.push_back(Rec{fam, bytes, a, b});
     }
 };
 
@@ -317,6 +341,11 @@ void klaunch(H* h, K kern, dim3 g, dim3 b, size_t shm, A... a) {
         r.e.assign((const unsigned char*)&e, (const unsigned char*)&e + sizeof e);
         if (g.z != 1) r.go = nullptr;  // refused when the batch groups the records
         h->rec->push_back(std::move(r));
+        return;
+    }
+    if (h->kt_a && !h->kt_fired) {
+        hipExtLaunchKernelGGL(kern, g, b, (std::uint32_t)shm, h->s, h->kt_a, h->kt_b, 0u, a...);
+        h->kt_fired = true;
         return;
     }
     hipLaunchKernelGGL(kern, g, b, shm, h->s, a...);
@@ -556,7 +585,7 @@ int smooth(H* h, Level& L, const TR* rhs, int nu, bool zero_start, bool fine) {
                                  : N * (((MR > 0 && (sm == 2.0 || sizeof(T) == 8)) ? nmet(L) : 9.0) * sm + sp + sr +
                                         fac + sp + seg + stage);
         {
-            KT kt(h, fine ? PLES_MG_K_SMOOTH_FINE : PLES_MG_K_SMOOTH, bytes);
+            KT kt(h, fine ? PLES_MG_K_SMOOTH_FINE : PLES_MG_K_SMOOTH, bytes, true);
             switch (L.P) {
                 case 16: launch_yline<16, T, TR>(h, L, rhs, zin); break;
                 case 12: launch_yline<12, T, TR>(h, L, rhs, zin); break;
@@ -585,7 +614,7 @@ int vresid(H* h, Level& L, const TR* rhs, bool fine) {
         cl.grid = dim3((L.G.nx + L.rbx - 1) / L.rbx, L.G.nz, 1);
     }
     const double N = (double)L.ncell();
-    KT kt(h, fine ? PLES_MG_K_RESID_FINE : PLES_MG_K_RESID, N * (nmet(L) * sizeof(T) + 2 * sizeof(T) + sizeof(TR)));
+    KT kt(h, fine ? PLES_MG_K_RESID_FINE : PLES_MG_K_RESID, N * (nmet(L) * sizeof(T) + 2 * sizeof(T) + sizeof(TR)), true);
     auto go = [&](auto x0) {
         constexpr bool X0 = decltype(x0)::value;
         KLAUNCH(h, resid, TPL(0, T, T, TR, T, double, X0), 512, 1, cl.grid, cl.block, 0, L.G, met<T>(L),
@@ -603,7 +632,7 @@ int restrict_to(H* h, Level& F, Level& Cc) {
     dim3 block(std::min(64, pow2ceil(Cc.G.nx)), 4);
     const int gx = (Cc.G.nx + block.x - 1) / block.x;
     dim3 grid(gx * Cc.G.nz, (Cc.G.ny + 3) / 4);
-    KT kt(h, PLES_MG_K_TRANSFER, (double)F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC));
+    KT kt(h, PLES_MG_K_TRANSFER, (double)F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC), true);
     KLAUNCH(h, restrict, TPL(TF, TC), 256, 1, grid, block, 0, Cc.G, F.G, (const TF*)F.r, (TC*)Cc.b,
             F.G.nx / Cc.G.nx, F.G.ny / Cc.G.ny, gx);
     CK(hipGetLastError());
@@ -626,7 +655,7 @@ int resid_restrict(H* h, Level& F, Level& Cc, const TR* rhs, bool fine) {
     const dim3 block((unsigned)bx, (unsigned)P, 1), grid((unsigned)((F.G.nx + bx - 1) / bx), (unsigned)(F.G.nz / 2));
     const double N = (double)F.ncell();
     KT kt(h, fine ? PLES_MG_K_RESID_FINE : PLES_MG_K_RESID,
-          N * (nmet(F) * sizeof(T) + sizeof(T) + sizeof(TR)) + (double)Cc.ncell() * sizeof(TC));
+          N * (nmet(F) * sizeof(T) + sizeof(T) + sizeof(TR)) + (double)Cc.ncell() * sizeof(TC), true);
     auto go = [&](auto x0) {
         constexpr bool X0 = decltype(x0)::value;
         KLAUNCH(h, resid_restrict, TPL(T, T, TR, TC, X0), 512, 1, grid, block, 0, F.G, met<T>(F),
@@ -644,7 +673,7 @@ int prolong_to(H* h, Level& Cc, Level& F) {
         dim3 block(std::min(64, pow2ceil(Cc.G.nx + 2)), 4);
         const int gx = (Cc.G.nx + 2 + block.x - 1) / block.x;
         dim3 grid(gx * (Cc.G.nz + 1), (Cc.G.ny + 2 + 3) / 4);
-        KT kt(h, PLES_MG_K_TRANSFER, 2.0 * F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC));
+        KT kt(h, PLES_MG_K_TRANSFER, 2.0 * F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC), true);
         KLAUNCH(h, prolong_z, TPL(TC, TF), 256, 1, grid, block, 0, Cc.G, F.G, (const TC*)Cc.p[Cc.cur],
                 (TF*)F.p[F.cur], gx);
         CK(hipGetLastError());
@@ -654,7 +683,7 @@ int prolong_to(H* h, Level& Cc, Level& F) {
         dim3 block(std::min(64, pow2ceil(Cc.G.nx + 2)), 4);
         const int gx = (Cc.G.nx + 2 + block.x - 1) / block.x;
         dim3 grid(gx * (Cc.G.nz + 1), (Cc.G.ny + 1 + 3) / 4);
-        KT kt(h, PLES_MG_K_TRANSFER, 2.0 * F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC));
+        KT kt(h, PLES_MG_K_TRANSFER, 2.0 * F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC), true);
         KLAUNCH(h, prolong_yz, TPL(TC, TF), 256, 1, grid, block, 0, Cc.G, F.G, (const TC*)Cc.p[Cc.cur],
                 (TF*)F.p[F.cur], gx);
         CK(hipGetLastError());
@@ -664,7 +693,7 @@ int prolong_to(H* h, Level& Cc, Level& F) {
     dim3 block(std::min(64, pow2ceil(Cc.G.nx + 1)), 4);
     const int gx = (Cc.G.nx + 1 + block.x - 1) / block.x;
     dim3 grid(gx * (Cc.G.nz + 1), (Cc.G.ny + 1 + 3) / 4);
-    KT kt(h, PLES_MG_K_TRANSFER, 2.0 * F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC));
+    KT kt(h, PLES_MG_K_TRANSFER, 2.0 * F.ncell() * sizeof(TF) + (double)Cc.ncell() * sizeof(TC), true);
     KLAUNCH(h, prolong_blk, TPL(TC, TF), 256, 1, grid, block, 0, Cc.G, F.G, (const TC*)Cc.p[Cc.cur],
             (TF*)F.p[F.cur], gx);
     CK(hipGetLastError());
@@ -3830,14 +3859,26 @@ int ples_mg_server_step(ples_mg_server* sv, int* n_done) {
     CK(hipSetDevice(sv->so.device));
     int done = 0;
     // admit in arrival order; a request whose case is busy waits without
-    // blocking the ones behind it
+    // blocking the ones behind it, except later requests of the same case: a
+    // case's time steps run in the order they were submitted (a cold start
+    // landing on the worker during this scan must not let a later step of
+    // the case start before an earlier one that was passed over)
+    std::vector<uint64_t> passed;
     for (auto it = sv->queue.begin(); it != sv->queue.end();) {
         if ((int)sv->active.size() >= sv->so.max_active) break;
         SReq& r = sv->reqs.at(*it);
+        if (std::find(passed.begin(), passed.end(), r.case_id) != passed.end()) {
+            ++it;
+            continue;
+        }
         const int a = try_admit(sv, r);
         if (a < 0 || a > 1) return a;  // error status
-        if (a == 1) it = sv->queue.erase(it);
-        else ++it;
+        if (a == 1) {
+            it = sv->queue.erase(it);
+        } else {
+            passed.push_back(r.case_id);
+            ++it;
+        }
     }
     for (size_t k = 0; k < sv->active.size();) {
         SReq& r = sv->reqs.at(sv->active[k]);
