@@ -106,10 +106,14 @@ typedef struct ples_mg_opts {
                         solution.  Used on the fine level (the autotuner may
                         choose per level).  0 = FP32 metrics everywhere.
                         Default 1 */
-    int coarse_exact; /* 1 = a coarsest level of at most 512 cells is solved
-                        exactly (a dense map built at create from the level's
-                        operator with its wall ghosts eliminated, applied as one
-                        matrix-vector product; default); 0 = nu_coarse sweeps */
+    int coarse_exact; /* 1 = the V-cycle descends to the first coarse level
+                        of at most 2048 cells (512 on grids with xi or zeta
+                        walls; else to the last level) and solves that level
+                        exactly when it is that small: a dense map built at create from the level's
+                        operator with its wall ghosts eliminated (inverted on
+                        the device), applied as one matrix-vector product;
+                        levels below it are built but not visited (default).
+                        0 = all nlevel levels, nu_coarse sweeps on the last */
     int sweep_schedule; /* damping of the nu sweeps of a visit: 1 = 1 / (the
                         roots of the degree-nu Chebyshev polynomial on
                         [0.48, 2.4]), which damps that part of the smoother's
@@ -455,6 +459,12 @@ int ples_mg_streams_bandwidth(int device, size_t bytes, int reps, double* gbs);
 /* Launch shape of level `level` (1-based): smoother segments per column P,
  * V-cycle residual segments rP x lanes rbx (outputs may be NULL). */
 int ples_mg_tiles(const ples_mg_handle* h, int level, int* P, int* rP, int* rbx);
+
+/* Number of levels the V-cycle visits: nlevel, or fewer with coarse_exact,
+ * where the cycle stops at the first coarse level small enough for the exact
+ * dense solve (the reference's multi visits all nlevel levels,
+ * fsrc/multigrid.f03:216-563). */
+int ples_mg_cycle_levels(const ples_mg_handle* h, int* levels);
 
 /* HIP stream the handle launches on (hipStream_t as void*). */
 void* ples_mg_stream(ples_mg_handle* h);

@@ -1823,6 +1823,69 @@ __global__ void __launch_bounds__(256) k_coarse_dense(Geo G, const double* __res
 
 
 // ------------------------------------------------------------------------
+// Gauss-Jordan inverse with partial pivoting of an nb x nb matrix B, held
+// with the identity as the row-major nb x 2nb array M = [B | I]; after the
+// nb column steps the right half is B^-1 (set-up of the exact coarse solve).
+// Column step col: one block picks the pivot row (largest |M[r][col]|, r >=
+// col, the smallest such r on ties), swaps it into row col, scales that row
+// and records every row's elimination factor; then the other rows are
+// updated in the columns right of col (the columns left of it are already
+// unit columns, and column col itself is not read again).
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_gj_pivot(double* __restrict__ M, long nb, long col,
+                                                   double* __restrict__ f, int* __restrict__ status) {
+    __shared__ double bv[1024];
+    __shared__ long bi[1024];
+    const long w = 2 * nb;
+    const int t = threadIdx.x;
+    double best = -1.0;
+    long bidx = col;
+    for (long r = col + t; r < nb; r += blockDim.x) {
+        const double a = fabs(M[r * w + col]);
+        if (a > best) {
+            best = a;
+            bidx = r;
+        }
+    }
+    bv[t] = best;
+    bi[t] = bidx;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if (t < h && (bv[t + h] > bv[t] || (bv[t + h] == bv[t] && bi[t + h] < bi[t]))) {
+            bv[t] = bv[t + h];
+            bi[t] = bi[t + h];
+        }
+        __syncthreads();
+    }
+    const long p = bi[0];
+    const double pv = M[p * w + col];
+    if (!(fabs(pv) > 0.0)) {  // singular (every thread sees the same value)
+        if (t == 0) *status = 1;
+        return;
+    }
+    __syncthreads();  // every thread has read the pivot before the rows change
+    const double inv = 1.0 / pv;
+    for (long c = t; c < w; c += blockDim.x) {
+        const double a = M[p * w + c], b = M[col * w + c];
+        M[col * w + c] = a * inv;
+        if (p != col) M[p * w + c] = b;
+    }
+    __syncthreads();
+    for (long r = t; r < nb; r += blockDim.x) f[r] = (r == col) ? 0.0 : M[r * w + col];
+}
+// grid: (columns right of col, 256 per block) x rows
+__global__ void __launch_bounds__(256) k_gj_elim(double* __restrict__ M, long nb, long col,
+                                                 const double* __restrict__ f) {
+    const long w = 2 * nb;
+    const long c = col + 1 + (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long r = blockIdx.y;
+    if (c >= w) return;
+    const double fr = f[r];
+    if (fr == 0.0) return;
+    M[r * w + c] -= fr * M[col * w + c];
+}
+
+// ------------------------------------------------------------------------
 // Smoother factors of the segmented eta-line solve (see LineF), computed in
 // FP64 and stored in FP32; the v and w solves use the stored factors so the
 // partitioned solve is consistent with the smoother's local solves.

@@ -83,6 +83,8 @@ struct ples_mg_handle {
     hipStream_t s = nullptr;
     ples_mg_opts o{};
     int jx = 0, jy = 0, jz = 0, ip = 0, jp = 1, kp = 0, L = 1;
+    int Lv = 1;  // levels the V-cycle visits: with coarse_exact it stops at the
+                 // first coarse level of at most kCoarseDenseMax cells, solved exactly
     std::vector<Level> lv;
     double* nodes[3] = {};
     double* x = nullptr;  // resident iterate (FP64, ghosted)
@@ -761,8 +763,13 @@ int lv_prolong(H* h, int n) {  // n+1 -> n
 
 // coarsest level in one launch of one workgroup when its columns fit a block
 // and it has no xi/zeta walls (whose ghosts need the separate halo launches)
-// coarsest levels up to this many cells get the exact dense solve
-constexpr long kCoarseDenseMax = 512;
+// coarse levels up to this many cells get the exact dense solve: the
+// V-cycle stops at the first such level (a level of 2048 cells, e.g.
+// 16x8x16, costs one 53 MB matrix-vector product instead of its sweeps,
+// transfers and the levels below it).  Levels with xi or zeta walls keep 512:
+// the set-up's ghost fixed point of each unit vector converges slowly there.
+constexpr long kCoarseDenseMax = 2048, kCoarseDenseMaxWalls = 512;
+long dense_max(int ip, int kp) { return (ip == 0 && kp == 0) ? kCoarseDenseMax : kCoarseDenseMaxWalls; }
 // interval of the Chebyshev sweep schedule on the spectrum of M_line^-1 A
 // (measured on the terrain workloads: one V-cycle fewer than omega = 0.8 at
 // 256x128x256 in both precisions and at 512x256x512; the cycle counts are
@@ -811,7 +818,7 @@ int coarse_solve(H* h, Level& L, const double* rhs, int nu) {
 // initial guess (fine_zero) or continuing from the current one; the coarse
 // levels always solve for corrections from zero.
 int vcycle(H* h, const double* r0, bool fine_zero = true) {
-    const int L = h->L;
+    const int L = h->Lv;
     for (int n = 0; n < L; ++n) {
         const int nu = (n == L - 1) ? h->o.nu_coarse : h->o.nu_pre;
         Level& lvn = h->lv[n];
@@ -990,7 +997,7 @@ int step32_body(H* h, int q) {
     // One V-cycle from zero flips the fine ping-pong pair max(1, nu_pre) +
     // nu_post times: the buffer its last sweep writes is known, so Zq stands in
     // for it and receives z directly (no copy); Zp is not touched.
-    const bool direct = h->inner == 1 && h->L >= 2;
+    const bool direct = h->inner == 1 && h->Lv >= 2;
     const int fin = L0.cur ^ ((std::max(1, h->o.nu_pre) + std::max(0, h->o.nu_post)) & 1);
     void* held = L0.p[fin];
     if (direct) L0.p[fin] = Zq;
@@ -1380,7 +1387,8 @@ bool coarse_one_block(const Level& L);
 // the level keeps the fixed rule's shape)
 int tune_level(H* h, int n, bool time_missing = true) {
     Level& L = h->lv[n];
-    if (n == h->L - 1 && (L.kd || coarse_one_block(L))) return PLES_MG_OK;  // one-launch coarse solver
+    if (n >= h->Lv) return PLES_MG_OK;  // below the V-cycle's coarsest level: not visited
+    if (n == h->Lv - 1 && (L.kd || coarse_one_block(L))) return PLES_MG_OK;  // one-launch coarse solver
     const bool half = L.g16[0] != nullptr;
     const TuneKey key{L.G.nx, L.G.ny, L.G.nz, L.G.ip, L.G.kp, L.f32 ? 1 : 0, n == 0 ? 1 : 0, half ? 1 : 0};
     std::lock_guard<std::mutex> lock(g_tune_mu);
@@ -1446,7 +1454,7 @@ int tune_level(H* h, int n, bool time_missing = true) {
         L.rbx = bx;
         double ms = 0;
         // the V-cycle runs the residual fused with the restriction below the coarsest level
-        CKS(time_launches(h, 3, ms, [&] { return n < h->L - 1 ? lv_resid_restrict(h, n, r0) : lv_resid(h, n, r0); }));
+        CKS(time_launches(h, 3, ms, [&] { return n < h->Lv - 1 ? lv_resid_restrict(h, n, r0) : lv_resid(h, n, r0); }));
         if (ms < 0.97 * tbest || tbest == 1e30) {
             tbest = ms;
             best.rP = P;
@@ -1554,6 +1562,8 @@ int build_coarse_dense(H* h, Level& L) {
     std::vector<double> A((size_t)(N * N), 0.0);   // column-major
     std::vector<double> E((size_t)(ng * N), 0.0);  // column-major: ghosted image of e_m
     std::vector<double> v((size_t)ng), prev((size_t)ng);
+    std::vector<long> seen((size_t)N, 0), rows((size_t)N);
+    long stamp = 0, nrows = 0;
     const double inv_pot = 1.0 / L.pot;
     for (long m = 0; m < N; ++m) {
         std::fill(v.begin(), v.end(), 0.0);
@@ -1571,7 +1581,28 @@ int build_coarse_dense(H* h, Level& L) {
             if (d <= 1e-17) break;  // unit-scale vector: converged to rounding
         }
         std::copy(v.begin(), v.end(), E.begin() + (size_t)(m * ng));
-        for (long r = 0; r < N; ++r) {
+        // the 19-point stencil of a row reads only cells within one cell of it:
+        // rows next to a nonzero entry of the ghosted image (the unit cell, its
+        // periodic images, the wall ghosts it sets) are the column's nonzeros
+        ++stamp;
+        nrows = 0;
+        for (long gc = 0; gc < ng; ++gc) {
+            if (v[(size_t)gc] == 0.0) continue;
+            const int gi = (int)(gc % NX), gj = (int)((gc / NX) % NY), gk = (int)(gc / (NX * NY));
+            for (int dk = -1; dk <= 1; ++dk)
+                for (int dj = -1; dj <= 1; ++dj)
+                    for (int di = -1; di <= 1; ++di) {
+                        const int i = gi + di, j = gj + dj, k = gk + dk;
+                        if (i < 1 || i > G.nx || j < 1 || j > G.ny || k < 1 || k > G.nz) continue;
+                        const long r = (long)(i - 1) + (long)G.nx * ((j - 1) + (long)G.ny * (k - 1));
+                        if (seen[(size_t)r] != stamp) {
+                            seen[(size_t)r] = stamp;
+                            rows[(size_t)nrows++] = r;
+                        }
+                    }
+        }
+        for (long q = 0; q < nrows; ++q) {
+            const long r = rows[(size_t)q];
             const long c = cell(r);
             Win<double> w;
             win_init(w, v.data(), c - G.sy, G.sy, G.sz);
@@ -1580,39 +1611,43 @@ int build_coarse_dense(H* h, Level& L) {
             A[(size_t)(r + N * m)] = stencil(w, M, c, G.sz, gmv, g0, inv_pot);
         }
     }
-    // Gauss-Jordan inverse of the bordered matrix, partial pivoting
-    const long nb = N + 1;
-    std::vector<double> B((size_t)(nb * nb), 0.0), I((size_t)(nb * nb), 0.0);  // row-major
-    for (long r = 0; r < N; ++r) {
-        for (long c = 0; c < N; ++c) B[(size_t)(r * nb + c)] = A[(size_t)(r + N * c)];
-        B[(size_t)(r * nb + N)] = 1.0;
-        B[(size_t)(N * nb + r)] = 1.0;
-    }
-    for (long r = 0; r < nb; ++r) I[(size_t)(r * nb + r)] = 1.0;
-    for (long col = 0; col < nb; ++col) {
-        long piv = col;
-        for (long r = col + 1; r < nb; ++r)
-            if (std::fabs(B[(size_t)(r * nb + col)]) > std::fabs(B[(size_t)(piv * nb + col)])) piv = r;
-        if (!(std::fabs(B[(size_t)(piv * nb + col)]) > 0.0)) return PLES_MG_ERR_NONFINITE;
-        if (piv != col)
-            for (long c = 0; c < nb; ++c) {
-                std::swap(B[(size_t)(piv * nb + c)], B[(size_t)(col * nb + c)]);
-                std::swap(I[(size_t)(piv * nb + c)], I[(size_t)(col * nb + c)]);
-            }
-        const double d = 1.0 / B[(size_t)(col * nb + col)];
-        for (long c = 0; c < nb; ++c) {
-            B[(size_t)(col * nb + c)] *= d;
-            I[(size_t)(col * nb + c)] *= d;
+    // Gauss-Jordan inverse of the bordered matrix (partial pivoting) on the
+    // device: [B | I] is nb x 2nb, the right half becomes B^-1
+    const long nb = N + 1, w = 2 * nb;
+    std::vector<double> I((size_t)(nb * nb), 0.0);  // row-major inverse
+    {
+        std::vector<double> Mh((size_t)(nb * w), 0.0);
+        for (long r = 0; r < N; ++r) {
+            for (long c = 0; c < N; ++c) Mh[(size_t)(r * w + c)] = A[(size_t)(r + N * c)];
+            Mh[(size_t)(r * w + N)] = 1.0;
+            Mh[(size_t)(N * w + r)] = 1.0;
         }
-        for (long r = 0; r < nb; ++r) {
-            if (r == col) continue;
-            const double f = B[(size_t)(r * nb + col)];
-            if (f == 0.0) continue;
-            for (long c = 0; c < nb; ++c) {
-                B[(size_t)(r * nb + c)] -= f * B[(size_t)(col * nb + c)];
-                I[(size_t)(r * nb + c)] -= f * I[(size_t)(col * nb + c)];
-            }
+        for (long r = 0; r < nb; ++r) Mh[(size_t)(r * w + nb + r)] = 1.0;
+        double* Md = nullptr;
+        double* fd = nullptr;
+        int* std_ = nullptr;
+        CKS(dalloc(h, (void**)&Md, (size_t)(nb * w) * 8));
+        CKS(dalloc(h, (void**)&fd, (size_t)nb * 8));
+        CKS(dalloc(h, (void**)&std_, sizeof(int)));
+        CK(hipMemcpyAsync(Md, Mh.data(), (size_t)(nb * w) * 8, hipMemcpyHostToDevice, h->s));
+        CK(hipMemsetAsync(std_, 0, sizeof(int), h->s));
+        for (long col = 0; col < nb; ++col) {
+            hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, h->s, Md, nb, col, fd, std_);
+            const long nc = w - col - 1;
+            if (nc > 0)
+                hipLaunchKernelGGL(k_gj_elim, dim3((unsigned)((nc + 255) / 256), (unsigned)nb), dim3(256), 0, h->s, Md,
+                                   nb, col, (const double*)fd);
         }
+        CK(hipGetLastError());
+        int sing = 0;
+        CK(hipMemcpy2DAsync(I.data(), (size_t)nb * 8, Md + nb, (size_t)w * 8, (size_t)nb * 8, (size_t)nb,
+                            hipMemcpyDeviceToHost, h->s));
+        CK(hipMemcpyAsync(&sing, std_, sizeof(int), hipMemcpyDeviceToHost, h->s));
+        CK(hipStreamSynchronize(h->s));
+        dfree(h, Md);
+        dfree(h, fd);
+        dfree(h, std_);
+        if (sing) return PLES_MG_ERR_NONFINITE;
     }
     // K_full = E K (row-major, ghosted rows x interior columns)
     std::vector<double> KF((size_t)(ng * N), 0.0);
@@ -1785,11 +1820,21 @@ size_t estimate(const ples_mg_grid* g, const ples_mg_opts* o) {
     size_t tot = 0;
     const bool mixed = o && o->precision == PLES_MG_MIXED;
     const bool semi = o && o->semicoarsen;
+    // levels the V-cycle visits (create's rule; with semicoarsening the level
+    // shapes depend on the grid, so both hierarchies' smallest first level counts)
+    int lv = g->nlevel;
+    if (o && o->coarse_exact)
+        for (int n = 1; n < g->nlevel && lv == g->nlevel; ++n) {
+            const int st = 1 << n, sx = semi ? st / 2 : st, sy = (semi && o->semicoarsen == 1) ? st / 2 : st;
+            const long full = (long)(g->jx / st) * (g->jy / st) * (g->jz / st);
+            const long semic = (long)(g->jx / sx) * (g->jy / sy) * (g->jz / st);
+            if (std::min(full, semic) <= dense_max(g->ip, g->kp)) lv = n + 1;
+        }
     for (int n = 0; n < g->nlevel; ++n) {
         const int st = 1 << n, sx = (semi && n > 0) ? st / 2 : st;
         const int sy = (semi && n > 0 && o->semicoarsen == 1) ? st / 2 : st;
         const size_t cells = (size_t)(g->jx / sx + 2) * (g->jy / sy + 2) * (g->jz / st + 2);
-        const bool f32 = mixed && n < g->nlevel - 1;
+        const bool f32 = mixed && n < lv - 1;
         const size_t sp = f32 ? 4 : 8;
         tot += cells * 9 * sp;                  // V-cycle metrics
         if (f32 && o->smoother_fp16) tot += cells * (9 * 2 + 4 + 12 + 4);  // smoother's 16-bit metrics, spikes, packed
@@ -1798,13 +1843,13 @@ size_t estimate(const ples_mg_grid* g, const ples_mg_opts* o) {
         tot += cells * sp * 2;                  // ping-pong
         if (n > 0) tot += cells * sp;           // rhs
         if (n < g->nlevel - 1) tot += cells * sp;  // residual
-        if (n == g->nlevel - 1 && o && o->coarse_exact) {  // dense map of either hierarchy, the larger
+        if (n == lv - 1 && o && o->coarse_exact) {  // dense map of either hierarchy, the larger
             size_t dense = 0;
             for (int f : {sx, st}) {
                 const int fy = f == st ? st : sy;
                 const size_t ni = (size_t)(g->jx / f) * (g->jy / fy) * (g->jz / st);
                 const size_t cf = (size_t)(g->jx / f + 2) * (g->jy / fy + 2) * (g->jz / st + 2);
-                if ((long)ni <= kCoarseDenseMax) dense = std::max(dense, cf * ni * 8);
+                if ((long)ni <= dense_max(g->ip, g->kp)) dense = std::max(dense, cf * ni * 8);
             }
             tot += dense;
         }
@@ -1932,6 +1977,7 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
     h->jx = g->jx; h->jy = g->jy; h->jz = g->jz;
     h->ip = g->ip; h->jp = g->jp; h->kp = g->kp;
     h->L = g->nlevel;
+    h->Lv = h->L;
     const bool mixed = o.precision == PLES_MG_MIXED;
     // nodes
     const size_t nn = (size_t)(g->jx + 1) * (g->jy + 1) * (g->jz + 1);
@@ -1958,7 +2004,9 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
         L.sy = (semi && n > 0 && o.semicoarsen == 1) ? st / 2 : st;
         L.G = make_geo(g->jx / L.sx, g->jy / L.sy, g->jz / st, g->ip, g->jp, g->kp);
         L.pot = (double)st;
-        L.f32 = mixed && n < h->L - 1;
+        // the first coarse level small enough for the dense solve ends the V-cycle
+        if (h->Lv == h->L && n >= 1 && o.coarse_exact && L.ncell() <= dense_max(g->ip, g->kp)) h->Lv = n + 1;
+        L.f32 = mixed && n < h->Lv - 1;
         const size_t cells = (size_t)L.G.n;
         for (int q = 0; q < 9; ++q) CKS(dalloc(h, (void**)&L.g64[q], cells * 8));
         CKS(compute_metrics(h, n, L.g64, nullptr));
@@ -2024,7 +2072,7 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
             CKS(dalloc(h, (void**)&L.lv_w, cells * 4));
         }
         CKS(level_factors(h, L));
-        if (n == h->L - 1 && o.coarse_exact && L.ncell() <= (long)kCoarseDenseMax) CKS(build_coarse_dense(h, L));
+        if (n == h->Lv - 1 && o.coarse_exact && L.ncell() <= dense_max(g->ip, g->kp)) CKS(build_coarse_dense(h, L));
         const size_t sp = L.f32 ? 4 : 8;
         CKS(dalloc(h, &L.p[0], cells * sp));
         CKS(dalloc(h, &L.p[1], cells * sp));
@@ -2042,7 +2090,7 @@ static int create_impl(const ples_mg_grid* g, const ples_mg_opts* oin, H* h) {
         CKS(dalloc(h, (void**)&h->Z[q], c0 * 8));
         CKS(dalloc(h, (void**)&h->W[q], c0 * 8));
     }
-    h->ir32 = mixed && h->L >= 2 && o.fp32_krylov != 0;
+    h->ir32 = mixed && h->Lv >= 2 && o.fp32_krylov != 0;
     if (h->ir32) {
         CKS(dalloc(h, (void**)&h->e32, c0 * 4));
         CKS(dalloc(h, (void**)&h->rr32, c0 * 4));
@@ -3415,6 +3463,12 @@ int ples_mg_tiles(const ples_mg_handle* h, int level, int* P, int* rP, int* rbx)
     if (P) *P = L.P;
     if (rP) *rP = L.rP > 0 ? L.rP : L.P;
     if (rbx) *rbx = L.rP > 0 ? L.rbx : (int)cl.block.x;
+    return PLES_MG_OK;
+}
+
+int ples_mg_cycle_levels(const ples_mg_handle* h, int* levels) {
+    if (!h || !levels) return PLES_MG_ERR_ARG;
+    *levels = h->Lv;
     return PLES_MG_OK;
 }
 

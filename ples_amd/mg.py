@@ -136,6 +136,7 @@ def lib():
         L.ples_mg_batch_set_grouped.argtypes = [C.c_void_p, C.c_int]
         L.ples_mg_batch_families.argtypes = [C.c_void_p]
         L.ples_mg_tiles.argtypes = [C.c_void_p, C.c_int] + [C.POINTER(C.c_int)] * 3
+        L.ples_mg_cycle_levels.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.ples_mg_copy_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
         L.ples_mg_read_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
         L.ples_mg_streams_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
@@ -601,6 +602,12 @@ class Solver:
     @property
     def device_bytes(self) -> int:
         return int(lib().ples_mg_device_bytes(self._h))
+
+    def cycle_levels(self) -> int:
+        """Levels the V-cycle visits (nlevel, or fewer with coarse_exact)."""
+        v = C.c_int()
+        _check(lib().ples_mg_cycle_levels(self._h, C.byref(v)), "ples_mg_cycle_levels")
+        return v.value
 
     def tiles(self, level: int):
         """(smoother segments P, residual segments rP, residual lanes rbx) of a level."""

@@ -86,10 +86,12 @@ def test_solutions_checked_by_oracle_operator_256x128x256():
 
 
 def test_mixed_equals_fp64_and_linearity_512x256x512():
-    """At 512x256x512: mixed and FP64 pressures agree to 1e-9; and with two
-    rhs b1 (the device-generated divergence) and b2 (seeded noise, mean
-    removed), the mixed solve of 2 b1 + b2 equals 2 x1 + x2 to 1e-9 (each
-    solve converged to 1e-12)."""
+    """At 512x256x512: mixed and FP64 pressures, each solved from zero to the
+    benchmark tolerance and continued to 1e-12, agree to 1e-9 (at 1e-10 the
+    smoothest error modes leave 1e-9 to 1e-8 of the pressure, differently for
+    the two precisions); and with two rhs b1 (the device-generated
+    divergence) and b2 (seeded noise, mean removed), the mixed solve of
+    2 b1 + b2 equals 2 x1 + x2 to 1e-9 (each solve converged to 1e-12)."""
     cfg, g = _grid("c3")
     sols = {}
     for prec in ("fp64", "mixed"):
@@ -97,6 +99,8 @@ def test_mixed_equals_fp64_and_linearity_512x256x512():
             s.case_rhs(cfg.seed)
             s.zero_fi()
             res = s.solve_resident(cfg.tol, 200)
+            assert res.ok, (prec, res)
+            res = s.solve_resident(1e-12, 200)
             assert res.ok, (prec, res)
             sols[prec] = s.get_fi()
             if prec == "mixed":

@@ -475,3 +475,27 @@ def test_segment_counts_same_solution(P, tmp_path):
         assert r.ok, r
     assert rmax < 1e-13
     assert cases.rel_l2_meanfree(fi, fiv) < 1e-9
+
+
+def test_vcycle_stops_at_first_dense_level():
+    """With coarse_exact the V-cycle descends to the first coarse level of at
+    most 2048 cells on grids periodic in xi and zeta (512 with xi/zeta walls),
+    solves it exactly and still reaches the oracle's pressure; the deeper
+    levels stay built (their shapes are reported) but are not visited."""
+    g = cases.grid(64, 32, 64, gamma=1.5, amp=0.2)
+    rhs_g = O.case_rhs(g, 93)
+    ref = _oracle_solution(g, 3, rhs_g)  # the discrete solution does not depend on the level count
+    for prec in ("fp64", "mixed"):
+        with _solver(g, 5, prec) as s:
+            # 64x32x64, 32x16x32, 16x8x16 (2048 cells: exact), then 8x4x8, 4x2x4 unvisited
+            assert s.cycle_levels() == 3
+            assert s.level_shape(5) == (4, 2, 4)
+            fi = np.zeros(s.fi_shape)
+            res = s.multi(fi, cases.interior(rhs_g), eps=1e-12, ficycle=200)
+            assert res.ok, (prec, res)
+        assert cases.rel_l2_meanfree(fi, ref) < 1e-9, prec
+        with _solver(g, 5, prec, coarse_exact=0) as s:
+            assert s.cycle_levels() == 5
+    gw = cases.grid(32, 16, 32, gamma=1.0, amp=0.1, ip=1, kp=1)
+    with _solver(gw, 3, "fp64") as s:
+        assert s.cycle_levels() == 3  # 16x8x16 is above the walled limit: down to 8x4x8
