@@ -1167,8 +1167,14 @@ __global__ void __launch_bounds__(512) k_resid_restrict(Geo G, Met<TM> M, const 
 // S_DONE: set by k_fin_update once the stop test (S_MODE 0: r.r < S_THR,
 // 1: max|r| < S_THR) holds; later refinement steps then leave x and r alone,
 // so the host may enqueue steps ahead of reading the residual.
+// Device-driven iterative refinement (batched mixed solves) keeps its state
+// in the outer set: S_RN (norm the inner rhs is scaled by), S_STALL and
+// S_MOLD (refinements that did not halve the outer measure; the measure
+// before), S_EPS, S_BN (|b|), S_MAXC, S_ABS (stop mode), S_TOTC (V-cycles
+// applied), S_REF / S_NOREF (this step ends a refinement / it does not).
 enum { S_RR = 0, S_RMAX, S_BB, S_WW, S_WWP, S_RW, S_RWP, S_WPWP, S_ALPHA, S_BETA, S_HAVEP, S_RR0,
-       S_SUM, S_SUM2, S_DONE, S_THR, S_MODE, S_CYC, S_COUNT = 20 };
+       S_SUM, S_SUM2, S_DONE, S_THR, S_MODE, S_CYC, S_RN, S_STALL, S_MOLD, S_EPS, S_BN, S_MAXC, S_ABS,
+       S_TOTC, S_REF, S_NOREF, S_COUNT = 32 };
 
 // sum (or max) of nb partials per quantity, one block of 256 threads,
 // fixed order: strided per-thread sums then a tree
@@ -2340,6 +2346,117 @@ __global__ void __launch_bounds__(256) k_add_back(Geo G, double* __restrict__ x,
     long c;
     if (!cell_of(G, c)) return;
     x[c] += s * (double)e32[c];
+}
+
+// ------------------------------------------------------------------------
+// Device-driven iterative refinement: the refinement loop of the mixed
+// solver (inner FP32 GCR to a relative tolerance, x += |r| e, FP64 residual,
+// next tolerance) with every decision taken on the device, so a batch of
+// solves advances by graph-replayed steps without host round trips.  Each
+// step is one inner step followed by this tail, whose kernels do nothing
+// unless the step ends a refinement (S_REF).
+// ------------------------------------------------------------------------
+// Does this step end a refinement?  The inner solve met its tolerance, or it
+// ran 40 steps, or the cycle budget is spent.
+__device__ __forceinline__ void d_ir_check(double* S, const double* S2) {
+    if (threadIdx.x != 0) return;
+    double ref = 0.0;
+    if (S[S_DONE] == 0.0) {
+        const double n = S2[S_CYC];
+        const double lim = fmin(40.0, S[S_MAXC] - S[S_TOTC]);
+        if (S2[S_DONE] != 0.0 || n >= lim) ref = 1.0;
+    }
+    S[S_REF] = ref;
+    S[S_NOREF] = 1.0 - ref;
+}
+__global__ void k_ir_check(double* S, const double* S2) { d_ir_check(S, S2); }
+
+// x += |r| e32 at the end of a refinement
+// interior cell c of a 1-D launch over the ghosted array
+__device__ inline bool flat_interior(const Geo& G, long& c) {
+    c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= G.n) return false;
+    const long NX = G.nx + 2, NY = G.ny + 2;
+    const int i = (int)(c % NX), j = (int)((c / NX) % NY), k = (int)(c / (NX * NY));
+    return i >= 1 && i <= G.nx && j >= 1 && j <= G.ny && k >= 1 && k <= G.nz;
+}
+
+__device__ __forceinline__ void d_add_back_if(Geo G, double* __restrict__ x, const double* __restrict__ S,
+                                              const float* __restrict__ e32) {
+    if (S[S_REF] == 0.0) return;
+    long c;
+    if (!flat_interior(G, c)) return;
+    x[c] += S[S_RN] * (double)e32[c];
+}
+__global__ void __launch_bounds__(256) k_add_back_if(Geo G, double* __restrict__ x, const double* __restrict__ S,
+                                                     const float* __restrict__ e32) {
+    d_add_back_if(G, x, S, e32);
+}
+
+// the new FP64 residual's norms (k_resid MODE 1 partials), the stop and
+// stall tests, and the next refinement's scale and inner tolerance (S_DONE:
+// 1 converged, 2 not converged, 3 non-finite); a finished solve clears S_REF
+__device__ __forceinline__ void d_fin_ir(const double* __restrict__ part, int nb, double* S, double* S2) {
+    if (S[S_REF] == 0.0) return;  // uniform
+    const bool mx[3] = {false, true, false};
+    double o[3];
+    fin_reduce<3>(part, nb, mx, o);
+    if (threadIdx.x != 0) return;
+    const double rr = o[0], rmax = o[1];
+    S[S_RR] = rr;
+    S[S_RMAX] = rmax;
+    S[S_TOTC] += S2[S_CYC];
+    const bool absm = S[S_ABS] != 0.0;
+    const double m = absm ? rmax : sqrt(rr) / S[S_BN];
+    double done = 0.0;
+    if (!isfinite(rr)) {
+        done = 3.0;
+    } else if (m < S[S_EPS]) {
+        done = 1.0;
+    } else {
+        if (m > 0.5 * S[S_MOLD]) {
+            S[S_STALL] += 1.0;
+            if (S[S_STALL] >= 3.0) done = 2.0;  // FP32 cannot reduce it further
+        } else {
+            S[S_STALL] = 0.0;
+        }
+        S[S_MOLD] = m;
+        if (done == 0.0 && S[S_TOTC] >= S[S_MAXC]) done = 2.0;
+    }
+    const double rn = sqrt(rr);
+    if (done == 0.0 && !(rn > 0.0)) done = 1.0;
+    if (done != 0.0) {
+        S[S_DONE] = done;
+        S[S_REF] = 0.0;
+        S[S_NOREF] = 1.0;
+        return;
+    }
+    S[S_RN] = rn;
+    const double need = absm ? S[S_EPS] / fmax(rmax, 1e-300) : S[S_EPS] * S[S_BN] / rn;
+    const double tol = fmin(0.1, fmax(1e-5, 0.5 * need));
+    S2[S_DONE] = 0.0;
+    S2[S_THR] = tol * tol;
+    S2[S_MODE] = 0.0;
+    S2[S_CYC] = 0.0;
+    S2[S_HAVEP] = 0.0;
+}
+__global__ void __launch_bounds__(256) k_fin_ir(const double* __restrict__ part, int nb, double* S, double* S2) {
+    d_fin_ir(part, nb, S, S2);
+}
+
+// next refinement's inner rhs r / |r| in FP32 and inner iterate e = 0
+__device__ __forceinline__ void d_scale32_if(Geo G, const double* __restrict__ r, const double* __restrict__ S,
+                                             float* __restrict__ r32, float* __restrict__ e32) {
+    if (S[S_REF] == 0.0 || S[S_DONE] != 0.0) return;
+    long c;
+    if (!flat_interior(G, c)) return;
+    r32[c] = (float)((1.0 / S[S_RN]) * r[c]);
+    e32[c] = 0.0f;
+}
+__global__ void __launch_bounds__(256) k_scale32_if(Geo G, const double* __restrict__ r,
+                                                    const double* __restrict__ S, float* __restrict__ r32,
+                                                    float* __restrict__ e32) {
+    d_scale32_if(G, r, S, r32, e32);
 }
 
 // Bandwidth probes.  Each lane keeps four independent 16-byte loads in

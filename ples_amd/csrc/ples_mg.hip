@@ -114,6 +114,9 @@ struct ples_mg_handle {
     // at its own start and end (per-dispatch timestamps, no gap of their own)
     hipEvent_t kt_a = nullptr, kt_b = nullptr;
     bool kt_fired = false;
+    // slot of S whose nonzero value makes a grouped launch skip this problem
+    // (S_DONE; S_NOREF for the refinement tail of device-driven refinement)
+    int skip = S_DONE;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     hipEvent_t sev[4] = {};          // read-back ring of the refinement loop
     hipGraphExec_t gexec[2] = {};    // captured refinement step per (Z, W) pair
@@ -338,7 +341,7 @@ struct KT {
 template <auto F, int LB, int WPE, class K, class... A>
 void klaunch(H* h, K kern, dim3 g, dim3 b, size_t shm, A... a) {
     if (h->rec) {
-        const GEntry<A...> e{h->S + S_DONE, make_pack(a...)};
+        const GEntry<A...> e{h->S + h->skip, make_pack(a...)};
         KRec r{&launch_group<F, LB, WPE, A...>, g, b, shm, {}};
         r.e.assign((const unsigned char*)&e, (const unsigned char*)&e + sizeof e);
         if (g.z != 1) r.go = nullptr;  // refused when the batch groups the records
@@ -1014,32 +1017,71 @@ int step32_body(H* h, int q) {
     if (!direct) {
         const long n = L0.G.n;
         const int nb = (int)std::min<long>((n + 255) / 256, 8192);
-        hipLaunchKernelGGL((k_convert<float, float>), dim3(nb), dim3(256), 0, h->s, n, (const float*)L0.p[L0.cur],
-                           Zq);
+        KLAUNCH(h, convert, TPL(float, float), 1024, 1, dim3(nb), dim3(256), 0, n, (const float*)L0.p[L0.cur], Zq);
     }
     CKS(halo<float>(h, L0, Zq, 0.0, false, std::max(1, h->halo_passes32), true));
     const double N0 = (double)L0.ncell();
     const bool kry = h->o.krylov != 0;
     {
         KT kt(h, PLES_MG_K_OUTER_APPLY, N0 * (4 * nmet(L0) + 12 + (kry ? 4 : 0)));
-        auto go = [&](auto kern) {
-            hipLaunchKernelGGL(kern, cl.grid, cl.block, 0, h->s, L0.G, met<float>(L0), (const float*)Zq,
-                               (const float*)nullptr, Wq, 1.0f, (const float*)h->rr32,
-                               (const float*)(kry ? Wp : nullptr), h->part, cl.nblocks, L0.P);
+        auto go = [&](auto x0) {
+            constexpr bool X0 = decltype(x0)::value;
+            KLAUNCH(h, resid, TPL(2, float, float, float, float, float, X0), 512, 1, cl.grid, cl.block, 0, L0.G,
+                    met<float>(L0), (const float*)Zq, (const float*)nullptr, Wq, 1.0f, (const float*)h->rr32,
+                    (const float*)(kry ? Wp : nullptr), h->part, cl.nblocks, L0.P);
         };
-        if (L0.x0) go(k_resid<2, float, float, float, float, float, true>);
-        else go(k_resid<2, float, float, float, float, float, false>);
+        if (L0.x0) go(std::true_type{});
+        else go(std::false_type{});
     }
-    hipLaunchKernelGGL(k_fin_apply, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S2,
-                       h->o.krylov);
+    KLAUNCH0(h, fin_apply, 256, 1, dim3(1), dim3(256), 0, (const double*)h->part, cl.nblocks, h->S2, h->o.krylov);
     {
         KT kt(h, PLES_MG_K_OUTER_UPDATE, N0 * (24 + (kry ? 16 + 8 : 0)));
-        hipLaunchKernelGGL(k_gcr_update<float>, cl.grid, cl.block, 0, h->s, L0.G, h->e32, h->rr32, Zq, Wq,
-                           (const float*)Zp, (const float*)Wp, (const double*)h->S2, h->part, cl.nblocks, L0.P);
+        KLAUNCH(h, gcr_update, TPL(float), 512, 1, cl.grid, cl.block, 0, L0.G, h->e32, h->rr32, Zq, Wq,
+                (const float*)Zp, (const float*)Wp, (const double*)h->S2, h->part, cl.nblocks, L0.P);
     }
-    hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S2);
+    KLAUNCH0(h, fin_update, 256, 1, dim3(1), dim3(256), 0, (const double*)h->part, cl.nblocks, h->S2);
     CK(hipGetLastError());
     return PLES_MG_OK;
+}
+
+// Tail of a step of device-driven iterative refinement (after step32_body):
+// where the inner solve is done (or out of steps) it adds x += |r| e, takes
+// the FP64 residual, tests convergence and stalling and arms the next
+// refinement, all on the device; in a grouped launch the tail's field
+// kernels skip a problem whose step does not end a refinement (S_NOREF), and
+// every kernel of it tests S_REF itself.  1-D grids (groupable).
+int ir_tail(H* h) {
+    Level& L0 = h->lv[0];
+    const ColLaunch cl = seg_launch(L0.G, L0.P);
+    const dim3 g1((unsigned)((L0.G.n + 255) / 256)), b1(256);
+    const int sk = h->skip;
+    KLAUNCH0(h, ir_check, 64, 1, dim3(1), dim3(64), 0, h->S, (const double*)h->S2);
+    h->skip = S_NOREF;
+    KLAUNCH0(h, add_back_if, 256, 1, g1, b1, 0, L0.G, h->x, (const double*)h->S, (const float*)h->e32);
+    int st = halo64(h, h->x, true);
+    if (st == PLES_MG_OK) {
+        auto go = [&](auto x0) {
+            constexpr bool X0 = decltype(x0)::value;
+            KLAUNCH(h, resid, TPL(1, double, double, double, double, double, X0), 512, 1, cl.grid, cl.block, 0, L0.G,
+                    met<double>(L0), (const double*)h->x, (const double*)h->bvec, h->rvec, 1.0,
+                    (const double*)nullptr, (const double*)nullptr, h->part, cl.nblocks, L0.P);
+        };
+        if (L0.x0) go(std::true_type{});
+        else go(std::false_type{});
+        KLAUNCH0(h, fin_ir, 256, 1, dim3(1), dim3(256), 0, (const double*)h->part, cl.nblocks, h->S, h->S2);
+        KLAUNCH0(h, scale32_if, 256, 1, g1, b1, 0, L0.G, (const double*)h->rvec, (const double*)h->S, h->rr32,
+                 h->e32);
+    }
+    h->skip = sk;
+    CKS(st);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
+// one step of device-driven iterative refinement (batched mixed solves)
+int step_ir_body(H* h, int q) {
+    CKS(step32_body(h, q));
+    return ir_tail(h);
 }
 
 void drop_graphs32(H* h) {
@@ -2959,6 +3001,10 @@ struct ples_mg_batch {
     };
     std::vector<Fam> fam[2];
     void* gtab[2] = {};
+    // mixed handles with the FP32 inner Krylov: device-driven iterative
+    // refinement steps (inner FP32 GCR step + refinement tail), else FP64 GCR
+    // steps around the FP32 V-cycle
+    bool ir = false;
 };
 
 namespace {
@@ -3000,7 +3046,7 @@ int batch_group_prepare(B* b, int slot) {
         H* h = b->hs[i];
         h->rec = &rec[i];
         for (int k = 0; k < b->K && st == PLES_MG_OK; ++k) {
-            st = step_body(h, b->q[i]);
+            st = b->ir ? step_ir_body(h, b->q[i]) : step_body(h, b->q[i]);
             b->q[i] ^= 1;
         }
         h->rec = nullptr;
@@ -3088,7 +3134,7 @@ int batch_chunk_body(B* b) {
         H* h = b->hs[i];
         CK(hipStreamWaitEvent(h->s, b->fork, 0));
         for (int k = 0; k < b->K; ++k) {
-            CKS(step_body(h, b->q[i]));
+            CKS(b->ir ? step_ir_body(h, b->q[i]) : step_body(h, b->q[i]));
             b->q[i] ^= 1;
         }
         CK(hipEventRecord(b->join[i], h->s));
@@ -3161,6 +3207,143 @@ int batch_enqueue_chunk(B* b, bool use_graph) {
     return PLES_MG_OK;
 }
 
+// Batched mixed solves by device-driven iterative refinement: the host arms
+// each problem's first refinement (FP64 residual, FP32 rhs r / |r|, inner
+// tolerance: the single solve's first refinement), then replays graphs of K
+// steps of the whole batch (each an inner FP32 GCR step and the refinement
+// tail) and only reads the stop codes back.  Per problem the steps, decisions
+// and arithmetic are those of the single handle's refinement loop.
+int batch_solve_ir(B* b, double eps, int maxcycles, int K, ples_mg_result* res) {
+    const int n = (int)b->hs.size();
+    b->q.assign(n, 0);
+    bool use_graph = true;
+    for (H* h : b->hs) use_graph = use_graph && h->o.graphs && h->timing == 0;
+    hipEvent_t t0, t1;
+    CK(hipEventCreate(&t0));
+    CK(hipEventCreate(&t1));
+    CK(hipEventRecord(t0, b->ms));
+    std::vector<double> bnorm(n), rr(n), rmax(n);
+    std::vector<int> done(n, 0), cycles(n, 0), status(n, PLES_MG_NOT_CONVERGED);
+    for (int i = 0; i < n; ++i) {
+        H* h = b->hs[i];
+        CK(hipStreamWaitEvent(h->s, t0, 0));
+        if (h->o.project_rhs) CKS(project_rhs_async(h, h->cs_sum / h->dt, true));
+        CKS(outer_residual(h, h->x, h->rvec));
+        CK(hipMemcpyAsync(h->Sh, h->S, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, h->s));
+    }
+    for (int i = 0; i < n; ++i) {
+        H* h = b->hs[i];
+        CK(hipStreamSynchronize(h->s));
+        if (h->o.project_rhs) h->last_removed = removed_mean(h, h->cs_sum / h->dt, true);
+        const bool absmode = h->o.stop_mode == PLES_MG_STOP_ABS_MAX;
+        bnorm[i] = std::sqrt(h->Sh[S_BB]) > 0 ? std::sqrt(h->Sh[S_BB]) : 1.0;
+        rr[i] = h->Sh[S_RR];
+        rmax[i] = h->Sh[S_RMAX];
+        const double m = absmode ? rmax[i] : std::sqrt(rr[i]) / bnorm[i];
+        const double rn = std::sqrt(rr[i]);
+        if (!std::isfinite(rr[i])) {
+            status[i] = PLES_MG_ERR_NONFINITE;
+            done[i] = 1;
+        } else if (m < eps || !(rn > 0.0)) {
+            status[i] = PLES_MG_OK;
+            done[i] = 1;
+        }
+        const Level& L0 = h->lv[0];
+        if (!done[i]) {
+            const ColLaunch ce = cell_launch(L0.G);
+            hipLaunchKernelGGL(k_scale_to32, ce.grid, ce.block, 0, h->s, L0.G, (const double*)h->rvec, 1.0 / rn,
+                               h->rr32);
+            CK(hipMemsetAsync(h->e32, 0, (size_t)L0.G.n * sizeof(float), h->s));
+            CK(hipGetLastError());
+            const double need = absmode ? eps / std::max(rmax[i], 1e-300) : eps * bnorm[i] / rn;
+            const double tol = std::min(0.1, std::max(1e-5, 0.5 * need));
+            CKS(arm_stop(h, h->S2, tol * tol, 0));
+        }
+        // the refinement fields of the outer set (staging after arm_stop's)
+        double* st = h->Sh + 6 * S_COUNT + 8;
+        st[0] = rn;                             // S_RN
+        st[1] = 0.0;                            // S_STALL
+        st[2] = m;                              // S_MOLD
+        st[3] = eps;                            // S_EPS
+        st[4] = bnorm[i];                       // S_BN
+        st[5] = (double)maxcycles;              // S_MAXC
+        st[6] = absmode ? 1.0 : 0.0;            // S_ABS
+        st[7] = 0.0;                            // S_TOTC
+        st[8] = 0.0;                            // S_REF
+        st[9] = 1.0;                            // S_NOREF
+        st[10] = done[i] ? 1.0 : 0.0;           // S_DONE
+        CK(hipMemcpyAsync(h->S + S_RN, st, 10 * sizeof(double), hipMemcpyHostToDevice, h->s));
+        CK(hipMemcpyAsync(h->S + S_DONE, st + 10, sizeof(double), hipMemcpyHostToDevice, h->s));
+        CK(hipEventRecord(b->join[i], h->s));
+        CK(hipStreamWaitEvent(b->ms, b->join[i], 0));
+    }
+    auto all_done = [&]() {
+        for (int i = 0; i < n; ++i)
+            if (!done[i]) return false;
+        return true;
+    };
+    // every solve stops within maxcycles + 1 steps (the device caps the inner
+    // steps at the budget)
+    const int max_launch = (maxcycles + 1 + K - 1) / K + 1;
+    int enq = 0, seen = 0;
+    while (!all_done()) {
+        while (enq - seen < 2 && enq < max_launch) {
+            CKS(batch_enqueue_chunk(b, use_graph));
+            double* slot = b->ring + (size_t)(enq % 2) * n * S_COUNT;
+            CK(hipMemcpyAsync(slot, b->Sall, (size_t)n * S_COUNT * sizeof(double), hipMemcpyDeviceToHost, b->ms));
+            CK(hipEventRecord(b->rev[enq % 2], b->ms));
+            ++enq;
+        }
+        if (seen == enq) break;
+        CK(hipEventSynchronize(b->rev[seen % 2]));
+        const double* slot = b->ring + (size_t)(seen % 2) * n * S_COUNT;
+        ++seen;
+        for (int i = 0; i < n; ++i) {
+            if (done[i]) continue;
+            const double* Si = slot + (size_t)i * S_COUNT;
+            if (Si[S_DONE] == 0.0) continue;
+            done[i] = 1;
+            status[i] = Si[S_DONE] == 1.0 ? PLES_MG_OK : (Si[S_DONE] == 3.0 ? PLES_MG_ERR_NONFINITE : PLES_MG_NOT_CONVERGED);
+            cycles[i] = (int)Si[S_TOTC];
+            rr[i] = Si[S_RR];
+            rmax[i] = Si[S_RMAX];
+        }
+    }
+    CK(hipStreamSynchronize(b->ms));
+    for (int i = 0; i < n; ++i) {
+        H* h = b->hs[i];
+        if (!done[i] && seen > 0) {  // out of launches (not reached: the device caps every solve)
+            const double* Si = b->ring + (size_t)((seen - 1) % 2) * n * S_COUNT + (size_t)i * S_COUNT;
+            cycles[i] = (int)Si[S_TOTC];
+            rr[i] = Si[S_RR];
+            rmax[i] = Si[S_RMAX];
+        }
+        CKS(halo64(h, h->x, true));
+        CK(hipEventRecord(b->join[i], h->s));
+        CK(hipStreamWaitEvent(b->ms, b->join[i], 0));
+    }
+    CK(hipEventRecord(t1, b->ms));
+    CK(hipEventSynchronize(t1));
+    float ms = 0.f;
+    CK(hipEventElapsedTime(&ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    int worst = PLES_MG_OK;
+    for (int i = 0; i < n; ++i) {
+        H* h = b->hs[i];
+        if (res) {
+            res[i].status = status[i];
+            res[i].cycles = cycles[i];
+            res[i].rel_res = std::sqrt(rr[i]) / bnorm[i];
+            res[i].abs_res = rmax[i];
+            res[i].rhs_mean_removed = h->last_removed;
+            res[i].seconds = ms * 1e-3;
+        }
+        if (status[i] != PLES_MG_OK && worst == PLES_MG_OK) worst = status[i];
+    }
+    return worst;
+}
+
 int batch_set_done(B* b, int i, double v) {
     H* h = b->hs[i];
     h->Sh[S_DONE] = v;
@@ -3191,6 +3374,8 @@ int ples_mg_batch_create(int n, const ples_mg_grid* grids, const ples_mg_opts* o
     };
     if (st != PLES_MG_OK) return fail(st);
     b->dev = b->hs[0]->dev;
+    b->ir = true;
+    for (H* h : b->hs) b->ir = b->ir && h->ir32;
     if (hipSetDevice(b->dev) != hipSuccess) return fail(PLES_MG_ERR_HIP);
     if (hipStreamCreateWithFlags(&b->ms, hipStreamNonBlocking) != hipSuccess) return fail(PLES_MG_ERR_HIP);
     if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) return fail(PLES_MG_ERR_HIP);
@@ -3271,6 +3456,7 @@ int ples_mg_batch_solve(ples_mg_batch* b, double eps, int maxcycles, int steps_p
         batch_drop_graph(b);
         b->K = K;
     }
+    if (b->ir) return batch_solve_ir(b, eps, maxcycles, K, res);
     b->q.assign(n, 0);
     bool use_graph = true;
     for (H* h : b->hs) use_graph = use_graph && h->o.graphs && h->timing == 0;

@@ -19,23 +19,27 @@ def _problems(n=6):
     return PC.c4_problems(n=n, seed0=2100, sizes=SIZES)
 
 
-def _alone(pr, precision, tol):
-    # batches run the FP32 V-cycle directly inside the FP64 GCR (fp32_krylov = 0)
-    with mg.Solver(pr["x"], pr["y"], pr["z"], 0, 1, 0, pr["nlevel"], precision=precision, fp32_krylov=0) as s:
+def _alone(pr, precision, tol, fp32_krylov):
+    with mg.Solver(pr["x"], pr["y"], pr["z"], 0, 1, 0, pr["nlevel"], precision=precision,
+                   fp32_krylov=fp32_krylov) as s:
         s.case_rhs(pr["seed"])
         s.zero_fi()
         r = s.solve_resident(tol, 200)
         return r, s.get_fi()
 
 
-@pytest.mark.parametrize("precision,grouped", [("mixed", True), ("fp64", True), ("mixed", False)])
-def test_batch_equals_alone(precision, grouped):
+# mixed with fp32_krylov 1 (default): the batch runs the single solve's
+# iterative refinement (inner FP32 GCR) driven on the device; with 0, FP64 GCR
+# steps around the FP32 V-cycle, as the single solve with that option
+@pytest.mark.parametrize("precision,grouped,fp32_krylov",
+                         [("mixed", True, 1), ("mixed", False, 1), ("mixed", True, 0), ("fp64", True, 1)])
+def test_batch_equals_alone(precision, grouped, fp32_krylov):
     probs = _problems(10)   # 10 problems over 6 shapes: some families hold several
     shapes = {(p["shape"], p["nlevel"]) for p in probs}
     assert len(shapes) < len(probs)
     tol = 1e-8
-    ref = [_alone(pr, precision, tol) for pr in probs]
-    with mg.Batch(probs, precision=precision, grouped=grouped) as b:
+    ref = [_alone(pr, precision, tol, fp32_krylov) for pr in probs]
+    with mg.Batch(probs, precision=precision, grouped=grouped, fp32_krylov=fp32_krylov) as b:
         for steps in (2, 1, 4):
             for pr, s in zip(probs, b.solvers):
                 s.case_rhs(pr["seed"])

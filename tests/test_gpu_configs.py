@@ -133,7 +133,9 @@ def test_c3_warm_start_residual_under_the_oracle_operator(precision):
 
 
 def _alone(pr, tol):
-    with mg.Solver(pr["x"], pr["y"], pr["z"], 0, 1, 0, pr["nlevel"], precision="mixed", fp32_krylov=0) as s:
+    # the batch's handle options: the mixed solver's FP32 inner Krylov, which
+    # the batch runs as device-driven iterative refinement
+    with mg.Solver(pr["x"], pr["y"], pr["z"], 0, 1, 0, pr["nlevel"], precision="mixed") as s:
         s.case_rhs(pr["seed"])
         s.zero_fi()
         r = s.solve_resident(tol, 200)
