@@ -1891,6 +1891,35 @@ __global__ void __launch_bounds__(256) k_gj_elim(double* __restrict__ M, long nb
     M[r * w + c] -= fr * M[col * w + c];
 }
 
+// The exact coarse solve's map K_full = E K (ghosted rows x interior
+// columns): E's column m is the ghosted image of the unit vector e_m (stored
+// column-major, ng per column), K the inverse's top-left N x N block, the
+// right half of the Gauss-Jordan array M (row m at M + m w + nb).  An interior
+// row g is the inverse's row of its cell; a ghost row is the combination of
+// the inverse's rows its image entries weight.  One thread per (row g,
+// column c); a block row shares g, so E's entries are block-uniform.
+__global__ void __launch_bounds__(256) k_dense_map(Geo G, const double* __restrict__ E, const double* __restrict__ M,
+                                                   long nb, long N, double* __restrict__ KF) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long g = blockIdx.y;
+    if (c >= N) return;
+    const long w = 2 * nb;
+    const long NX = G.nx + 2, NY = G.ny + 2;
+    const int i = (int)(g % NX), j = (int)((g / NX) % NY), k = (int)(g / (NX * NY));
+    double acc = 0.0;
+    if (i >= 1 && i <= G.nx && j >= 1 && j <= G.ny && k >= 1 && k <= G.nz) {
+        const long m = (long)(i - 1) + (long)G.nx * ((j - 1) + (long)G.ny * (k - 1));
+        acc = M[m * w + nb + c];
+    } else {
+        const long ng = G.n;
+        for (long m = 0; m < N; ++m) {
+            const double e = E[m * ng + g];
+            if (e != 0.0) acc += e * M[m * w + nb + c];
+        }
+    }
+    KF[g * N + c] = acc;
+}
+
 // ------------------------------------------------------------------------
 // Smoother factors of the segmented eta-line solve (see LineF), computed in
 // FP64 and stored in FP32; the v and w solves use the stored factors so the

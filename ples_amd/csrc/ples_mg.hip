@@ -1537,40 +1537,72 @@ int check_grid(const ples_mg_grid* g) {
 // the level's shape, node stride and metrics, so a handle re-created for the
 // same grid (a served case admitted again after its eviction, a batch of
 // equal problems) takes the stored map instead of rebuilding it on the host
-// (tens of milliseconds).  Keyed by the shape and stride; a hit also compares
-// the metric bytes, so the map used is always the one these metrics give.
+// (tens to hundreds of milliseconds).  Keyed by the shape and stride; a hit
+// also compares the metric bytes, so the map used is always the one these
+// metrics give.  Least recently used entries leave first; the limit holds the
+// maps of a few dozen served cases with 2048-cell coarse levels (53 MB each).
+// Pinned host buffer (falls back to pageable memory if pinning fails): the
+// cache's stored maps (a hit is one DMA copy to the device, the buffer shared
+// with the handles copying from it while the cache may drop it) and the
+// server's node copies (uploaded at each cold start).
+struct PinnedBuf {
+    double* p = nullptr;
+    size_t n = 0;
+    bool pinned = false;
+    explicit PinnedBuf(size_t n_) : n(n_) {
+        if (hipHostMalloc((void**)&p, n * sizeof(double), hipHostMallocDefault) == hipSuccess) {
+            pinned = true;
+        } else {
+            (void)hipGetLastError();
+            p = new double[n];
+        }
+    }
+    ~PinnedBuf() {
+        if (pinned) (void)hipHostFree(p);
+        else delete[] p;
+    }
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+};
 struct DenseEntry {
     int nx, ny, nz, ip, jp, kp;
     double pot;
     std::vector<double> met;  // the nine metric arrays, concatenated
-    std::vector<double> map;
+    std::shared_ptr<const PinnedBuf> map;
 };
 std::mutex g_dense_mu;
-std::deque<DenseEntry> g_dense;  // oldest first
+// least recently used first; never destroyed (its pinned buffers are not
+// released during the runtime's own teardown at exit)
+std::deque<DenseEntry>& g_dense = *new std::deque<DenseEntry>;
 size_t g_dense_bytes = 0;
-constexpr size_t kDenseCacheMax = size_t(1) << 30;
+constexpr size_t kDenseCacheMax = size_t(4) << 30;
 
-bool dense_lookup(const Geo& G, double pot, const std::vector<double>& met, std::vector<double>& out) {
+std::shared_ptr<const PinnedBuf> dense_lookup(const Geo& G, double pot, const std::vector<double>& met) {
     std::lock_guard<std::mutex> lk(g_dense_mu);
-    for (const DenseEntry& e : g_dense)
+    for (auto it = g_dense.begin(); it != g_dense.end(); ++it) {
+        const DenseEntry& e = *it;
         if (e.nx == G.nx && e.ny == G.ny && e.nz == G.nz && e.ip == G.ip && e.jp == G.jp && e.kp == G.kp &&
             e.pot == pot && e.met.size() == met.size() &&
             std::memcmp(e.met.data(), met.data(), met.size() * sizeof(double)) == 0) {
-            out = e.map;
-            return true;
+            std::shared_ptr<const PinnedBuf> out = e.map;
+            DenseEntry hit = std::move(*it);  // most recently used: to the back
+            g_dense.erase(it);
+            g_dense.push_back(std::move(hit));
+            return out;
         }
-    return false;
+    }
+    return nullptr;
 }
 
-void dense_store(const Geo& G, double pot, std::vector<double> met, const std::vector<double>& map) {
-    const size_t b = (met.size() + map.size()) * sizeof(double);
+void dense_store(const Geo& G, double pot, std::vector<double> met, std::shared_ptr<const PinnedBuf> map) {
+    const size_t b = (met.size() + map->n) * sizeof(double);
     if (b > kDenseCacheMax) return;
     std::lock_guard<std::mutex> lk(g_dense_mu);
     while (!g_dense.empty() && g_dense_bytes + b > kDenseCacheMax) {
-        g_dense_bytes -= (g_dense.front().met.size() + g_dense.front().map.size()) * sizeof(double);
+        g_dense_bytes -= (g_dense.front().met.size() + g_dense.front().map->n) * sizeof(double);
         g_dense.pop_front();
     }
-    g_dense.push_back(DenseEntry{G.nx, G.ny, G.nz, G.ip, G.jp, G.kp, pot, std::move(met), map});
+    g_dense.push_back(DenseEntry{G.nx, G.ny, G.nz, G.ip, G.jp, G.kp, pot, std::move(met), std::move(map)});
     g_dense_bytes += b;
 }
 
@@ -1584,14 +1616,11 @@ int build_coarse_dense(H* h, Level& L) {
     std::vector<double> metcat;
     metcat.reserve((size_t)(9 * ng));
     for (int q = 0; q < 9; ++q) metcat.insert(metcat.end(), gm[q].begin(), gm[q].end());
-    {
-        std::vector<double> cached;
-        if (dense_lookup(G, L.pot, metcat, cached)) {
-            CKS(dalloc(h, (void**)&L.kd, (size_t)(ng * N) * 8));
-            CK(hipMemcpyAsync(L.kd, cached.data(), (size_t)(ng * N) * 8, hipMemcpyHostToDevice, h->s));
-            CK(hipStreamSynchronize(h->s));
-            return PLES_MG_OK;
-        }
+    if (std::shared_ptr<const PinnedBuf> cached = dense_lookup(G, L.pot, metcat)) {
+        CKS(dalloc(h, (void**)&L.kd, (size_t)(ng * N) * 8));
+        CK(hipMemcpyAsync(L.kd, cached->p, (size_t)(ng * N) * 8, hipMemcpyHostToDevice, h->s));
+        CK(hipStreamSynchronize(h->s));
+        return PLES_MG_OK;
     }
     Met<double> M;
     for (int q = 0; q < 9; ++q) M.g[q] = gm[q].data();
@@ -1603,60 +1632,71 @@ int build_coarse_dense(H* h, Level& L) {
     auto cell = [&](long c) { return G.idx((int)(c % G.nx) + 1, (int)((c / G.nx) % G.ny) + 1, (int)(c / ((long)G.nx * G.ny)) + 1); };
     std::vector<double> A((size_t)(N * N), 0.0);   // column-major
     std::vector<double> E((size_t)(ng * N), 0.0);  // column-major: ghosted image of e_m
-    std::vector<double> v((size_t)ng), prev((size_t)ng);
-    std::vector<long> seen((size_t)N, 0), rows((size_t)N);
-    long stamp = 0, nrows = 0;
     const double inv_pot = 1.0 / L.pot;
-    for (long m = 0; m < N; ++m) {
-        std::fill(v.begin(), v.end(), 0.0);
-        v[(size_t)cell(m)] = 1.0;
-        for (int pass = 0; pass < 400; ++pass) {
-            prev = v;
-            for (int set = 0; set < 3; ++set) {
-                const long nset = face_set_size(G, set);
-                for (int color = 0; color < 2; ++color)
-                    for (long t = 0; t < nset; ++t) halo_face_point(G, M, v.data(), t, color, bc0, set, 0);
+    // columns m are independent: each worker takes a contiguous range with
+    // its own work vectors (column m writes only column m of A and of E)
+    auto columns = [&](long m0, long m1) {
+        std::vector<double> v((size_t)ng), prev((size_t)ng);
+        std::vector<long> seen((size_t)N, -1), rows((size_t)N);
+        for (long m = m0; m < m1; ++m) {
+            std::fill(v.begin(), v.end(), 0.0);
+            v[(size_t)cell(m)] = 1.0;
+            for (int pass = 0; pass < 400; ++pass) {
+                prev = v;
+                for (int set = 0; set < 3; ++set) {
+                    const long nset = face_set_size(G, set);
+                    for (int color = 0; color < 2; ++color)
+                        for (long t = 0; t < nset; ++t) halo_face_point(G, M, v.data(), t, color, bc0, set, 0);
+                }
+                for (long t = 0; t < ne; ++t) halo_edge_point(G, v.data(), t);
+                double d = 0.0;
+                for (long t = 0; t < ng; ++t) d = std::max(d, std::fabs(v[(size_t)t] - prev[(size_t)t]));
+                if (d <= 1e-17) break;  // unit-scale vector: converged to rounding
             }
-            for (long t = 0; t < ne; ++t) halo_edge_point(G, v.data(), t);
-            double d = 0.0;
-            for (long t = 0; t < ng; ++t) d = std::max(d, std::fabs(v[(size_t)t] - prev[(size_t)t]));
-            if (d <= 1e-17) break;  // unit-scale vector: converged to rounding
-        }
-        std::copy(v.begin(), v.end(), E.begin() + (size_t)(m * ng));
-        // the 19-point stencil of a row reads only cells within one cell of it:
-        // rows next to a nonzero entry of the ghosted image (the unit cell, its
-        // periodic images, the wall ghosts it sets) are the column's nonzeros
-        ++stamp;
-        nrows = 0;
-        for (long gc = 0; gc < ng; ++gc) {
-            if (v[(size_t)gc] == 0.0) continue;
-            const int gi = (int)(gc % NX), gj = (int)((gc / NX) % NY), gk = (int)(gc / (NX * NY));
-            for (int dk = -1; dk <= 1; ++dk)
-                for (int dj = -1; dj <= 1; ++dj)
-                    for (int di = -1; di <= 1; ++di) {
-                        const int i = gi + di, j = gj + dj, k = gk + dk;
-                        if (i < 1 || i > G.nx || j < 1 || j > G.ny || k < 1 || k > G.nz) continue;
-                        const long r = (long)(i - 1) + (long)G.nx * ((j - 1) + (long)G.ny * (k - 1));
-                        if (seen[(size_t)r] != stamp) {
-                            seen[(size_t)r] = stamp;
-                            rows[(size_t)nrows++] = r;
+            std::copy(v.begin(), v.end(), E.begin() + (size_t)(m * ng));
+            // the 19-point stencil of a row reads only cells within one cell of
+            // it: rows next to a nonzero entry of the ghosted image (the unit
+            // cell, its periodic images, the wall ghosts it sets) are the
+            // column's nonzeros
+            long nrows = 0;
+            for (long gc = 0; gc < ng; ++gc) {
+                if (v[(size_t)gc] == 0.0) continue;
+                const int gi = (int)(gc % NX), gj = (int)((gc / NX) % NY), gk = (int)(gc / (NX * NY));
+                for (int dk = -1; dk <= 1; ++dk)
+                    for (int dj = -1; dj <= 1; ++dj)
+                        for (int di = -1; di <= 1; ++di) {
+                            const int i = gi + di, j = gj + dj, k = gk + dk;
+                            if (i < 1 || i > G.nx || j < 1 || j > G.ny || k < 1 || k > G.nz) continue;
+                            const long r = (long)(i - 1) + (long)G.nx * ((j - 1) + (long)G.ny * (k - 1));
+                            if (seen[(size_t)r] != m) {
+                                seen[(size_t)r] = m;
+                                rows[(size_t)nrows++] = r;
+                            }
                         }
-                    }
+            }
+            for (long q = 0; q < nrows; ++q) {
+                const long r = rows[(size_t)q];
+                const long c = cell(r);
+                Win<double> w;
+                win_init(w, v.data(), c - G.sy, G.sy, G.sz);
+                win_load(w, v.data(), c, G.sy, G.sz);
+                const G2<double> g0 = load_g2<double>(M, c), gmv = load_g2<double>(M, c - G.sy);
+                A[(size_t)(r + N * m)] = stencil(w, M, c, G.sz, gmv, g0, inv_pot);
+            }
         }
-        for (long q = 0; q < nrows; ++q) {
-            const long r = rows[(size_t)q];
-            const long c = cell(r);
-            Win<double> w;
-            win_init(w, v.data(), c - G.sy, G.sy, G.sz);
-            win_load(w, v.data(), c, G.sy, G.sz);
-            const G2<double> g0 = load_g2<double>(M, c), gmv = load_g2<double>(M, c - G.sy);
-            A[(size_t)(r + N * m)] = stencil(w, M, c, G.sz, gmv, g0, inv_pot);
-        }
+    };
+    {
+        const long nt = std::max<long>(1, std::min<long>({(long)std::thread::hardware_concurrency(), 16, N / 64}));
+        std::vector<std::thread> pool;
+        for (long t = 1; t < nt; ++t) pool.emplace_back(columns, N * t / nt, N * (t + 1) / nt);
+        columns(0, N / nt);
+        for (std::thread& th : pool) th.join();
     }
     // Gauss-Jordan inverse of the bordered matrix (partial pivoting) on the
-    // device: [B | I] is nb x 2nb, the right half becomes B^-1
+    // device: [B | I] is nb x 2nb, the right half becomes B^-1; then the map
+    // K_full = E K (ghosted rows x interior columns) from it on the device
     const long nb = N + 1, w = 2 * nb;
-    std::vector<double> I((size_t)(nb * nb), 0.0);  // row-major inverse
+    auto KF = std::make_shared<PinnedBuf>((size_t)(ng * N));
     {
         std::vector<double> Mh((size_t)(nb * w), 0.0);
         for (long r = 0; r < N; ++r) {
@@ -1667,11 +1707,15 @@ int build_coarse_dense(H* h, Level& L) {
         for (long r = 0; r < nb; ++r) Mh[(size_t)(r * w + nb + r)] = 1.0;
         double* Md = nullptr;
         double* fd = nullptr;
+        double* Ed = nullptr;
         int* std_ = nullptr;
         CKS(dalloc(h, (void**)&Md, (size_t)(nb * w) * 8));
         CKS(dalloc(h, (void**)&fd, (size_t)nb * 8));
+        CKS(dalloc(h, (void**)&Ed, (size_t)(ng * N) * 8));
         CKS(dalloc(h, (void**)&std_, sizeof(int)));
+        CKS(dalloc(h, (void**)&L.kd, (size_t)(ng * N) * 8));
         CK(hipMemcpyAsync(Md, Mh.data(), (size_t)(nb * w) * 8, hipMemcpyHostToDevice, h->s));
+        CK(hipMemcpyAsync(Ed, E.data(), (size_t)(ng * N) * 8, hipMemcpyHostToDevice, h->s));
         CK(hipMemsetAsync(std_, 0, sizeof(int), h->s));
         for (long col = 0; col < nb; ++col) {
             hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, h->s, Md, nb, col, fd, std_);
@@ -1680,35 +1724,22 @@ int build_coarse_dense(H* h, Level& L) {
                 hipLaunchKernelGGL(k_gj_elim, dim3((unsigned)((nc + 255) / 256), (unsigned)nb), dim3(256), 0, h->s, Md,
                                    nb, col, (const double*)fd);
         }
+        hipLaunchKernelGGL(k_dense_map, dim3((unsigned)((N + 255) / 256), (unsigned)ng), dim3(256), 0, h->s, G,
+                           (const double*)Ed, (const double*)Md, nb, N, L.kd);
         CK(hipGetLastError());
         int sing = 0;
-        CK(hipMemcpy2DAsync(I.data(), (size_t)nb * 8, Md + nb, (size_t)w * 8, (size_t)nb * 8, (size_t)nb,
-                            hipMemcpyDeviceToHost, h->s));
+        CK(hipMemcpyAsync(KF->p, L.kd, (size_t)(ng * N) * 8, hipMemcpyDeviceToHost, h->s));
         CK(hipMemcpyAsync(&sing, std_, sizeof(int), hipMemcpyDeviceToHost, h->s));
         CK(hipStreamSynchronize(h->s));
         dfree(h, Md);
         dfree(h, fd);
+        dfree(h, Ed);
         dfree(h, std_);
         if (sing) return PLES_MG_ERR_NONFINITE;
     }
-    // K_full = E K (row-major, ghosted rows x interior columns)
-    std::vector<double> KF((size_t)(ng * N), 0.0);
-    for (long m = 0; m < N; ++m) {
-        const double* e = E.data() + (size_t)(m * ng);
-        const double* krow = I.data() + (size_t)(m * nb);  // row m of K
-        for (long g = 0; g < ng; ++g) {
-            const double em = e[(size_t)g];
-            if (em == 0.0) continue;
-            double* out = KF.data() + (size_t)(g * N);
-            for (long c = 0; c < N; ++c) out[(size_t)c] += em * krow[(size_t)c];
-        }
-    }
-    for (double x : KF)
-        if (!std::isfinite(x)) return PLES_MG_ERR_NONFINITE;
-    dense_store(G, L.pot, std::move(metcat), KF);
-    CKS(dalloc(h, (void**)&L.kd, (size_t)(ng * N) * 8));
-    CK(hipMemcpyAsync(L.kd, KF.data(), (size_t)(ng * N) * 8, hipMemcpyHostToDevice, h->s));
-    CK(hipStreamSynchronize(h->s));
+    for (size_t t = 0; t < KF->n; ++t)
+        if (!std::isfinite(KF->p[t])) return PLES_MG_ERR_NONFINITE;
+    dense_store(G, L.pot, std::move(metcat), std::move(KF));
     return PLES_MG_OK;
 }
 
@@ -3678,7 +3709,7 @@ struct Creation {
 struct SCase {
     uint64_t id = 0;
     int jx = 0, jy = 0, jz = 0, ip = 0, kp = 0, nlevel = 1;
-    std::vector<double> x, y, z;  // host copy of the nodes
+    std::shared_ptr<PinnedBuf> x, y, z;  // host copy of the nodes (pinned)
     uint64_t rhs_seed = 0;
     double dt = 1.0;
     H* h = nullptr;               // resident handle (nullptr = cold)
@@ -3742,7 +3773,7 @@ ples_mg_grid case_grid(const SCase& c) {
     g.jx = c.jx; g.jy = c.jy; g.jz = c.jz;
     g.ip = c.ip; g.jp = 1; g.kp = c.kp;
     g.nlevel = c.nlevel;
-    g.x = c.x.data(); g.y = c.y.data(); g.z = c.z.data();
+    g.x = c.x->p; g.y = c.y->p; g.z = c.z->p;
     return g;
 }
 
@@ -4066,9 +4097,12 @@ int ples_mg_server_add_case(ples_mg_server* sv, uint64_t case_id, const ples_mg_
     c.ip = g->ip; c.kp = g->kp;
     c.nlevel = g->nlevel;
     const size_t nn = (size_t)(g->jx + 1) * (g->jy + 1) * (g->jz + 1);
-    c.x.assign(g->x, g->x + nn);
-    c.y.assign(g->y, g->y + nn);
-    c.z.assign(g->z, g->z + nn);
+    std::shared_ptr<PinnedBuf>* dst[3] = {&c.x, &c.y, &c.z};
+    const double* src[3] = {g->x, g->y, g->z};
+    for (int q = 0; q < 3; ++q) {
+        *dst[q] = std::make_shared<PinnedBuf>(nn);
+        std::memcpy((*dst[q])->p, src[q], nn * sizeof(double));
+    }
     c.rhs_seed = rhs_seed;
     c.dt = dt;
     c.est = estimate(g, &sv->o) + ((size_t)64 << 20);  // + allocator slack and temporaries
