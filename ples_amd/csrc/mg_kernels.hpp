@@ -1805,28 +1805,79 @@ __global__ void k_halo_edges(Geo G, T* __restrict__ p) {
 
 // Exact coarsest-level solve: p (every ghosted position) = K b, K the dense
 // map built at set-up (rows: ghosted positions, columns: interior cells, i
-// fastest), b the interior of the ghosted rhs.  One wavefront per row, lanes
-// stride the columns, fixed shuffle tree: deterministic.
-__device__ __forceinline__ void d_coarse_dense(Geo G, const double* __restrict__ K, int N,
-                                               const double* __restrict__ b, double* __restrict__ p) {
+// fastest, each row padded with zeros to kp columns, a multiple of 256), b
+// the interior of the ghosted rhs, gathered through gi (the ghosted index of
+// each interior cell).  The block stages b's interior in LDS (the dense solve
+// is for levels of at most kDenseCols cells); each wavefront takes one row
+// and issues all its 16-byte loads of the row before the first product (an
+// FP64 row of 2048 entries is 16 loads per lane), then sums in a fixed order
+// and a fixed shuffle tree: deterministic.  An FP32 map (the mixed solver's:
+// the V-cycle it preconditions runs in FP32) halves the bytes; sums are FP64.
+constexpr int kDenseCols = 2048, kDenseWaves = 8;
+template <class TK>
+__device__ __forceinline__ void d_coarse_dense(Geo G, const TK* __restrict__ K, const int* __restrict__ gi,
+                                               int N, int kp, const double* __restrict__ b,
+                                               double* __restrict__ p) {
+    __shared__ double bs[kDenseCols];
+    constexpr int VW = 16 / sizeof(TK);          // map entries per 16-byte load
+    constexpr int NL = kDenseCols / (64 * VW);   // loads per lane of the longest row
+    using V = typename std::conditional<sizeof(TK) == 4, float4, double2>::type;
     const int lane = threadIdx.x & 63;
-    const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= G.n) return;  // whole wavefronts leave together
-    const double* kr = K + row * (long)N;
-    double acc = 0.0;
-    for (int c = lane; c < N; c += 64) {
-        const int i = c % G.nx + 1, j = (c / G.nx) % G.ny + 1, k = c / (G.nx * G.ny) + 1;
-        acc += kr[c] * b[G.idx(i, j, k)];
+    const long row = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const bool act = row < G.n;  // uniform over the wavefront
+    // the row's loads first: they are in flight while the block stages b
+    V kv[NL];
+    if (act) {
+        const V* kr = reinterpret_cast<const V*>(K + row * (long)kp) + lane;
+#pragma unroll
+        for (int u = 0; u < NL; ++u)
+            if (u * 64 * VW < kp) kv[u] = kr[u * 64];
     }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (lane == 0) p[row] = acc;
+    for (int c = threadIdx.x; c < kp; c += blockDim.x) bs[c] = (c < N) ? b[gi[c]] : 0.0;
+    __syncthreads();
+    if (!act) return;
+    double a = 0.0;
+#pragma unroll
+    for (int u = 0; u < NL; ++u) {
+        if (u * 64 * VW >= kp) continue;  // uniform; no break, so the loop unrolls and kv stays in registers
+        const int c = (u * 64 + lane) * VW;
+        if constexpr (VW == 4) {
+            a += ((double)kv[u].x * bs[c] + (double)kv[u].y * bs[c + 1]) +
+                 ((double)kv[u].z * bs[c + 2] + (double)kv[u].w * bs[c + 3]);
+        } else {
+            a += (double)kv[u].x * bs[c] + (double)kv[u].y * bs[c + 1];
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+    if (lane == 0) p[row] = a;
 }
-__global__ void __launch_bounds__(256) k_coarse_dense(Geo G, const double* __restrict__ K, int N,
-                                                      const double* __restrict__ b, double* __restrict__ p) {
-    d_coarse_dense(G, K, N, b, p);
+template <class TK>
+__global__ void __launch_bounds__(64 * kDenseWaves) k_coarse_dense(Geo G, const TK* __restrict__ K,
+                                                                   const int* __restrict__ gi, int N, int kp,
+                                                                   const double* __restrict__ b,
+                                                                   double* __restrict__ p) {
+    d_coarse_dense<TK>(G, K, gi, N, kp, b, p);
 }
 
+// ghosted index of each interior cell (i fastest): the dense solve's gather
+__global__ void __launch_bounds__(256) k_interior_index(Geo G, int* __restrict__ gi) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (long)G.nx * G.ny * G.nz) return;
+    const int i = (int)(c % G.nx) + 1, j = (int)((c / G.nx) % G.ny) + 1, k = (int)(c / ((long)G.nx * G.ny)) + 1;
+    gi[c] = (int)G.idx(i, j, k);
+}
 
+// the set-up's FP64 map (rows of N entries) in the solve's layout: rows
+// padded with zeros to kp entries, converted to TK
+template <class TK>
+__global__ void __launch_bounds__(256) k_pad_map(const double* __restrict__ src, long rows, int N, int kp,
+                                                 TK* __restrict__ dst) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= rows * kp) return;
+    const long r = t / kp;
+    const int c = (int)(t % kp);
+    dst[t] = (TK)(c < N ? src[r * N + c] : 0.0);
+}
 
 // ------------------------------------------------------------------------
 // Gauss-Jordan inverse with partial pivoting of an nb x nb matrix B, held

@@ -40,7 +40,10 @@ struct Level {
     double h16s = 1.0;      // 2^-m: 1/pot of the smoother reading g16
     bool use16 = false;     // the smoother reads g16
     bool x0 = false;        // g13 and g31 identically zero: stencil kernels skip them
-    double* kd = nullptr;   // coarsest level: dense exact solve map (ghosted x interior)
+    void* kd = nullptr;     // coarsest level: dense exact solve map (ghosted x interior, rows padded to kdp)
+    bool kd32 = false;      // the map is FP32 (mixed solver), else FP64
+    int kdp = 0;            // padded row length of the map
+    int* kdi = nullptr;     // ghosted index of each interior cell (the map's columns)
     _Float16* v16 = nullptr;  // 16-bit copies of lv_v, lv_w (levels with g16)
     _Float16* w16 = nullptr;
     H2* pk_a = nullptr;       // packed 16-bit smoother metrics (levels with g16 and x0)
@@ -772,6 +775,7 @@ int lv_prolong(H* h, int n) {  // n+1 -> n
 // transfers and the levels below it).  Levels with xi or zeta walls keep 512:
 // the set-up's ghost fixed point of each unit vector converges slowly there.
 constexpr long kCoarseDenseMax = 2048, kCoarseDenseMaxWalls = 512;
+static_assert(kCoarseDenseMax <= kDenseCols, "the dense solve stages at most kDenseCols rhs values");
 long dense_max(int ip, int kp) { return (ip == 0 && kp == 0) ? kCoarseDenseMax : kCoarseDenseMaxWalls; }
 // interval of the Chebyshev sweep schedule on the spectrum of M_line^-1 A
 // (measured on the terrain workloads: one V-cycle fewer than omega = 0.8 at
@@ -791,9 +795,14 @@ size_t coarse_lds_bytes(const Level& L) { return (size_t)L.G.n * (12 * sizeof(do
 
 int coarse_solve(H* h, Level& L, const double* rhs, int nu) {
     if (L.kd) {  // exact: one matrix-vector product gives the ghosted solution
-        KT kt(h, PLES_MG_K_SMOOTH, (double)L.G.n * L.ncell() * 8.0);
-        KLAUNCH0(h, coarse_dense, 256, 1, dim3((unsigned)((L.G.n + 3) / 4)), dim3(256), 0, L.G,
-                 (const double*)L.kd, (int)L.ncell(), rhs, (double*)L.p[1 - L.cur]);
+        KT kt(h, PLES_MG_K_SMOOTH, (double)L.G.n * L.kdp * (L.kd32 ? 4.0 : 8.0));
+        const dim3 grid((unsigned)((L.G.n + kDenseWaves - 1) / kDenseWaves)), blk(64 * kDenseWaves);
+        if (L.kd32)
+            KLAUNCH(h, coarse_dense, TPL(float), 64 * kDenseWaves, 1, grid, blk, 0, L.G, (const float*)L.kd,
+                    (const int*)L.kdi, (int)L.ncell(), L.kdp, rhs, (double*)L.p[1 - L.cur]);
+        else
+            KLAUNCH(h, coarse_dense, TPL(double), 64 * kDenseWaves, 1, grid, blk, 0, L.G, (const double*)L.kd,
+                    (const int*)L.kdi, (int)L.ncell(), L.kdp, rhs, (double*)L.p[1 - L.cur]);
         CK(hipGetLastError());
         L.cur ^= 1;
         return PLES_MG_OK;
@@ -1606,6 +1615,25 @@ void dense_store(const Geo& G, double pot, std::vector<double> met, std::shared_
     g_dense_bytes += b;
 }
 
+// the solve's copy of a level's FP64 dense map (device, rows of N entries):
+// rows padded to a multiple of 256 entries, FP32 for the mixed solver
+int install_map(H* h, Level& L, const double* kf) {
+    const long ng = L.G.n;
+    const int N = (int)L.ncell();
+    L.kdp = (N + 255) / 256 * 256;
+    L.kd32 = h->o.precision == PLES_MG_MIXED;
+    CKS(dalloc(h, &L.kd, (size_t)ng * L.kdp * (L.kd32 ? 4 : 8)));
+    CKS(dalloc(h, (void**)&L.kdi, (size_t)N * sizeof(int)));
+    hipLaunchKernelGGL(k_interior_index, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->s, L.G, L.kdi);
+    const dim3 grid((unsigned)((ng * L.kdp + 255) / 256));
+    if (L.kd32)
+        hipLaunchKernelGGL(k_pad_map<float>, grid, dim3(256), 0, h->s, kf, ng, N, L.kdp, (float*)L.kd);
+    else
+        hipLaunchKernelGGL(k_pad_map<double>, grid, dim3(256), 0, h->s, kf, ng, N, L.kdp, (double*)L.kd);
+    CK(hipGetLastError());
+    return PLES_MG_OK;
+}
+
 int build_coarse_dense(H* h, Level& L) {
     const Geo& G = L.G;
     const long N = L.ncell(), ng = G.n;
@@ -1617,9 +1645,12 @@ int build_coarse_dense(H* h, Level& L) {
     metcat.reserve((size_t)(9 * ng));
     for (int q = 0; q < 9; ++q) metcat.insert(metcat.end(), gm[q].begin(), gm[q].end());
     if (std::shared_ptr<const PinnedBuf> cached = dense_lookup(G, L.pot, metcat)) {
-        CKS(dalloc(h, (void**)&L.kd, (size_t)(ng * N) * 8));
-        CK(hipMemcpyAsync(L.kd, cached->p, (size_t)(ng * N) * 8, hipMemcpyHostToDevice, h->s));
+        double* kf = nullptr;
+        CKS(dalloc(h, (void**)&kf, (size_t)(ng * N) * 8));
+        CK(hipMemcpyAsync(kf, cached->p, (size_t)(ng * N) * 8, hipMemcpyHostToDevice, h->s));
+        CKS(install_map(h, L, kf));
         CK(hipStreamSynchronize(h->s));
+        dfree(h, kf);
         return PLES_MG_OK;
     }
     Met<double> M;
@@ -1697,6 +1728,7 @@ int build_coarse_dense(H* h, Level& L) {
     // K_full = E K (ghosted rows x interior columns) from it on the device
     const long nb = N + 1, w = 2 * nb;
     auto KF = std::make_shared<PinnedBuf>((size_t)(ng * N));
+    double* kf = nullptr;  // the FP64 map on the device
     {
         std::vector<double> Mh((size_t)(nb * w), 0.0);
         for (long r = 0; r < N; ++r) {
@@ -1713,7 +1745,7 @@ int build_coarse_dense(H* h, Level& L) {
         CKS(dalloc(h, (void**)&fd, (size_t)nb * 8));
         CKS(dalloc(h, (void**)&Ed, (size_t)(ng * N) * 8));
         CKS(dalloc(h, (void**)&std_, sizeof(int)));
-        CKS(dalloc(h, (void**)&L.kd, (size_t)(ng * N) * 8));
+        CKS(dalloc(h, (void**)&kf, (size_t)(ng * N) * 8));
         CK(hipMemcpyAsync(Md, Mh.data(), (size_t)(nb * w) * 8, hipMemcpyHostToDevice, h->s));
         CK(hipMemcpyAsync(Ed, E.data(), (size_t)(ng * N) * 8, hipMemcpyHostToDevice, h->s));
         CK(hipMemsetAsync(std_, 0, sizeof(int), h->s));
@@ -1725,20 +1757,29 @@ int build_coarse_dense(H* h, Level& L) {
                                    nb, col, (const double*)fd);
         }
         hipLaunchKernelGGL(k_dense_map, dim3((unsigned)((N + 255) / 256), (unsigned)ng), dim3(256), 0, h->s, G,
-                           (const double*)Ed, (const double*)Md, nb, N, L.kd);
+                           (const double*)Ed, (const double*)Md, nb, N, kf);
         CK(hipGetLastError());
         int sing = 0;
-        CK(hipMemcpyAsync(KF->p, L.kd, (size_t)(ng * N) * 8, hipMemcpyDeviceToHost, h->s));
+        CK(hipMemcpyAsync(KF->p, kf, (size_t)(ng * N) * 8, hipMemcpyDeviceToHost, h->s));
         CK(hipMemcpyAsync(&sing, std_, sizeof(int), hipMemcpyDeviceToHost, h->s));
         CK(hipStreamSynchronize(h->s));
         dfree(h, Md);
         dfree(h, fd);
         dfree(h, Ed);
         dfree(h, std_);
-        if (sing) return PLES_MG_ERR_NONFINITE;
+        if (sing) {
+            dfree(h, kf);
+            return PLES_MG_ERR_NONFINITE;
+        }
     }
     for (size_t t = 0; t < KF->n; ++t)
-        if (!std::isfinite(KF->p[t])) return PLES_MG_ERR_NONFINITE;
+        if (!std::isfinite(KF->p[t])) {
+            dfree(h, kf);
+            return PLES_MG_ERR_NONFINITE;
+        }
+    CKS(install_map(h, L, kf));
+    CK(hipStreamSynchronize(h->s));
+    dfree(h, kf);
     dense_store(G, L.pot, std::move(metcat), std::move(KF));
     return PLES_MG_OK;
 }
