@@ -1064,6 +1064,58 @@ __global__ void __launch_bounds__(512) k_resid(Geo G, Met<TM> M, const T* __rest
 // the four (j, k) children of the thread's own column.  Segments have an even
 // number of rows and start at an odd row.  Block: bx lanes along i x P
 // segments; XCD-aware block order.
+// Rows of the fused residual-restriction with the next row of each plane
+// loaded while a row is computed and the eta flux carried between rows (the
+// FP64 kernel, whose loads-at-use form is scheduled by the compiler with one
+// load in flight: 1.7x slower; and the FP32 one with the packed 16-bit
+// metrics of the smoother, a preconditioner-side residual whose metric
+// rounding, like the smoother's, only perturbs the coarse correction).
+template <class T, class MV, class TR, class TC, bool X0>
+__device__ __forceinline__ void resid_restrict_rows(const Geo& G, const MV& M, const T* __restrict__ p,
+                                                    const TR* __restrict__ b, const Geo& Gc, TC* __restrict__ rc,
+                                                    T inv_pot, int m, int i, int j0, int kc, int tx, bool act,
+                                                    long cb0, long cb1, int fx) {
+    using RI = RowIn<T, MV, TR, X0, true>;
+    const LineF F{};
+    const long sy = G.sy, sz = G.sz;
+    Win<T> w0, w1;
+    T F0 = 0, F1 = 0;
+    RI n0, n1;
+    if (act) {
+        win_init(w0, p, cb0, sy, sz);
+        win_init(w1, p, cb1, sy, sz);
+        F0 = flux_eta_lo(w0, (T)mg(M, 3, cb0), (T)mg(M, 4, cb0), (T)mg(M, 5, cb0), inv_pot);
+        F1 = flux_eta_lo(w1, (T)mg(M, 3, cb1), (T)mg(M, 4, cb1), (T)mg(M, 5, cb1), inv_pot);
+        n0.load(p, b, M, F, cb0 + sy, sy, sz);
+        n1.load(p, b, M, F, cb1 + sy, sy, sz);
+    }
+    for (int rp = 0; rp < m / 2; ++rp) {
+        T v = 0;
+        if (act) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int row = 2 * rp + h;
+                const RI c0 = n0, c1 = n1;
+                if (row + 1 < m) {
+                    n0.load(p, b, M, F, cb0 + sy * (row + 2), sy, sz);
+                    n1.load(p, b, M, F, cb1 + sy * (row + 2), sy, sz);
+                }
+                T mm[15];
+                v += row_apply(w0, c0, F0, inv_pot, mm) - (T)c0.rhs;
+                v += row_apply(w1, c1, F1, inv_pot, mm) - (T)c1.rhs;
+                win_shift(w0);
+                win_shift(w1);
+            }
+        }
+        if (fx == 1) {
+            if (act) rc[Gc.idx(i, (j0 + 1) / 2 + rp, kc)] = (TC)(-v * (T)0.25);
+            continue;
+        }
+        v += __shfl_down(v, 1, 64);
+        if (act && !(tx & 1)) rc[Gc.idx((i + 1) >> 1, (j0 + 1) / 2 + rp, kc)] = (TC)(-v * (T)0.125);
+    }
+}
+
 template <class T, class TM, class TR, class TC, bool X0 = false>
 __device__ __forceinline__ void d_resid_restrict(Geo G, Met<TM> M, const T* __restrict__ p,
                                                  const TR* __restrict__ b, Geo Gc, TC* __restrict__ rc,
@@ -1077,47 +1129,8 @@ __device__ __forceinline__ void d_resid_restrict(Geo G, Met<TM> M, const T* __re
     const long sy = G.sy, sz = G.sz;
     const long cb0 = act ? G.idx(i, j0 - 1, 2 * kc - 1) : 0, cb1 = cb0 + sz;
     if constexpr (sizeof(T) == 8) {
-        // FP64: each plane's next row is loaded while a row is computed, the
-        // eta flux carried between rows (the loads-at-use form below is
-        // scheduled by the compiler with one load in flight in FP64: 1.7x slower)
-        using RI = RowIn<T, Met<TM>, TR, X0, true>;
-        const LineF F{};
-        Win<T> w0, w1;
-        T F0 = 0, F1 = 0;
-        RI n0, n1;
-        if (act) {
-            win_init(w0, p, cb0, sy, sz);
-            win_init(w1, p, cb1, sy, sz);
-            F0 = flux_eta_lo(w0, (T)M.g[3][cb0], (T)M.g[4][cb0], (T)M.g[5][cb0], inv_pot);
-            F1 = flux_eta_lo(w1, (T)M.g[3][cb1], (T)M.g[4][cb1], (T)M.g[5][cb1], inv_pot);
-            n0.load(p, b, M, F, cb0 + sy, sy, sz);
-            n1.load(p, b, M, F, cb1 + sy, sy, sz);
-        }
-        for (int rp = 0; rp < m / 2; ++rp) {
-            T v = 0;
-            if (act) {
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int row = 2 * rp + h;
-                    const RI c0 = n0, c1 = n1;
-                    if (row + 1 < m) {
-                        n0.load(p, b, M, F, cb0 + sy * (row + 2), sy, sz);
-                        n1.load(p, b, M, F, cb1 + sy * (row + 2), sy, sz);
-                    }
-                    T mm[15];
-                    v += row_apply(w0, c0, F0, inv_pot, mm) - (T)c0.rhs;
-                    v += row_apply(w1, c1, F1, inv_pot, mm) - (T)c1.rhs;
-                    win_shift(w0);
-                    win_shift(w1);
-                }
-            }
-            if (fx == 1) {
-                if (act) rc[Gc.idx(i, (j0 + 1) / 2 + rp, kc)] = (TC)(-v * (T)0.25);
-                continue;
-            }
-            v += __shfl_down(v, 1, 64);
-            if (act && !(tx & 1)) rc[Gc.idx((i + 1) >> 1, (j0 + 1) / 2 + rp, kc)] = (TC)(-v * (T)0.125);
-        }
+        resid_restrict_rows<T, Met<TM>, TR, TC, X0>(G, M, p, b, Gc, rc, inv_pot, m, i, j0, kc, tx, act, cb0, cb1,
+                                                      fx);
         return;
     }
     Win<T> w0, w1;
@@ -1158,6 +1171,28 @@ __global__ void __launch_bounds__(512) k_resid_restrict(Geo G, Met<TM> M, const 
                                                         const TR* __restrict__ b, Geo Gc, TC* __restrict__ rc,
                                                         T inv_pot, int P, int fx) {
     d_resid_restrict<T, TM, TR, TC, X0>(G, M, p, b, Gc, rc, inv_pot, P, fx);
+}
+// FP32 fused residual-restriction reading the packed 16-bit metrics (levels
+// without g13/g31 whose smoother reads them): same launch shape and XCD order
+template <class TR, class TC>
+__device__ __forceinline__ void d_resid_restrict_pk(Geo G, MetPk M, const float* __restrict__ p,
+                                                    const TR* __restrict__ b, Geo Gc, TC* __restrict__ rc,
+                                                    float inv_pot, int P, int fx) {
+    int bxi, bk;
+    xcd_tile(bxi, bk);
+    const int bx = blockDim.x, tx = threadIdx.x, s = threadIdx.y;
+    const int m = G.ny / P;
+    const int i = bxi * bx + tx + 1, kc = bk + 1, j0 = s * m + 1;
+    const bool act = i <= G.nx;
+    const long cb0 = act ? G.idx(i, j0 - 1, 2 * kc - 1) : 0, cb1 = cb0 + G.sz;
+    resid_restrict_rows<float, MetPk, TR, TC, true>(G, M, p, b, Gc, rc, inv_pot, m, i, j0, kc, tx, act, cb0, cb1,
+                                                     fx);
+}
+template <class TR, class TC>
+__global__ void __launch_bounds__(512) k_resid_restrict_pk(Geo G, MetPk M, const float* __restrict__ p,
+                                                           const TR* __restrict__ b, Geo Gc, TC* __restrict__ rc,
+                                                           float inv_pot, int P, int fx) {
+    d_resid_restrict_pk<TR, TC>(G, M, p, b, Gc, rc, inv_pot, P, fx);
 }
 
 

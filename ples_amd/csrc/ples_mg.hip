@@ -662,6 +662,17 @@ int resid_restrict(H* h, Level& F, Level& Cc, const TR* rhs, bool fine) {
     if ((bx * P) % 64 != 0 || (fx == 2 && (bx & 1))) return 1;
     const dim3 block((unsigned)bx, (unsigned)P, 1), grid((unsigned)((F.G.nx + bx - 1) / bx), (unsigned)(F.G.nz / 2));
     const double N = (double)F.ncell();
+    if constexpr (sizeof(T) == 4) {
+        if (F.x0 && F.pk_a) {  // the smoother's packed 16-bit metrics: 14 B per cell instead of 28
+            KT kt(h, fine ? PLES_MG_K_RESID_FINE : PLES_MG_K_RESID,
+                  N * (14.0 + sizeof(T) + sizeof(TR)) + (double)Cc.ncell() * sizeof(TC), true);
+            const MetPk mp{F.pk_a, F.pk_z, F.pk_b, F.g16[4]};
+            KLAUNCH(h, resid_restrict_pk, TPL(TR, TC), 512, 1, grid, block, 0, F.G, mp, (const float*)F.p[F.cur],
+                    rhs, Cc.G, (TC*)Cc.b, (float)(F.h16s / F.pot), P, fx);  // metrics scaled by 2^2m
+            CK(hipGetLastError());
+            return PLES_MG_OK;
+        }
+    }
     KT kt(h, fine ? PLES_MG_K_RESID_FINE : PLES_MG_K_RESID,
           N * (nmet(F) * sizeof(T) + sizeof(T) + sizeof(TR)) + (double)Cc.ncell() * sizeof(TC), true);
     auto go = [&](auto x0) {
