@@ -97,15 +97,17 @@ typedef struct ples_mg_opts {
     int fp32_krylov; /* PLES_MG_MIXED only: 1 = FP64 iterative refinement around
                         an FP32 GCR(2) preconditioned by the FP32 V-cycle (default);
                         0 = the FP32 V-cycle directly inside the FP64 GCR(2).
-                        Batched and served solves use the latter. */
+                        Batched solves follow the option (the refinement loop
+                        driven on the device); served solves use the latter. */
     int smoother_fp16; /* PLES_MG_MIXED only: 1 = the smoother of the FP32 levels
                         reads a 16-bit copy of the level's metrics (scaled by a
-                        power of four, folded into 1/pot); the V-cycle residual,
-                        the FP32 operator and the FP64 outer residual keep their
-                        metrics, so only the preconditioner changes, not the
-                        solution.  Used on the fine level (the autotuner may
-                        choose per level).  0 = FP32 metrics everywhere.
-                        Default 1 */
+                        power of four, folded into 1/pot), and so does the
+                        V-cycle's fused residual-restriction on levels without
+                        g13/g31; the FP32 operator of the Krylov loop and the
+                        FP64 outer residual keep their metrics, so only the
+                        preconditioner changes, not the solution.  Used on the
+                        fine level (the autotuner may choose per level for the
+                        smoother).  0 = FP32 metrics everywhere.  Default 1 */
     int coarse_exact; /* 1 = the V-cycle descends to the first coarse level
                         of at most 2048 cells (512 on grids with xi or zeta
                         walls; else to the last level) and solves that level
