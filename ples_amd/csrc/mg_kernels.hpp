@@ -997,6 +997,49 @@ __device__ __forceinline__ void d_resid(Geo G, Met<TM> M, const T* __restrict__ 
     Seg q;
     const bool act = seg_of(G, P, q);
     double acc[4] = {0, 0, 0, 0};
+    if constexpr (MODE == 2 && std::is_same<T, TM>::value && std::is_same<T, TO>::value &&
+                  std::is_same<T, TV>::value) {
+        // the Krylov operator w = A p: the next row (iterate, metrics, r and
+        // the previous direction's w) is loaded while a row is computed, the
+        // eta flux carried between rows
+        using RI = RowIn<T, Met<TM>, TV, X0, true>;  // its rhs field carries r
+        const LineF F{};
+        if (act) {
+            const long sy = G.sy, sz = G.sz;
+            const long cb = G.idx(q.i, q.j0 - 1, q.k);
+            const int m = q.j1 - q.j0 + 1;
+            Win<T> w;
+            win_init(w, p, cb, sy, sz);
+            T Flo = flux_eta_lo(w, (T)M.g[3][cb], (T)M.g[4][cb], (T)M.g[5][cb], inv_pot);
+            RI nx;
+            nx.load(p, rvec, M, F, cb + sy, sy, sz);
+            TV wpn = wp ? wp[cb + sy] : (TV)0;
+            for (int r = 0; r < m; ++r) {
+                const long c = cb + sy * (r + 1);
+                const RI cur = nx;
+                const TV wpc = wpn;
+                if (r + 1 < m) {
+                    nx.load(p, rvec, M, F, c + sy, sy, sz);
+                    if (wp) wpn = wp[c + sy];
+                }
+                T mm[15];
+                const TO wo = (TO)row_apply(w, cur, Flo, inv_pot, mm);
+                out[c] = wo;
+                const double wv = (double)wo, rv = (double)cur.rhs;
+                acc[0] += wv * wv;
+                if (wp) {
+                    acc[1] += wv * (double)wpc;
+                    acc[3] += rv * (double)wpc;
+                }
+                acc[2] += rv * wv;
+                win_shift(w);
+            }
+        }
+        double v[4] = {acc[0], acc[1], acc[2], acc[3]};
+        const bool mx[4] = {false, false, false, false};
+        block_reduce_store<4>(v, partials, nblocks, mx);
+        return;
+    }
     if (act) {
         const long sy = G.sy, sz = G.sz;
         const long cb = G.idx(q.i, q.j0 - 1, q.k);
