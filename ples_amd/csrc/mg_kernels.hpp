@@ -2435,23 +2435,41 @@ __global__ void __launch_bounds__(256) k_sum(Geo G, const double* __restrict__ a
     int i, k;
     const bool act = column_of(G, i, k);
     double acc[2] = {0, 0};
-    if (act)
-        for (int j = 1; j <= G.ny; ++j) {
-            const double v = a[G.idx(i, j, k)];
-            acc[0] += v;
-            acc[1] += v * v;
+    if (act) {
+        // four rows per trip, four partial sums each (the loads of a trip are
+        // independent); fixed order
+        const long c0 = G.idx(i, 1, k), sy = G.sy;
+        double s[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
+        int j = 0;
+#pragma unroll 2
+        for (; j + 4 <= G.ny; j += 4)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double v = a[c0 + sy * (j + u)];
+                s[u] += v;
+                q[u] += v * v;
+            }
+        for (; j < G.ny; ++j) {
+            const double v = a[c0 + sy * j];
+            s[0] += v;
+            q[0] += v * v;
         }
+        acc[0] = (s[0] + s[1]) + (s[2] + s[3]);
+        acc[1] = (q[0] + q[1]) + (q[2] + q[3]);
+    }
     const bool mx[2] = {false, false};
     block_reduce_store<2>(acc, partials, nblocks, mx);
 }
 
 // sums of the partials of k_sum / k_case_rhs into S[S_SUM], S[S_SUM+1]
-__global__ void k_fin_sum(const double* __restrict__ part, int nb, int nv, double* S) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int v = 0; v < nv; ++v) {
-        double s = 0;
-        for (int q = 0; q < nb; ++q) s += part[(long)v * nb + q];
-        S[S_SUM + v] = s;
+// (one block of 256 threads, fixed order)
+__global__ void __launch_bounds__(256) k_fin_sum(const double* __restrict__ part, int nb, double* S) {
+    const bool mx[2] = {false, false};
+    double o[2];
+    fin_reduce<2>(part, nb, mx, o);
+    if (threadIdx.x == 0) {
+        S[S_SUM] = o[0];
+        S[S_SUM + 1] = o[1];
     }
 }
 

@@ -2265,7 +2265,7 @@ static int project_rhs_async(H* h, double bflux, bool on) {
     const Level& L0 = h->lv[0];
     const ColLaunch cl = col_launch(L0.G);
     hipLaunchKernelGGL(k_sum, cl.grid, cl.block, 0, h->s, L0.G, (const double*)h->bvec, h->part, cl.nblocks);
-    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1), 0, h->s, (const double*)h->part, cl.nblocks, 2, h->S);
+    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S);
     if (on) {
         const ColLaunch ce = cell_launch(L0.G);
         hipLaunchKernelGGL(k_sub_mean, ce.grid, ce.block, 0, h->s, L0.G, h->bvec, (const double*)h->S, bflux,
@@ -2694,7 +2694,7 @@ int ples_mg_perturb_fi(ples_mg_handle* h, uint64_t seed, double rel) {
     const Level& L0 = h->lv[0];
     const ColLaunch cl = col_launch(L0.G);
     hipLaunchKernelGGL(k_sum, cl.grid, cl.block, 0, h->s, L0.G, (const double*)h->x, h->part, cl.nblocks);
-    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1), 0, h->s, (const double*)h->part, cl.nblocks, 2, h->S);
+    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S);
     const ColLaunch ce = cell_launch(L0.G);
     hipLaunchKernelGGL(k_perturb, ce.grid, ce.block, 0, h->s, L0.G, h->x, (const double*)h->S, seed, rel,
                        (double)L0.ncell());
@@ -3962,7 +3962,7 @@ int try_admit(SV* sv, SReq& r) {
         const Level& L0 = h->lv[0];
         const ColLaunch cl = col_launch(L0.G);
         hipLaunchKernelGGL(k_sum, cl.grid, cl.block, 0, h->s, L0.G, (const double*)h->x, h->part, cl.nblocks);
-        hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1), 0, h->s, (const double*)h->part, cl.nblocks, 2, h->S);
+        hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(256), 0, h->s, (const double*)h->part, cl.nblocks, h->S);
         const ColLaunch ce = cell_launch(L0.G);
         hipLaunchKernelGGL(k_perturb, ce.grid, ce.block, 0, h->s, L0.G, h->x, (const double*)h->S, r.seed, r.rel,
                            (double)L0.ncell());
