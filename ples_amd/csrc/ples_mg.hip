@@ -101,6 +101,14 @@ struct ples_mg_handle {
     double dt = 1.0;
     double omega_now = 0.8;               // damping of the sweep being launched
     int halo_passes32 = 6;                // wall-ghost passes of the FP32 inner direction
+    // (Z, W) pair of the next FP64 / FP32 refinement step.  Steps alternate the
+    // pair across solves and refinements too: a step leaves the levels' ping-pong
+    // selectors where the other pair's captured graph starts (the dense coarse
+    // solve flips its level once per step), so restarting at pair 0 after an odd
+    // number of steps would drop and recapture both step graphs.  A restart of
+    // the GCR history (no previous direction) reads the other pair only through
+    // terms multiplied by zero, so the pair does not change the results.
+    int qnext[2] = {0, 0};
     void* snap = nullptr;                 // fine-level eta-wall ghost rows (k_eta_ghosts start)
     size_t snap_bytes = 0;
     double cheb_lo = 0.0, cheb_hi = 0.0;  // Chebyshev sweep schedule interval (0 = constant omega)
@@ -1156,7 +1164,7 @@ struct StepRun {
 
 template <class Meas>
 int run_steps(H* h, int kind, double* Sd, double start, double eps, int maxsteps, Meas meas, StepRun& out) {
-    int q = 0, enq = 0, seen = 0;
+    int q = h->qnext[kind], enq = 0, seen = 0;
     double last = start, rho = 0.5;
     const int depth = 2;
     out = StepRun{};
@@ -1191,6 +1199,7 @@ int run_steps(H* h, int kind, double* Sd, double start, double eps, int maxsteps
         last = m;
         out.applied = seen;
     }
+    h->qnext[kind] = q;
     CK(hipStreamSynchronize(h->s));  // steps in flight past convergence are no-ops
     return PLES_MG_OK;
 }
@@ -3785,7 +3794,7 @@ struct SReq {
     bool warm = false;
     double t_submit = 0, t_admit = 0, t_done = 0;
     double bnorm = 1, thr = 0, last = 0, rho = 0.5;
-    int enq = 0, seen = 0, q = 0, cycles = 0, base = 0;
+    int enq = 0, seen = 0, cycles = 0, base = 0;
     hipEvent_t ev = nullptr;       // init / confirmation read-back
     ples_mg_result res{};
 };
@@ -4033,7 +4042,6 @@ int advance(SV* sv, SReq& r) {
         CKS(arm_stop(h, h->S, r.thr, absmode));
         r.base = r.cycles;
         r.enq = r.seen = 0;
-        r.q = 0;
         r.last = measure(rr, rmax);
         r.state = RQ_RUN;
     }
@@ -4068,8 +4076,8 @@ int advance(SV* sv, SReq& r) {
     // keep up to two steps in flight unless the one in flight should converge
     while (r.enq - r.seen < 2 && r.base + r.enq < r.maxcyc) {
         if (r.enq > r.seen && r.last * std::pow(r.rho, (double)(r.enq - r.seen)) < 4.0 * r.eps) break;
-        CKS(enqueue_step(h, r.q));
-        r.q ^= 1;
+        CKS(enqueue_step(h, h->qnext[0]));
+        h->qnext[0] ^= 1;
         double* slot = h->Sh + 2 * S_COUNT + (r.enq % 4) * S_COUNT;
         CK(hipMemcpyAsync(slot, h->S, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, h->s));
         CK(hipEventRecord(h->sev[r.enq % 4], h->s));
