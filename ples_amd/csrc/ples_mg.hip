@@ -1200,7 +1200,8 @@ int run_steps(H* h, int kind, double* Sd, double start, double eps, int maxsteps
         out.applied = seen;
     }
     h->qnext[kind] = q;
-    CK(hipStreamSynchronize(h->s));  // steps in flight past convergence are no-ops
+    // steps still in flight past convergence are no-ops; the callers' next
+    // work (the true residual and its read-back) follows them on the stream
     return PLES_MG_OK;
 }
 
