@@ -404,7 +404,7 @@ long face_set_size(const Geo& G, int set) {
 template <class T, class TM>
 bool eta_ghosts(H* h, const Geo& G, const Met<TM>& m, T* p, const BCs& bc, int passes) {
     if (G.ip != 0 || G.kp != 0 || G.jp != 1 || passes < 1) return false;
-    constexpr int TX = 32, TK = 8;
+    constexpr int TX = sizeof(T) == 4 ? 64 : 32, TK = 8;  // the FP32 tile's halo fits 64 KB of LDS at 64 wide
     const int hw = 2 * passes;
     const size_t lds = (size_t)(TX + 2 * hw) * (TK + 2 * hw) * 4 * sizeof(T);
     const long nrow = 2L * (G.nx + 2) * (G.nz + 2);
