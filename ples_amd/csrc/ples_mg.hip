@@ -10,6 +10,7 @@
 #include <condition_variable>
 #include <functional>
 #include <memory>
+#include <new>
 #include <mutex>
 #include <thread>
 #include <chrono>
@@ -1584,7 +1585,7 @@ struct PinnedBuf {
             pinned = true;
         } else {
             (void)hipGetLastError();
-            p = new double[n];
+            p = new (std::nothrow) double[n];  // nullptr when out of memory: callers check
         }
     }
     ~PinnedBuf() {
@@ -1740,8 +1741,16 @@ int build_coarse_dense(H* h, Level& L) {
     {
         const long nt = std::max<long>(1, std::min<long>({(long)std::thread::hardware_concurrency(), 16, N / 64}));
         std::vector<std::thread> pool;
-        for (long t = 1; t < nt; ++t) pool.emplace_back(columns, N * t / nt, N * (t + 1) / nt);
+        long t = 1;
+        for (; t < nt; ++t) {
+            try {
+                pool.emplace_back(columns, N * t / nt, N * (t + 1) / nt);
+            } catch (...) {  // no more threads: this one takes the remaining ranges
+                break;
+            }
+        }
         columns(0, N / nt);
+        if (t < nt) columns(N * t / nt, N);
         for (std::thread& th : pool) th.join();
     }
     // Gauss-Jordan inverse of the bordered matrix (partial pivoting) on the
@@ -1749,6 +1758,7 @@ int build_coarse_dense(H* h, Level& L) {
     // K_full = E K (ghosted rows x interior columns) from it on the device
     const long nb = N + 1, w = 2 * nb;
     auto KF = std::make_shared<PinnedBuf>((size_t)(ng * N));
+    if (!KF->p) return PLES_MG_ERR_OOM;
     double* kf = nullptr;  // the FP64 map on the device
     {
         std::vector<double> Mh((size_t)(nb * w), 0.0);
@@ -4162,6 +4172,7 @@ int ples_mg_server_add_case(ples_mg_server* sv, uint64_t case_id, const ples_mg_
     const double* src[3] = {g->x, g->y, g->z};
     for (int q = 0; q < 3; ++q) {
         *dst[q] = std::make_shared<PinnedBuf>(nn);
+        if (!(*dst[q])->p) return PLES_MG_ERR_OOM;
         std::memcpy((*dst[q])->p, src[q], nn * sizeof(double));
     }
     c.rhs_seed = rhs_seed;
