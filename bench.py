@@ -550,10 +550,11 @@ def main():
         "data": "synthetic (seeded Gaussian divergence rhs generated on device, projected in every step)",
         "config": {"workload": cfg.text, "cells": cfg.cells, "levels": cfg.nlevel, "tolerance": cfg.tol,
                    "launch_shapes": "tuning table" if tuned else "tuned at create",
-                   "precision": ("mixed: FP32 V-cycles (the fine smoothers read 16-bit copies of the "
-                                 "metrics and line spikes; the V-cycle residual and FP32 operator FP32 "
-                                 "metrics), FP64 coarsest level, FP64 GCR refinement, outer residual and "
-                                 "rhs projection; fp64: FP64 throughout (FP32 line factors in the smoother)")},
+                   "precision": ("mixed: FP32 V-cycles (the smoothers and the fused V-cycle residual read "
+                                 "16-bit copies of the metrics, the smoothers 16-bit line spikes; the FP32 "
+                                 "Krylov operator FP32 metrics), exact coarse solve with FP64 vectors and an "
+                                 "FP32 dense map, FP64 refinement, outer residual and rhs projection; fp64: "
+                                 "FP64 throughout (FP32 line factors in the smoother)")},
         "fp64_solves_per_s": args.steps / fp64["seconds"],
         "speedup_mixed_vs_fp64": fp64["seconds"] / mixed["seconds"],
         "cycles": {"mixed": mixed["cycles"][-1], "fp64": fp64["cycles"][-1]},

@@ -1242,7 +1242,8 @@ int solve_gcr64(H* h, double eps, int maxcyc, int& cycles, double& rr, double& r
 // preconditioned by FP32 V-cycles.  Each refinement takes the FP64 residual
 // r = b - A x (FP64 metrics), solves A32 e = r/|r| in FP32 to a relative
 // tolerance the outer test needs (at most 1e-5: FP32 precision), and adds
-// x += |r| e in FP64.  The coarsest level stays FP64 inside the V-cycle.
+// x += |r| e in FP64.  The exact coarse solve keeps FP64 vectors (its dense
+// map is FP32 in this solver).
 int solve_ir32(H* h, double eps, int maxcyc, int& cycles, double& rr, double& rmax, double bnorm) {
     const bool absmode = h->o.stop_mode == PLES_MG_STOP_ABS_MAX;
     auto measure = [&](double r2, double rm) { return absmode ? rm : std::sqrt(r2) / bnorm; };
