@@ -113,7 +113,8 @@ typedef struct ples_mg_opts {
                         walls; else to the last level) and solves that level
                         exactly when it is that small: a dense map built at create from the level's
                         operator with its wall ghosts eliminated (inverted on
-                        the device), applied as one matrix-vector product;
+                        the device), applied as one matrix-vector product
+                        (the map in FP32 for PLES_MG_MIXED, FP64 vectors);
                         levels below it are built but not visited (default).
                         0 = all nlevel levels, nu_coarse sweeps on the last */
     int sweep_schedule; /* damping of the nu sweeps of a visit: 1 = 1 / (the
