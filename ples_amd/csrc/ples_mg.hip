@@ -119,6 +119,7 @@ struct ples_mg_handle {
     int part_n = 0;
     size_t bytes = 0;
     std::vector<std::pair<void*, size_t>> allocs;
+    int pooled = -1;  // buffers from the stream-ordered pool (1) or hipMalloc (0); -1 before the first
     unsigned timing = 0;  // bit f: time kernel family f
     std::vector<Rec> recs;
     std::vector<KRec>* rec = nullptr;  // set: kernel launches are recorded, not enqueued
@@ -181,10 +182,35 @@ int hip_status(hipError_t e) {
         if (st_ != PLES_MG_OK) return st_; \
     } while (0)
 
+// Device buffers of a handle come from the device's stream-ordered memory
+// pool, allocated and freed in order on the handle's stream: a cold start
+// (about a hundred buffers) and an eviction's frees cost pool bookkeeping
+// instead of a driver allocation and a device-wide synchronising free each.
+// The pool keeps freed memory for the next handles (release threshold at the
+// maximum), so the process holds its peak; a device whose runtime lacks the
+// pool falls back to hipMalloc / hipFree.
+std::mutex g_pool_mu;
+std::vector<int> g_pool_state;  // per device: 0 unknown, 1 pool, 2 plain allocations
+
+bool use_pool(int dev) {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    if ((int)g_pool_state.size() <= dev) g_pool_state.resize((size_t)dev + 1, 0);
+    if (g_pool_state[(size_t)dev] == 0) {
+        hipMemPool_t pool = nullptr;
+        uint64_t thr = UINT64_MAX;
+        const bool ok = hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool &&
+                        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr) == hipSuccess;
+        if (!ok) (void)hipGetLastError();
+        g_pool_state[(size_t)dev] = ok ? 1 : 2;
+    }
+    return g_pool_state[(size_t)dev] == 1;
+}
+
 int dalloc(H* h, void** p, size_t nbytes) {
     *p = nullptr;
     if (nbytes == 0) return PLES_MG_OK;
-    hipError_t e = hipMalloc(p, nbytes);
+    if (h->pooled < 0) h->pooled = use_pool(h->dev) ? 1 : 0;
+    hipError_t e = h->pooled ? hipMallocAsync(p, nbytes, h->s) : hipMalloc(p, nbytes);
     if (e != hipSuccess) {
         *p = nullptr;
         (void)hipGetLastError();
@@ -205,7 +231,8 @@ void dfree(H* h, void* p) {
             h->allocs.erase(it);
             break;
         }
-    (void)hipFree(p);
+    if (h->pooled > 0) (void)hipFreeAsync(p, h->s);
+    else (void)hipFree(p);
 }
 
 Geo make_geo(int nx, int ny, int nz, int ip, int jp, int kp) {
@@ -2073,7 +2100,11 @@ int ples_mg_destroy(ples_mg_handle* h) {
     }
     drop_graphs(h);  // before the buffers they reference
     drop_graphs32(h);
-    for (auto& a : h->allocs) (void)hipFree(a.first);
+    for (auto& a : h->allocs) {
+        if (h->pooled > 0) (void)hipFreeAsync(a.first, h->s);
+        else (void)hipFree(a.first);
+    }
+    if (h->s && h->pooled > 0) (void)hipStreamSynchronize(h->s);  // the frees, before the stream goes
     for (int q = 0; q < 4; ++q)
         if (h->sev[q]) (void)hipEventDestroy(h->sev[q]);
     if (h->Sh) (void)hipHostFree(h->Sh);
