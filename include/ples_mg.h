@@ -490,6 +490,12 @@ size_t ples_mg_estimate_bytes(const ples_mg_grid* grid, const ples_mg_opts* opts
 /* Event timing of every launch of the families whose bit is set in mask
  * (bit f = family f; PLES_MG_TIME_ALL, 0 = off); resets the stats. */
 int ples_mg_set_timing(ples_mg_handle* h, int mask);
+/* With timing on, time one refinement step in every `every` (launched
+ * eagerly with the events); the other steps replay their captured graphs, so
+ * the timed run keeps the graphs' launch rate.  1 = every step (default).
+ * No counterpart in the reference: the per-kernel measurement of bench-style
+ * callers that time kernels inside a throughput run. */
+int ples_mg_set_timing_every(ples_mg_handle* h, int every);
 /* Launch count, summed device milliseconds and algorithmic bytes per kernel
  * family since timing was enabled. */
 int ples_mg_kernel_stats(ples_mg_handle* h, int family, long* launches, double* ms,

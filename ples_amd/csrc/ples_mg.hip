@@ -121,6 +121,8 @@ struct ples_mg_handle {
     std::vector<std::pair<void*, size_t>> allocs;
     int pooled = -1;  // buffers from the stream-ordered pool (1) or hipMalloc (0); -1 before the first
     unsigned timing = 0;  // bit f: time kernel family f
+    int timing_every = 1;  // with timing on: every n-th refinement step is launched eagerly and timed
+    long timing_step = 0;  // refinement steps enqueued since timing was set
     std::vector<Rec> recs;
     std::vector<KRec>* rec = nullptr;  // set: kernel launches are recorded, not enqueued
     // timing of a single-dispatch scope: the next launch records these events
@@ -1014,14 +1016,28 @@ int capture_step(H* h, int q) {
     return PLES_MG_OK;
 }
 
+// whether the next refinement step is launched eagerly with its kernels timed
+// (kernel timing on, and the step one of every timing_every); the other steps
+// replay their graphs, captured with timing suspended
+bool timed_step(H* h) {
+    if (h->timing == 0) return false;
+    const bool t = h->timing_every <= 1 || h->timing_step % h->timing_every == 0;
+    ++h->timing_step;
+    return t;
+}
+
 // enqueue one refinement step: replay of the captured graph of pair q when
-// graphs are on and no kernel timing is requested, else eager launches
+// graphs are on and the step is not a timed one, else eager launches
 int enqueue_step(H* h, int q) {
-    if (!h->o.graphs || h->timing != 0) return step_body(h, q);
+    if (!h->o.graphs || timed_step(h)) return step_body(h, q);
     if (h->gexec[q] && step_state(h) != h->gin[q]) drop_graphs(h);  // state moved: recapture
     if (!h->gexec[q]) {
         const std::vector<int> before = step_state(h);
-        CKS(capture_step(h, q));
+        const unsigned tm = h->timing;
+        h->timing = 0;
+        const int st = capture_step(h, q);
+        h->timing = tm;
+        CKS(st);
         set_step_state(h, before);
     }
     CK(hipGraphLaunch(h->gexec[q], h->s));
@@ -1150,13 +1166,16 @@ void drop_graphs32(H* h) {
 }
 
 int enqueue_step32(H* h, int q) {
-    if (!h->o.graphs || h->timing != 0) return step32_body(h, q);
+    if (!h->o.graphs || timed_step(h)) return step32_body(h, q);
     if (h->gexec32[q] && step_state(h) != h->gin32[q]) drop_graphs32(h);
     if (!h->gexec32[q]) {
         const std::vector<int> before = step_state(h);
         h->gin32[q] = before;
+        const unsigned tm = h->timing;
+        h->timing = 0;  // no timing events inside the capture
         CK(hipStreamBeginCapture(h->s, hipStreamCaptureModeThreadLocal));
         const int st = step32_body(h, q);
+        h->timing = tm;
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(h->s, &g);
         if (st != PLES_MG_OK) {
@@ -2817,6 +2836,14 @@ int ples_mg_set_timing(ples_mg_handle* h, int mask) {
     }
     h->recs.clear();
     h->timing = (unsigned)mask;
+    h->timing_step = 0;
+    return PLES_MG_OK;
+}
+
+int ples_mg_set_timing_every(ples_mg_handle* h, int every) {
+    if (!h || every < 1) return PLES_MG_ERR_ARG;
+    h->timing_every = every;
+    h->timing_step = 0;
     return PLES_MG_OK;
 }
 

@@ -176,6 +176,7 @@ def lib():
         L.ples_mg_estimate_bytes.argtypes = [C.POINTER(_Grid), C.POINTER(_Opts)]
         L.ples_mg_estimate_bytes.restype = C.c_size_t
         L.ples_mg_set_timing.argtypes = [C.c_void_p, C.c_int]
+        L.ples_mg_set_timing_every.argtypes = [C.c_void_p, C.c_int]
         L.ples_mg_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long),
                                            C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _lib = L
@@ -615,9 +616,10 @@ class Solver:
         _check(lib().ples_mg_tiles(self._h, level, *[C.byref(a) for a in v]), "ples_mg_tiles")
         return tuple(a.value for a in v)
 
-    def set_timing(self, families=None):
+    def set_timing(self, families=None, every=1):
         """Time the given kernel families (names or ids); None/False = off,
-        True = all."""
+        True = all.  every: time one refinement step in `every` (the others
+        replay their graphs)."""
         if families is True:
             mask = 0xFF
         elif not families:
@@ -627,6 +629,7 @@ class Solver:
             for f in families:
                 mask |= 1 << (KERNEL_FAMILIES[f] if isinstance(f, str) else int(f))
         _check(lib().ples_mg_set_timing(self._h, mask), "ples_mg_set_timing")
+        _check(lib().ples_mg_set_timing_every(self._h, int(every)), "ples_mg_set_timing_every")
 
     def kernel_stats(self, family) -> tuple[int, float, float]:
         fam = KERNEL_FAMILIES[family] if isinstance(family, str) else int(family)

@@ -395,6 +395,29 @@ def test_diver_and_gradie_match_oracle():
         assert np.abs(outs[3 + q] - ref).max() < 1e-12 * np.abs(ref).max()
 
 
+def test_kernel_timing_leaves_results_unchanged():
+    """Kernel timing, on every refinement step or sampled (one step in four
+    eager and timed, the rest replayed from their graphs), gives bitwise the
+    untimed solve; sampling records about a quarter of the launches."""
+    g = cases.grid(64, 32, 64, gamma=2.0, amp=0.2)
+    out = {}
+    with _solver(g, 4, "mixed") as s:
+        s.case_rhs(1003)
+        for every in (None, 1, 4):
+            s.set_timing(["smooth_fine"] if every else None, every=every or 1)
+            s.zero_fi()
+            r = s.solve_resident(1e-10, 200)
+            assert r.ok, (every, r)
+            n = s.kernel_stats("smooth_fine")[0] if every else 0
+            out[every] = (r.cycles, s.get_fi(), n)
+        s.set_timing(None)
+    for every in (1, 4):
+        assert out[every][0] == out[None][0]
+        assert np.array_equal(out[every][1], out[None][1]), every
+    n1, n4 = out[1][2], out[4][2]
+    assert n1 > 0 and 0 < n4 < n1 and abs(4 * n4 - n1) <= 4 * 6 + 6, (n1, n4)
+
+
 def test_warm_start_resident_and_determinism():
     g = cases.grid(64, 32, 64, gamma=2.0, amp=0.2)
     with _solver(g, 4, "mixed") as s:
